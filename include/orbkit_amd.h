@@ -1,0 +1,129 @@
+/*
+ * C ABI of liborbkit_amd.so: orbkit's grid-density path on MI355X (gfx950).
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a nonzero status otherwise;
+ *     okmi_last_error() then gives a message (thread-local).
+ *   - Pointers named d_* are device pointers (HIP global memory); all other
+ *     array pointers are host pointers.  Arrays are C-contiguous.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     Work is enqueued on it; functions do not synchronise unless stated.
+ *   - Grid points: a vector grid (x[i], y[i], z[i]) or a regular grid given
+ *     by its three axes, flattened with x slowest and z fastest
+ *     (orbkit/cy_grid.pyx:12-30).  [p0, p0+np) selects a contiguous range
+ *     of flattened points, which is how orbkit's slices are expressed
+ *     (orbkit/core.py:504-514).
+ *   - Derivative codes: 0 none, 1 x, 2 y, 3 z, 4 xx, 5 yy, 6 zz, 7 xy, 8 xz,
+ *     9 yz (orbkit/tools.py:225-239; codes 7-9 use orbkit's own formula,
+ *     orbkit/c_support.c:121-168).
+ */
+#ifndef ORBKIT_AMD_H
+#define ORBKIT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OKMI_F64 = 0, OKMI_F32 = 1 };
+
+/* arithmetic of the MO contraction in okmi_rho_fused */
+enum { OKMI_PREC_F64 = 0, OKMI_PREC_F32 = 1, OKMI_PREC_BF16X3 = 2, OKMI_PREC_BF16X6 = 3 };
+
+typedef struct okmi_grid {
+    const double* d_x;
+    const double* d_y;
+    const double* d_z;
+    int64_t nx, ny, nz;   /* regular: axis lengths; vector: nx = number of points, ny = nz = 0 */
+    int regular;
+    int pad;
+} okmi_grid;
+
+typedef struct OkmiBasis okmi_basis;
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* okmi_last_error(void);
+
+/*
+ * Builds the device basis table.  The first arrays are exactly those orbkit's
+ * AO kernel receives from AOClass (orbkit/cy_core.pyx:49-78, arguments of
+ * cy_core.aocreator): Cartesian exponents lxlylz[nao_cart][3], components
+ * per contraction, primitive (exponent, coefficient) pairs prim_coeffs[nprim][2],
+ * primitives per contraction, atom of each contraction, atom positions
+ * geo[natoms][3] (bohr) and the pre-normalised flag (orbkit/c_support.c:182).
+ *   cart_scale   NULL or one factor per Cartesian row (orbkit's ao_spec[0]['N'],
+ *                orbkit/core.py:98-100)
+ *   nout_per_cont, transform
+ *                NULL for Cartesian output rows; otherwise a dense map per
+ *                contraction, nout x ncart (row-major, columns in that
+ *                contraction's lxlylz order), concatenated: the
+ *                Cartesian-to-spherical combinations of orbkit/core.py:135-176
+ *                with orbkit/tools.py:148-220's coefficients.
+ * Replaces: cy_core.aocreator's argument flattening (orbkit/cy_core.pyx:49-78)
+ * and the per-call norm table of c_lcreator (orbkit/c_grid-based.c:27-36).
+ */
+int okmi_basis_create(int ncont, const int* ncart_per_cont, const int* lxlylz,
+                      const int* nprim_per_cont, const double* prim_coeffs,
+                      int natoms, const double* geo, const int* cont_atom,
+                      int is_normalized, const double* cart_scale,
+                      const int* nout_per_cont, const double* transform,
+                      okmi_basis** out);
+int okmi_basis_destroy(okmi_basis* b);
+/* number of output AO rows (Cartesian or spherical) */
+int okmi_basis_nao(const okmi_basis* b);
+
+/*
+ * AO values (or derivatives) on points [p0, p0+np):
+ *   d_out[j][row][i] = derivative drv[j] of AO row `row` at point p0+i,
+ *   row stride ld (>= np), set stride nao*ld, out_dtype OKMI_F64/OKMI_F32.
+ * Replaces c_lcreator (orbkit/c_grid-based.h:3-5) over all contractions as
+ * done by cy_core.aocreator (orbkit/cy_core.pyx:49-78), plus the row
+ * renormalisation and cartesian2spherical of ao_creator (orbkit/core.py:98-102).
+ */
+int okmi_ao_eval(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np,
+                 const int* drv, int ndrv, int out_dtype, void* d_out, int64_t ld, void* stream);
+
+/*
+ * MO[i][j] = sum_k C[i][k] AO[k][j] in FP64 on the device (MFMA).
+ * d_ao[nao][ld_ao], d_c[nmo][nao], d_mo[nmo][ld_mo]; npts columns.
+ * Replaces cy_core.mocreator (orbkit/cy_core.pyx:80-101).
+ */
+int okmi_mocreator(const double* d_ao, int nao, int64_t npts, int64_t ld_ao,
+                   const double* d_c, int nmo, double* d_mo, int64_t ld_mo, void* stream);
+
+/*
+ * Workspace bytes okmi_rho / okmi_mo_eval need to process `chunk` points per
+ * pass with `nsets` AO derivative sets (FP64 AO staging).
+ */
+size_t okmi_workspace_bytes(const okmi_basis* b, int nsets, int64_t chunk);
+
+/*
+ * MOs (and their derivatives) on points [p0, p0+np), FP64:
+ *   d_mo[j][i][p] = derivative drv[j] of MO i at point p0+p, row stride ld_mo.
+ * Replaces slice_rho's calc_mo branch (orbkit/core.py:250-263).
+ */
+int okmi_mo_eval(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np,
+                 const int* drv, int ndrv, const double* d_c, int nmo,
+                 double* d_mo, int64_t ld_mo, void* d_ws, size_t ws_bytes, void* stream);
+
+/*
+ * Density and derivatives on points [p0, p0+np), FP64 throughout:
+ *   d_rho[p]        = sum_i occ_i phi_i(p)^2
+ *   d_delta[r][p]   for each label r of drv_out (codes 1-9):
+ *                     first derivatives 2 sum_i occ_i phi_i d_phi_i,
+ *                     second ones add 2 sum_i occ_i d_a phi_i d_b phi_i
+ *                     (orbkit/core.py:280-304); row stride ld_delta
+ *   d_mo_norm[i]   += sum_p phi_i(p)^2 when non-NULL (orbkit/core.py:266-274)
+ * Replaces slice_rho's density branch (orbkit/core.py:179-308).
+ */
+int okmi_rho(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np,
+             const double* d_c, const double* d_occ, int nmo,
+             const int* drv_out, int nout, double* d_rho, double* d_delta, int64_t ld_delta,
+             double* d_mo_norm, void* d_ws, size_t ws_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,455 @@
+'''Drop-in for orbkit's grid path (orbkit/core.py:38-839) running on MI355X.
+
+Same names, signatures, defaults, return shapes and exceptions as orbkit's
+``ao_creator``, ``mo_creator``, ``cartesian2spherical``, ``slice_rho``,
+``initializer``, ``rho_compute`` and ``rho_compute_no_slice``.  The numbers
+come from HIP kernels reached through liborbkit_amd.so; inputs are host
+numpy arrays and so are the results (float64, C order, owned by the caller).
+
+Differences from orbkit, all on the execution side:
+  * ``numproc`` is accepted and ignored (one GPU does the slices); a value
+    <= 0 still selects ``rho_compute_no_slice`` as in orbkit.
+  * ``slice_length`` keeps its meaning for the printed slice plan; the device
+    processes points in chunks sized to its memory, which gives the same
+    values because every point is independent.
+  * integer codes in ``drv`` are accepted in the density branch too (orbkit
+    raises TypeError there because it calls ``len()`` on them).
+  * extra keyword ``precision`` of ``rho_compute``: 'f64' (default) or one of
+    the reduced-precision fused paths ('f32', 'bf16x3', 'bf16x6').
+'''
+import ctypes
+
+import numpy
+
+from . import grid, _native
+from .basis import device_basis, spherical_matrix
+from .display import display
+from .qcinfo import QCinfo
+from .tools import validate_drv, require, convert, zeros, reshape
+
+#: device bytes for one staged chunk of AO values (FP64 path)
+WORKSPACE_BYTES = 1 << 30
+#: device bytes for one chunk of returned values
+OUTPUT_CHUNK_BYTES = 1 << 30
+
+
+# --------------------------------------------------------------------------
+# device plumbing
+def _torch():
+  import torch
+  if not torch.cuda.is_available():
+    raise RuntimeError('orbkit_amd needs a HIP device (torch.cuda.is_available() is False); '
+                       'it has no CPU compute path')
+  _native.lib()
+  return torch
+
+
+def _stream():
+  torch = _torch()
+  return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class DeviceGrid:
+  '''Points on the device: a regular grid (axes only) or a vector grid.'''
+
+  def __init__(self, x, y, z, regular):
+    torch = _torch()
+    dev = torch.device('cuda', torch.cuda.current_device())
+    t = lambda a: torch.as_tensor(numpy.ascontiguousarray(a, dtype=numpy.float64).reshape(-1), device=dev)
+    self.x, self.y, self.z = t(x), t(y), t(z)
+    self.regular = bool(regular)
+    if self.regular:
+      self.shape = (len(self.x), len(self.y), len(self.z))
+      self.npts = self.shape[0] * self.shape[1] * self.shape[2]
+    else:
+      if not (len(self.x) == len(self.y) == len(self.z)):
+        raise ValueError('Dimensions of x-, y-, and z- coordinate differ!')
+      self.shape = (len(self.x),)
+      self.npts = len(self.x)
+    g = _native.OkmiGrid()
+    g.d_x, g.d_y, g.d_z = self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr()
+    if self.regular:
+      g.nx, g.ny, g.nz, g.regular = self.shape[0], self.shape[1], self.shape[2], 1
+    else:
+      g.nx, g.ny, g.nz, g.regular = self.npts, 0, 0, 0
+    self.desc = g
+
+
+def _codes(drv_list):
+  return numpy.array([validate_drv(d) for d in drv_list], dtype=numpy.intc)
+
+
+def eval_aos(basis, dgrid, codes, p0=0, np_=None, out=None):
+  '''AO values for derivative codes on points [p0, p0+np): host array
+  (ncodes, nao, np).'''
+  torch = _torch()
+  L = _native.lib()
+  np_ = dgrid.npts - p0 if np_ is None else np_
+  codes = numpy.ascontiguousarray(codes, dtype=numpy.intc)
+  nd = len(codes)
+  if out is None:
+    out = numpy.empty((nd, basis.nao, np_))
+  per = max(1, nd * basis.nao * 8)
+  chunk = max(64, min(np_, OUTPUT_CHUNK_BYTES // per))
+  buf = torch.empty((nd, basis.nao, min(chunk, max(np_, 1))), dtype=torch.float64,
+                    device=dgrid.x.device)
+  for c0 in range(0, np_, chunk):
+    n = min(chunk, np_ - c0)
+    _native.check(L.okmi_ao_eval(basis.handle, ctypes.byref(dgrid.desc), p0 + c0, n,
+                                 codes.ctypes.data_as(_native.P_int), nd, _native.OKMI_F64,
+                                 ctypes.c_void_p(buf.data_ptr()), buf.shape[2], _stream()),
+                  'okmi_ao_eval')
+    out[:, :, c0:c0 + n] = buf[:, :, :n].cpu().numpy()
+  return out
+
+
+def eval_mos(basis, dgrid, codes, coeffs, p0=0, np_=None):
+  '''MO values (ncodes, nmo, np) for derivative codes.'''
+  torch = _torch()
+  L = _native.lib()
+  np_ = dgrid.npts - p0 if np_ is None else np_
+  codes = numpy.ascontiguousarray(codes, dtype=numpy.intc)
+  nd = len(codes)
+  C = numpy.ascontiguousarray(coeffs, dtype=numpy.float64)
+  nmo = C.shape[0]
+  if C.ndim != 2 or C.shape[1] != basis.nao:
+    raise ValueError('MO coefficients have shape %s but the basis has %d AOs' % (C.shape, basis.nao))
+  dev = dgrid.x.device
+  dC = torch.as_tensor(C, device=dev)
+  out = numpy.empty((nd, nmo, np_))
+  chunk = max(64, min(np_, OUTPUT_CHUNK_BYTES // max(1, nd * nmo * 8)))
+  buf = torch.empty((nd, nmo, min(chunk, max(np_, 1))), dtype=torch.float64, device=dev)
+  ws_chunk = max(64, WORKSPACE_BYTES // max(1, nd * basis.nao * 8))
+  ws = torch.empty(L.okmi_workspace_bytes(basis.handle, nd, min(ws_chunk, max(chunk, 64))) // 8 + 1,
+                   dtype=torch.float64, device=dev)
+  for c0 in range(0, np_, chunk):
+    n = min(chunk, np_ - c0)
+    _native.check(L.okmi_mo_eval(basis.handle, ctypes.byref(dgrid.desc), p0 + c0, n,
+                                 codes.ctypes.data_as(_native.P_int), nd, ctypes.c_void_p(dC.data_ptr()),
+                                 nmo, ctypes.c_void_p(buf.data_ptr()), buf.shape[2],
+                                 ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream()),
+                  'okmi_mo_eval')
+    out[:, :, c0:c0 + n] = buf[:, :, :n].cpu().numpy()
+  return out
+
+
+def eval_rho(basis, dgrid, coeffs, occ, codes, p0=0, np_=None, mo_norm=False):
+  '''Density (np,), derivative outputs (ncodes, np) and optionally the MO
+  norms sum_p phi^2 (nmo,), all FP64.'''
+  torch = _torch()
+  L = _native.lib()
+  np_ = dgrid.npts - p0 if np_ is None else np_
+  codes = numpy.ascontiguousarray(codes, dtype=numpy.intc)
+  nd = len(codes)
+  C = numpy.ascontiguousarray(coeffs, dtype=numpy.float64)
+  nmo = C.shape[0]
+  if C.ndim != 2 or C.shape[1] != basis.nao:
+    raise ValueError('MO coefficients have shape %s but the basis has %d AOs' % (C.shape, basis.nao))
+  dev = dgrid.x.device
+  dC = torch.as_tensor(C, device=dev)
+  docc = torch.as_tensor(numpy.ascontiguousarray(occ, dtype=numpy.float64), device=dev)
+  rho = torch.empty(max(np_, 1), dtype=torch.float64, device=dev)
+  delta = torch.empty((max(nd, 1), max(np_, 1)), dtype=torch.float64, device=dev)
+  nrm = torch.zeros(nmo, dtype=torch.float64, device=dev) if mo_norm else None
+  nsets = 1 + 3 + nd
+  ws_chunk = max(64, min(np_, WORKSPACE_BYTES // max(1, nsets * basis.nao * 8)))
+  ws = torch.empty(L.okmi_workspace_bytes(basis.handle, nsets, ws_chunk) // 8 + 1,
+                   dtype=torch.float64, device=dev)
+  _native.check(L.okmi_rho(basis.handle, ctypes.byref(dgrid.desc), p0, np_, ctypes.c_void_p(dC.data_ptr()),
+                           ctypes.c_void_p(docc.data_ptr()), nmo, codes.ctypes.data_as(_native.P_int), nd,
+                           ctypes.c_void_p(rho.data_ptr()), ctypes.c_void_p(delta.data_ptr()),
+                           delta.shape[1], ctypes.c_void_p(nrm.data_ptr()) if mo_norm else None,
+                           ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream()), 'okmi_rho')
+  r = rho[:np_].cpu().numpy()
+  d = delta[:nd, :np_].cpu().numpy()
+  return r, d, (nrm.cpu().numpy() if mo_norm else None)
+
+
+def gemm(A, B):
+  '''A (m x k) @ B (k x n) in FP64 on the device (the MO kernel).'''
+  torch = _torch()
+  L = _native.lib()
+  A = numpy.ascontiguousarray(A, dtype=numpy.float64)
+  B = numpy.ascontiguousarray(B, dtype=numpy.float64)
+  m, k = A.shape
+  if B.shape[0] != k:
+    raise ValueError('shape mismatch: (%d, %d) @ %s' % (m, k, B.shape))
+  n = B.shape[1]
+  out = numpy.empty((m, n))
+  if n == 0 or m == 0:
+    return out
+  dev = torch.device('cuda', torch.cuda.current_device())
+  dA = torch.as_tensor(A, device=dev)
+  chunk = max(64, min(n, OUTPUT_CHUNK_BYTES // (8 * max(m, k))))
+  dB = torch.empty((k, chunk), dtype=torch.float64, device=dev)
+  dO = torch.empty((m, chunk), dtype=torch.float64, device=dev)
+  for c0 in range(0, n, chunk):
+    w = min(chunk, n - c0)
+    dB[:, :w].copy_(torch.from_numpy(B[:, c0:c0 + w]))
+    _native.check(L.okmi_mocreator(ctypes.c_void_p(dB.data_ptr()), k, w, chunk,
+                                   ctypes.c_void_p(dA.data_ptr()), m, ctypes.c_void_p(dO.data_ptr()),
+                                   chunk, _stream()), 'okmi_mocreator')
+    out[:, c0:c0 + w] = dO[:, :w].cpu().numpy()
+  return out
+
+
+def _specs(qc):
+  if isinstance(qc, dict):
+    return qc['geo_spec'], qc['ao_spec'], qc['mo_spec']
+  return qc.geo_spec, qc.ao_spec, qc.mo_spec
+
+
+# --------------------------------------------------------------------------
+# orbkit's API
+def ao_creator(geo_spec, ao_spec, drv=None, x=None, y=None, z=None, is_vector=None):
+  '''All contracted AOs (or a derivative) on the grid: shape (NAO,)+N
+  (orbkit/core.py:38-105).'''
+  if all(v is None for v in [x, y, z, is_vector]) and not grid.is_initialized:
+    display('\nSetting up the grid...')
+    grid.grid_init(is_vector=True)
+    display(grid.get_grid())
+  if x is None: x = grid.x
+  if y is None: y = grid.y
+  if z is None: z = grid.z
+  if is_vector is None: is_vector = grid.is_vector
+  was_vector = is_vector
+  if not is_vector:
+    N = (len(x), len(y), len(z))
+  else:
+    if len(x) != len(y) or len(x) != len(z):
+      raise ValueError('Dimensions of x-, y-, and z- coordinate differ!')
+    N = (len(x),)
+  code = validate_drv(drv)
+  geo_spec = require(geo_spec, dtype='f')
+  basis = device_basis(geo_spec, ao_spec)
+  dg = DeviceGrid(x, y, z, regular=not was_vector)
+  ao_list = eval_aos(basis, dg, [code])[0]
+  if not was_vector:
+    return ao_list.reshape((len(ao_list),) + N, order='C')
+  return ao_list
+
+
+def mo_creator(ao_list, mo_spec):
+  '''MOs from AOs: shape (NMO,)+N (orbkit/core.py:107-132).'''
+  ao_list = require(ao_list, dtype='f')
+  shape = ao_list.shape
+  mo_coeffs = mo_spec.get_coeffs()
+  mo = gemm(mo_coeffs, ao_list.reshape((shape[0], -1)))
+  return mo.reshape(((len(mo_coeffs),) + shape[1:]), order='C')
+
+
+def cartesian2spherical(ao_list, ao_spec):
+  '''Cartesian AO rows to real spherical rows (orbkit/core.py:135-176).'''
+  ao_list = require(ao_list, dtype='f')
+  shape = ao_list.shape
+  T = spherical_matrix(ao_spec)
+  out = gemm(T, ao_list.reshape((shape[0], -1)))
+  return out.reshape((T.shape[0],) + shape[1:])
+
+
+Spec = None
+
+
+def initializer(global_args):
+  global Spec
+  Spec = global_args
+
+
+def slice_rho(xx):
+  '''One slice [xx[0], xx[1]) of the vector grid in ``grid`` (orbkit/core.py:179-308).'''
+  try:
+    geo_spec, ao_spec, mo_spec = _specs(Spec['qc'])
+    drv = Spec['derivative']
+    calc_mo = Spec['calc_mo']
+    x = grid.x[xx[0]:xx[1]]
+    y = grid.y[xx[0]:xx[1]]
+    z = grid.z[xx[0]:xx[1]]
+    basis = device_basis(require(geo_spec, dtype='f'), ao_spec)
+    dg = DeviceGrid(x, y, z, regular=False)
+    codes = [0] if drv is None else _codes(drv)
+    if calc_mo:
+      if Spec['calc_ao']:
+        vals = eval_aos(basis, dg, codes)
+      else:
+        vals = eval_mos(basis, dg, codes, mo_spec.get_coeffs())
+      return vals if drv is not None else vals[0]
+    occ = mo_spec.get_occ()
+    rho, delta, mo_norm = eval_rho(basis, dg, mo_spec.get_coeffs(), occ,
+                                   [] if drv is None else codes, mo_norm=True)
+    if drv is None:
+      return rho, mo_norm
+    return rho, mo_norm, delta
+  except KeyboardInterrupt:
+    return 0
+
+
+def _drv_list(drv, laplacian):
+  if laplacian:
+    if not (drv is None or drv == ['xx', 'yy', 'zz'] or drv == ['x2', 'y2', 'z2']):
+      display('Note: You have set the option `laplacian` and specified values\n' +
+              'for `drv`. Both options are not compatible.\n' +
+              'The option `drv` has been changed to `drv=["xx","yy","zz"]`.')
+    drv = ['xx', 'yy', 'zz']
+  if drv is not None:
+    try:
+      drv = list(drv)
+    except TypeError:
+      drv = [drv]
+  return drv
+
+
+def rho_compute(qc, calc_ao=False, calc_mo=False, drv=None, laplacian=False,
+                numproc=1, slice_length=1e4, vector=None, save_hdf5=False, **kwargs):
+  '''Density, MOs or AOs (and derivatives) on the grid module's grid
+  (orbkit/core.py:314-605).  Returns, as orbkit does:
+    calc_mo/calc_ao: mo_list (NMO,)+N, or (NDRV, NMO)+N with drv
+    density:         rho N; (rho, delta_rho) with drv;
+                     (rho, delta_rho, laplacian_rho) with laplacian
+  '''
+  precision = kwargs.pop('precision', 'f64')
+  if calc_ao and calc_mo:
+    raise ValueError('Choose either calc_ao=True or calc_mo=True')
+  elif calc_ao:
+    calc_mo = True
+  slice_length = slice_length if not vector else vector
+  if numproc <= 0:
+    return rho_compute_no_slice(qc, calc_ao=calc_ao, calc_mo=calc_mo, drv=drv,
+                                laplacian=laplacian, **kwargs)
+  drv = _drv_list(drv, laplacian)
+  is_drv = drv is not None
+  if isinstance(qc, dict):
+    qc = QCinfo(qc)
+  geo_spec, ao_spec, mo_spec = qc.geo_spec, qc.ao_spec, qc.mo_spec
+  codes = None if drv is None else _codes(drv)
+
+  if not grid.is_initialized:
+    display('\nSetting up the grid...')
+    grid.grid_init()
+    display(grid.get_grid())
+  was_vector = grid.is_vector
+  N = (len(grid.x),) if was_vector else (len(grid.x), len(grid.y), len(grid.z))
+  npts = int(numpy.prod(N))
+  if slice_length <= 0:
+    slice_length = numpy.ceil(npts / float(max(numproc, 1))) + 1
+  sNum = int(numpy.floor(npts / slice_length) + 1)
+  if slice_length >= npts:
+    slice_length, sNum = npts, 1
+  display('\nStarting the calculation of the %s...' % ('molecular orbitals' if calc_mo else 'density'))
+  display('The grid has been separated into %d slices each having %.2e grid points.' % (sNum, slice_length))
+
+  if save_hdf5:
+    import h5py  # noqa: F401  (as orbkit: the HDF5 sink needs h5py)
+
+  basis = device_basis(require(geo_spec, dtype='f'), ao_spec)
+  dg = DeviceGrid(grid.x, grid.y, grid.z, regular=not was_vector)
+
+  if calc_mo:
+    c = [0] if codes is None else codes
+    vals = eval_aos(basis, dg, c) if calc_ao else eval_mos(basis, dg, c, mo_spec.get_coeffs())
+    mo_list = vals if is_drv else vals[0]
+    if not was_vector and drv is None:
+      display('\nNorm of the MOs:')
+      for i in range(mo_list.shape[0]):
+        display('\t%.6f\t%s %d' % (numpy.sum(numpy.square(mo_list[i])) * grid.d3r,
+                                   'AO' if calc_ao else 'MO', i))
+    shape = (mo_list.shape[0],) if drv is None else (len(drv), mo_list.shape[1])
+    return mo_list.reshape(shape + N)
+
+  if precision != 'f64':
+    from .fused import rho_fused
+    rho, delta = rho_fused(basis, dg, mo_spec.get_coeffs(), mo_spec.get_occ(),
+                           [] if codes is None else codes, precision=precision)
+    mo_norm = None
+  else:
+    rho, delta, mo_norm = eval_rho(basis, dg, mo_spec.get_coeffs(), mo_spec.get_occ(),
+                                   [] if codes is None else codes, mo_norm=not display_quiet())
+  if not was_vector and drv is None and mo_norm is not None:
+    display('\nNorm of the MOs:')
+    for i, v in enumerate(mo_norm):
+      display('\t%.6f\tMO %d' % (v * grid.d3r, i))
+  if not was_vector:
+    display('We have ' + str(numpy.sum(rho) * grid.d3r) + ' electrons.')
+  rho = rho.reshape(N)
+  if not is_drv:
+    return rho
+  delta_rho = delta.reshape((len(drv),) + N)
+  if laplacian:
+    return rho, delta_rho, delta_rho.sum(axis=0)
+  return rho, delta_rho
+
+
+def display_quiet():
+  from . import display as _d
+  return _d.quiet
+
+
+def rho_compute_no_slice(qc, calc_ao=False, calc_mo=False, drv=None, laplacian=False,
+                         return_components=False, x=None, y=None, z=None, is_vector=None, **kwargs):
+  '''Same quantities on the whole grid at once, optionally with the AO and MO
+  arrays (orbkit/core.py:607-839).'''
+  if calc_ao and calc_mo:
+    raise ValueError('calc_ao and calc_mo are mutually exclusive arguments.' +
+                     'Use calc_mo and return_components instead!')
+  if all(v is None for v in [x, y, z, is_vector]) and not grid.is_initialized:
+    display('\nSetting up the grid...')
+    grid.grid_init()
+    display(grid.get_grid())
+  if x is None: x = grid.x
+  if y is None: y = grid.y
+  if z is None: z = grid.z
+  if is_vector is None: is_vector = grid.is_vector
+  was_vector = is_vector
+  if not is_vector:
+    N = (len(x), len(y), len(z))
+    d3r = 1.0
+    for a in (x, y, z):
+      if len(a) > 1:
+        d3r *= (a[1] - a[0])
+  else:
+    if len(x) != len(y) or len(x) != len(z):
+      raise ValueError('Dimensions of x-, y-, and z- coordinate differ!')
+    N = (len(x),)
+  if laplacian:
+    drv = _drv_list(None, True)
+  geo_spec, ao_spec, mo_spec = _specs(qc)
+  basis = device_basis(require(geo_spec, dtype='f'), ao_spec)
+  dg = DeviceGrid(x, y, z, regular=not was_vector)
+  C = mo_spec.get_coeffs()
+
+  if drv is not None:
+    drv = _drv_list(drv, False)
+    codes = _codes(drv)
+    delta_ao = eval_aos(basis, dg, codes)
+    delta_ao_list = convert(delta_ao, was_vector, N)
+    if calc_ao:
+      return delta_ao_list
+    delta_mo = gemm_sets(C, delta_ao)
+    delta_mo_list = convert(delta_mo, was_vector, N)
+    if calc_mo:
+      return (delta_ao_list, delta_mo_list) if return_components else delta_mo_list
+
+  ao = eval_aos(basis, dg, [0])[0]
+  ao_list = convert(ao, was_vector, N)
+  if calc_ao:
+    return ao_list
+  mo = gemm(C, ao)
+  mo_list = convert(mo, was_vector, N)
+  if calc_mo:
+    return (ao_list, mo_list) if return_components else mo_list
+  occ = mo_spec.get_occ()
+  codes = [] if drv is None else _codes(drv)
+  rho, delta, _ = eval_rho(basis, dg, C, occ, codes)
+  rho = rho.reshape(N)
+  if not was_vector:
+    display('We have ' + str(numpy.sum(rho) * d3r) + ' electrons.')
+  if drv is None:
+    return (ao_list, mo_list, rho) if return_components else rho
+  delta_rho = delta.reshape((len(drv),) + N)
+  delta = (delta_rho, delta_rho.sum(axis=0)) if laplacian else (delta_rho,)
+  return ((ao_list, mo_list, rho, delta_ao_list, delta_mo_list) + delta
+          if return_components else (rho,) + delta)
+
+
+def gemm_sets(C, sets):
+  '''C @ sets[j] for each derivative set j.'''
+  return numpy.array([gemm(C, s) for s in sets])

@@ -1,0 +1,372 @@
+// extern "C" entry points of liborbkit_amd.so (declared in include/orbkit_amd.h).
+#include "okmi_internal.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+thread_local std::string g_err;
+
+int dfact(int n)
+{
+    int r = 1;
+    for (; n > 0; n -= 2) r *= n;
+    return r;
+}
+
+int canon_index(int l, int lx, int ly, int lz)
+{
+    for (int c = 0; c < ncart(l); ++c)
+        if (cart_exp(l, c, 0) == lx && cart_exp(l, c, 1) == ly && cart_exp(l, c, 2) == lz) return c;
+    return -1;
+}
+
+template <typename T>
+int dev_upload(T** dst, const T* src, size_t n, const char* what)
+{
+    *dst = nullptr;
+    if (n == 0) return 0;
+    hipError_t e = hipMalloc((void**)dst, n * sizeof(T));
+    if (e != hipSuccess) return okmi_hip_fail(e, what);
+    e = hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return okmi_hip_fail(e, what);
+    return 0;
+}
+
+GridDesc to_desc(const okmi_grid* g)
+{
+    GridDesc d{};
+    d.x = g->d_x; d.y = g->d_y; d.z = g->d_z;
+    d.nx = g->nx; d.ny = g->ny; d.nz = g->nz;
+    d.regular = g->regular;
+    return d;
+}
+
+int check_grid(const okmi_grid* g, int64_t p0, int64_t np)
+{
+    if (!g || !g->d_x || !g->d_y || !g->d_z) {
+        okmi_set_error("grid: null coordinate pointer");
+        return 1;
+    }
+    int64_t total = g->regular ? g->nx * g->ny * g->nz : g->nx;
+    if (g->nx < 0 || (g->regular && (g->ny <= 0 || g->nz <= 0))) {
+        okmi_set_error("grid: invalid axis lengths");
+        return 1;
+    }
+    if (p0 < 0 || np < 0 || p0 + np > total) {
+        okmi_set_error("grid: point range [%lld, %lld) outside the %lld grid points", (long long)p0,
+                       (long long)(p0 + np), (long long)total);
+        return 1;
+    }
+    return 0;
+}
+
+int check_drv(const int* drv, int ndrv)
+{
+    if (ndrv < 1 || ndrv > OKMI_MAX_SETS || !drv) {
+        okmi_set_error("drv: %d derivative sets requested (1..%d supported)", ndrv, OKMI_MAX_SETS);
+        return 1;
+    }
+    for (int j = 0; j < ndrv; ++j)
+        if (drv[j] < 0 || drv[j] > 9) {
+            okmi_set_error("The selection `drv=%d` is not valid!", drv[j]);
+            return 1;
+        }
+    return 0;
+}
+}  // namespace
+
+void okmi_set_error(const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+int okmi_hip_fail(hipError_t e, const char* what)
+{
+    okmi_set_error("%s: %s", what, hipGetErrorString(e));
+    return 2;
+}
+
+int okmi_check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return okmi_hip_fail(e, what);
+    return 0;
+}
+
+extern "C" const char* okmi_last_error(void) { return g_err.c_str(); }
+
+extern "C" int okmi_basis_create(int ncont, const int* ncart_per_cont, const int* lxlylz, const int* nprim_per_cont,
+                                 const double* prim_coeffs, int natoms, const double* geo, const int* cont_atom,
+                                 int is_normalized, const double* cart_scale, const int* nout_per_cont,
+                                 const double* transform, okmi_basis** out)
+{
+    g_err.clear();
+    if (!out || ncont <= 0 || !ncart_per_cont || !lxlylz || !nprim_per_cont || !prim_coeffs || !geo || !cont_atom) {
+        okmi_set_error("basis_create: missing input");
+        return 1;
+    }
+    *out = nullptr;
+    std::vector<ShellRec> shells(ncont);
+    std::vector<PrimRec> prims;
+    std::vector<double> T, craw;
+    std::vector<int> pl;
+    const double pref = std::pow(2.0 / M_PI, 0.75);
+    int crow = 0, prow = 0, orow = 0, tmat = 0, max_l = 0;
+    for (int c = 0; c < ncont; ++c) {
+        const int nc = ncart_per_cont[c];
+        const int np = nprim_per_cont[c];
+        if (nc <= 0 || np <= 0) {
+            okmi_set_error("basis_create: contraction %d has %d components and %d primitives", c, nc, np);
+            return 1;
+        }
+        const int* lc = lxlylz + 3 * crow;
+        const int l = lc[0] + lc[1] + lc[2];
+        if (l > OKMI_MAX_L) {
+            okmi_set_error("basis_create: angular momentum %d in contraction %d (up to %d supported)", l, c,
+                           OKMI_MAX_L);
+            return 1;
+        }
+        std::vector<int> ci(nc);
+        std::vector<double> colf(nc);
+        for (int j = 0; j < nc; ++j) {
+            const int* e = lc + 3 * j;
+            if (e[0] < 0 || e[1] < 0 || e[2] < 0 || e[0] + e[1] + e[2] != l) {
+                okmi_set_error("basis_create: contraction %d mixes angular momenta (component %d)", c, j);
+                return 1;
+            }
+            ci[j] = canon_index(l, e[0], e[1], e[2]);
+            double f = is_normalized ? 1.0
+                                     : 1.0 / std::sqrt((double)(dfact(2 * e[0] - 1) * dfact(2 * e[1] - 1) *
+                                                                dfact(2 * e[2] - 1)));
+            colf[j] = f * (cart_scale ? cart_scale[crow + j] : 1.0);
+        }
+        const int NC = ncart(l);
+        const int nrow = nout_per_cont ? nout_per_cont[c] : nc;
+        if (nrow <= 0) {
+            okmi_set_error("basis_create: contraction %d has no output rows", c);
+            return 1;
+        }
+        std::vector<double> blk((size_t)nrow * NC, 0.0);
+        if (!nout_per_cont) {
+            for (int j = 0; j < nc; ++j) blk[(size_t)j * NC + ci[j]] += colf[j];
+        } else {
+            for (int r = 0; r < nrow; ++r)
+                for (int j = 0; j < nc; ++j) blk[(size_t)r * NC + ci[j]] += transform[tmat + (size_t)r * nc + j] * colf[j];
+            tmat += nrow * nc;
+        }
+        const int at = cont_atom[c];
+        if (at < 0 || at >= natoms) {
+            okmi_set_error("basis_create: contraction %d refers to atom %d of %d", c, at, natoms);
+            return 1;
+        }
+        ShellRec& s = shells[c];
+        s.cx = geo[3 * at]; s.cy = geo[3 * at + 1]; s.cz = geo[3 * at + 2];
+        s.l = l; s.nprim = np; s.prim0 = prow; s.row0 = orow; s.nrow = nrow;
+        s.t0 = (int)T.size(); s.atom = at;
+        T.insert(T.end(), blk.begin(), blk.end());
+        for (int p = 0; p < np; ++p) {
+            const double a = prim_coeffs[2 * (prow + p)], cc = prim_coeffs[2 * (prow + p) + 1];
+            const double nh = is_normalized ? 1.0 : pref * std::pow(2.0, l) * std::pow(a, (2.0 * l + 3.0) / 4.0);
+            prims.push_back(PrimRec{a, cc * nh});
+            craw.push_back(cc);
+            pl.push_back(l);
+        }
+        if (l > max_l) max_l = l;
+        crow += nc; prow += np; orow += nrow;
+    }
+    OkmiBasis* b = new OkmiBasis();
+    b->ncont = ncont; b->nshell = ncont; b->nao_cart = crow; b->nao = orow; b->nprim = prow;
+    b->ntab = (int)T.size(); b->max_l = max_l; b->natoms = natoms; b->is_normalized = is_normalized;
+    (void)hipGetDevice(&b->device);
+    b->h_shells = new ShellRec[ncont];
+    std::memcpy(b->h_shells, shells.data(), sizeof(ShellRec) * ncont);
+    b->h_prims = new PrimRec[prow];
+    std::memcpy(b->h_prims, prims.data(), sizeof(PrimRec) * prow);
+    b->h_T = new double[T.size()];
+    std::memcpy(b->h_T, T.data(), sizeof(double) * T.size());
+    b->h_c = new double[prow];
+    std::memcpy(b->h_c, craw.data(), sizeof(double) * prow);
+    b->h_prim_l = new int[prow];
+    std::memcpy(b->h_prim_l, pl.data(), sizeof(int) * prow);
+    int rc = dev_upload(&b->d_shells, shells.data(), shells.size(), "basis_create: shells");
+    if (!rc) rc = dev_upload(&b->d_prims, prims.data(), prims.size(), "basis_create: primitives");
+    if (!rc) rc = dev_upload(&b->d_T, T.data(), T.size(), "basis_create: transform");
+    if (rc) {
+        okmi_basis_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return 0;
+}
+
+extern "C" int okmi_basis_destroy(okmi_basis* b)
+{
+    if (!b) return 0;
+    if (b->d_shells) (void)hipFree(b->d_shells);
+    if (b->d_prims) (void)hipFree(b->d_prims);
+    if (b->d_T) (void)hipFree(b->d_T);
+    delete[] b->h_shells;
+    delete[] b->h_prims;
+    delete[] b->h_T;
+    delete[] b->h_c;
+    delete[] b->h_prim_l;
+    delete b;
+    return 0;
+}
+
+extern "C" int okmi_basis_nao(const okmi_basis* b) { return b ? b->nao : -1; }
+
+extern "C" int okmi_ao_eval(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np, const int* drv,
+                            int ndrv, int out_dtype, void* d_out, int64_t ld, void* stream)
+{
+    g_err.clear();
+    if (!b || !d_out) {
+        okmi_set_error("ao_eval: null basis or output");
+        return 1;
+    }
+    if (check_grid(g, p0, np) || check_drv(drv, ndrv)) return 1;
+    if (ld < np) {
+        okmi_set_error("ao_eval: row stride %lld < %lld points", (long long)ld, (long long)np);
+        return 1;
+    }
+    if (out_dtype != OKMI_F64 && out_dtype != OKMI_F32) {
+        okmi_set_error("ao_eval: unknown output type %d", out_dtype);
+        return 1;
+    }
+    return okmi_launch_ao_eval(b, to_desc(g), p0, np, drv, ndrv, out_dtype, d_out, ld, (int64_t)b->nao * ld,
+                               (hipStream_t)stream);
+}
+
+extern "C" int okmi_mocreator(const double* d_ao, int nao, int64_t npts, int64_t ld_ao, const double* d_c, int nmo,
+                              double* d_mo, int64_t ld_mo, void* stream)
+{
+    g_err.clear();
+    if (!d_ao || !d_c || !d_mo || nao <= 0 || nmo <= 0 || npts < 0 || ld_ao < npts || ld_mo < npts) {
+        okmi_set_error("mocreator: invalid arguments");
+        return 1;
+    }
+    return okmi_launch_gemm_f64(d_ao, ld_ao, 0, 1, nao, d_c, nmo, npts, d_mo, ld_mo, (hipStream_t)stream);
+}
+
+extern "C" size_t okmi_workspace_bytes(const okmi_basis* b, int nsets, int64_t chunk)
+{
+    if (!b || nsets <= 0 || chunk <= 0) return 0;
+    return (size_t)nsets * (size_t)b->nao * (size_t)chunk * sizeof(double);
+}
+
+namespace {
+int64_t chunk_points(const okmi_basis* b, int nsets, int64_t np, size_t ws_bytes)
+{
+    const size_t per = (size_t)nsets * (size_t)b->nao * sizeof(double);
+    int64_t c = (int64_t)(ws_bytes / per);
+    if (c >= np) return np;
+    c &= ~(int64_t)63;
+    return c;
+}
+}  // namespace
+
+extern "C" int okmi_mo_eval(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np, const int* drv,
+                            int ndrv, const double* d_c, int nmo, double* d_mo, int64_t ld_mo, void* d_ws,
+                            size_t ws_bytes, void* stream)
+{
+    g_err.clear();
+    if (!b || !d_c || !d_mo || nmo <= 0 || ld_mo < np) {
+        okmi_set_error("mo_eval: invalid arguments");
+        return 1;
+    }
+    if (check_grid(g, p0, np) || check_drv(drv, ndrv)) return 1;
+    if (np == 0) return 0;
+    const int64_t chunk = chunk_points(b, ndrv, np, ws_bytes);
+    if (chunk <= 0 || !d_ws) {
+        okmi_set_error("mo_eval: workspace of %zu bytes holds no chunk (%zu bytes per 64 points)", ws_bytes,
+                       okmi_workspace_bytes(b, ndrv, 64));
+        return 1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const GridDesc gd = to_desc(g);
+    for (int64_t c0 = 0; c0 < np; c0 += chunk) {
+        const int64_t n = np - c0 < chunk ? np - c0 : chunk;
+        int rc = okmi_launch_ao_eval(b, gd, p0 + c0, n, drv, ndrv, OKMI_F64, d_ws, chunk, (int64_t)b->nao * chunk, st);
+        if (rc) return rc;
+        rc = okmi_launch_gemm_f64((const double*)d_ws, chunk, (int64_t)b->nao * chunk, ndrv, b->nao, d_c, nmo, n,
+                                  d_mo + c0, ld_mo, st);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int okmi_rho_sets(const int* drv_out, int nout, int* codes, int* ncodes)
+{
+    if (nout < 0 || nout > 16) {
+        okmi_set_error("rho: %d derivative outputs (up to 16 supported)", nout);
+        return 1;
+    }
+    int mask = 1;
+    for (int r = 0; r < nout; ++r) {
+        const int d = drv_out[r];
+        if (d < 0 || d > 9) {
+            okmi_set_error("The selection `drv=%d` is not valid!", d);
+            return 1;
+        }
+        mask |= 1 << d;
+        if (d >= 4 && d <= 6) mask |= 1 << (d - 3);
+        if (d == 7) mask |= (1 << 1) | (1 << 2);
+        if (d == 8) mask |= (1 << 1) | (1 << 3);
+        if (d == 9) mask |= (1 << 2) | (1 << 3);
+    }
+    int n = 0;
+    for (int c = 0; c < OKMI_MAX_SETS; ++c)
+        if (mask >> c & 1) codes[n++] = c;
+    *ncodes = n;
+    return 0;
+}
+
+extern "C" int okmi_rho(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np, const double* d_c,
+                        const double* d_occ, int nmo, const int* drv_out, int nout, double* d_rho, double* d_delta,
+                        int64_t ld_delta, double* d_mo_norm, void* d_ws, size_t ws_bytes, void* stream)
+{
+    g_err.clear();
+    if (!b || !d_c || !d_occ || !d_rho || nmo <= 0 || (nout > 0 && (!d_delta || ld_delta < np))) {
+        okmi_set_error("rho: invalid arguments");
+        return 1;
+    }
+    if (check_grid(g, p0, np)) return 1;
+    int codes[OKMI_MAX_SETS];
+    int nsets = 0;
+    if (okmi_rho_sets(drv_out, nout, codes, &nsets)) return 1;
+    if (np == 0) return 0;
+    const int64_t chunk = chunk_points(b, nsets, np, ws_bytes);
+    if (chunk <= 0 || !d_ws) {
+        okmi_set_error("rho: workspace of %zu bytes holds no chunk (%zu bytes per 64 points)", ws_bytes,
+                       okmi_workspace_bytes(b, nsets, 64));
+        return 1;
+    }
+    int64_t set_off[OKMI_MAX_SETS] = {0};
+    int set_mask = 0;
+    for (int s = 0; s < nsets; ++s) {
+        set_off[codes[s]] = (int64_t)s * b->nao * chunk;
+        set_mask |= 1 << codes[s];
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const GridDesc gd = to_desc(g);
+    for (int64_t c0 = 0; c0 < np; c0 += chunk) {
+        const int64_t n = np - c0 < chunk ? np - c0 : chunk;
+        int rc = okmi_launch_ao_eval(b, gd, p0 + c0, n, codes, nsets, OKMI_F64, d_ws, chunk, (int64_t)b->nao * chunk,
+                                     st);
+        if (rc) return rc;
+        rc = okmi_launch_rho_f64((const double*)d_ws, chunk, set_off, set_mask, b->nao, d_c, nmo, d_occ, n, drv_out,
+                                 nout, d_rho + c0, d_delta ? d_delta + c0 : nullptr, ld_delta, d_mo_norm, st);
+        if (rc) return rc;
+    }
+    return 0;
+}

@@ -1,0 +1,116 @@
+// Per-point evaluation of one contracted Cartesian shell and its
+// derivatives, templated on the angular momentum so that every monomial
+// index is a compile-time constant (no register indexing, no scratch).
+//
+// Math (orbkit/c_grid-based.c:38-71, orbkit/c_support.c:28-175): with
+// X = x - A_x etc., r2 = X^2+Y^2+Z^2 and radial sums over primitives
+//   S0 = sum_p d_p e_p,  S1 = sum_p (-2 a_p) d_p e_p,  S2 = sum_p 4 a_p^2 d_p e_p,
+// (e_p = exp(-a_p r2)) a component x^lx y^ly z^lz gives
+//   drv 0   : X^lx Y^ly Z^lz S0
+//   d/dx    : (lx X^(lx-1) S0 + X^(lx+1) S1) Y^ly Z^lz
+//   d2/dx2  : (lx(lx-1) X^(lx-2) S0 + (2lx+1) X^lx S1 + X^(lx+2) S2) Y^ly Z^lz
+//   d2/dxdy : X^(lx+1) Y^(ly+1) Z^lz S2 + K S0, K as orbkit/c_support.c:121-168
+//             (lowering term only; the reference's mixed-derivative formula,
+//             reproduced for parity).
+#pragma once
+#include "okmi_common.h"
+
+template <int L, typename R>
+struct ShellPows {
+    R p[3][L + 3];
+    __device__ __forceinline__ void init(R X, R Y, R Z)
+    {
+        p[0][0] = R(1); p[1][0] = R(1); p[2][0] = R(1);
+#pragma unroll
+        for (int n = 1; n < L + 3; ++n) {
+            p[0][n] = p[0][n - 1] * X;
+            p[1][n] = p[1][n - 1] * Y;
+            p[2][n] = p[2][n - 1] * Z;
+        }
+    }
+};
+
+// monomial with one exponent shifted; exponent < 0 never reaches here
+template <int L, typename R>
+__device__ __forceinline__ R mono(const ShellPows<L, R>& P, int lx, int ly, int lz)
+{
+    return P.p[0][lx] * P.p[1][ly] * P.p[2][lz];
+}
+
+// value of Cartesian component C (canonical order) for derivative code D
+template <int L, int C, int D, typename R>
+__device__ __forceinline__ R cart_comp(const ShellPows<L, R>& P, R S0, R S1, R S2)
+{
+    constexpr int e[3] = {cart_exp(L, C, 0), cart_exp(L, C, 1), cart_exp(L, C, 2)};
+    if constexpr (D == 0) {
+        return mono<L, R>(P, e[0], e[1], e[2]) * S0;
+    } else if constexpr (D <= 3) {
+        constexpr int a = D - 1;
+        constexpr int u0 = e[0] + (a == 0), u1 = e[1] + (a == 1), u2 = e[2] + (a == 2);
+        R v = mono<L, R>(P, u0, u1, u2) * S1;
+        if constexpr (e[a] > 0) {
+            constexpr int d0 = e[0] - (a == 0), d1 = e[1] - (a == 1), d2 = e[2] - (a == 2);
+            v += R(e[a]) * mono<L, R>(P, d0, d1, d2) * S0;
+        }
+        return v;
+    } else if constexpr (D <= 6) {
+        constexpr int a = D - 4;
+        constexpr int u0 = e[0] + 2 * (a == 0), u1 = e[1] + 2 * (a == 1), u2 = e[2] + 2 * (a == 2);
+        R v = mono<L, R>(P, u0, u1, u2) * S2 + R(2 * e[a] + 1) * mono<L, R>(P, e[0], e[1], e[2]) * S1;
+        if constexpr (e[a] >= 2) {
+            constexpr int d0 = e[0] - 2 * (a == 0), d1 = e[1] - 2 * (a == 1), d2 = e[2] - 2 * (a == 2);
+            v += R(e[a] * (e[a] - 1)) * mono<L, R>(P, d0, d1, d2) * S0;
+        }
+        return v;
+    } else {   // 7: xy, 8: xz, 9: yz
+        constexpr int i = D == 9 ? 1 : 0;
+        constexpr int j = D == 7 ? 1 : 2;
+        constexpr int u0 = e[0] + (i == 0) + (j == 0), u1 = e[1] + (i == 1) + (j == 1), u2 = e[2] + (i == 2) + (j == 2);
+        R v = mono<L, R>(P, u0, u1, u2) * S2;
+        if constexpr (e[i] == 0 && e[j] > 0) {
+            constexpr int d0 = e[0] - (j == 0), d1 = e[1] - (j == 1), d2 = e[2] - (j == 2);
+            v += R(e[j]) * mono<L, R>(P, d0, d1, d2) * S0;
+        } else if constexpr (e[i] > 0 && e[j] == 0) {
+            constexpr int d0 = e[0] - (i == 0), d1 = e[1] - (i == 1), d2 = e[2] - (i == 2);
+            v += R(e[i]) * mono<L, R>(P, d0, d1, d2) * S0;
+        } else if constexpr (e[i] > 0 && e[j] > 0) {
+            constexpr int d0 = e[0] - (i == 0) - (j == 0), d1 = e[1] - (i == 1) - (j == 1),
+                          d2 = e[2] - (i == 2) - (j == 2);
+            v += R(e[i] * e[j]) * mono<L, R>(P, d0, d1, d2) * S0;
+        }
+        return v;
+    }
+}
+
+template <int L, int D, typename R, int C = 0>
+__device__ __forceinline__ void cart_all(const ShellPows<L, R>& P, R S0, R S1, R S2, R* out)
+{
+    if constexpr (C < ncart(L)) {
+        out[C] = cart_comp<L, C, D, R>(P, S0, S1, S2);
+        cart_all<L, D, R, C + 1>(P, S0, S1, S2, out);
+    }
+}
+
+// runtime (wave-uniform) derivative code -> compile-time evaluation
+template <int L, typename R>
+__device__ __forceinline__ void cart_drv(const ShellPows<L, R>& P, int d, R S0, R S1, R S2, R* out)
+{
+    switch (d) {
+    case 0: cart_all<L, 0, R>(P, S0, S1, S2, out); break;
+    case 1: cart_all<L, 1, R>(P, S0, S1, S2, out); break;
+    case 2: cart_all<L, 2, R>(P, S0, S1, S2, out); break;
+    case 3: cart_all<L, 3, R>(P, S0, S1, S2, out); break;
+    case 4: cart_all<L, 4, R>(P, S0, S1, S2, out); break;
+    case 5: cart_all<L, 5, R>(P, S0, S1, S2, out); break;
+    case 6: cart_all<L, 6, R>(P, S0, S1, S2, out); break;
+    case 7: cart_all<L, 7, R>(P, S0, S1, S2, out); break;
+    case 8: cart_all<L, 8, R>(P, S0, S1, S2, out); break;
+    default: cart_all<L, 9, R>(P, S0, S1, S2, out); break;
+    }
+}
+
+// exp(-t) in FP64 for t >= 0; t beyond 746 underflows to +0 exactly as libm.
+__device__ __forceinline__ double gauss_f64(double t)
+{
+    return t < 746.0 ? exp(-t) : 0.0;
+}

@@ -1,0 +1,106 @@
+// AO evaluation kernel: one lane per grid point, wave-uniform loop over
+// shells, output rows written as coalesced SoA  out[set][row][ld].
+//
+// Replaces orbkit's c_lcreator (orbkit/c_grid-based.c:9-79) looped over
+// contractions by cy_core.aocreator (orbkit/cy_core.pyx:49-78), the ORCA row
+// renormalisation (orbkit/core.py:98-100) and cartesian2spherical
+// (orbkit/core.py:135-176): all of these are one pass here.
+#include "okmi_internal.h"
+#include "ao_device.h"
+
+namespace {
+
+struct AoArgs {
+    const ShellRec* shells;
+    const PrimRec* prims;
+    const double* T;
+    GridDesc grid;
+    int64_t p0, np;
+    int64_t ld;          // row stride of out (elements)
+    int64_t set_stride;  // stride between derivative sets (elements)
+    int nshell;
+    int ndrv;
+    int need1, need2;
+    int drv[OKMI_MAX_SETS];
+};
+
+template <int L, typename OutT>
+__device__ __forceinline__ void shell_out(const AoArgs& a, const ShellRec& sh, double px, double py, double pz,
+                                          int64_t i, bool act, OutT* out)
+{
+    const double X = px - sh.cx, Y = py - sh.cy, Z = pz - sh.cz;
+    const double rr = X * X + Y * Y + Z * Z;
+    double S0 = 0.0, S1 = 0.0, S2 = 0.0;
+    for (int p = 0; p < sh.nprim; ++p) {
+        const PrimRec pr = a.prims[sh.prim0 + p];
+        const double de = pr.d * gauss_f64(pr.alpha * rr);
+        S0 += de;
+        if (a.need1) S1 += (-2.0 * pr.alpha) * de;
+        if (a.need2) S2 += (4.0 * pr.alpha * pr.alpha) * de;
+    }
+    ShellPows<L, double> P;
+    P.init(X, Y, Z);
+    constexpr int NC = ncart(L);
+    const double* T = a.T + sh.t0;
+    for (int j = 0; j < a.ndrv; ++j) {
+        double c[NC];
+        cart_drv<L, double>(P, a.drv[j], S0, S1, S2, c);
+        OutT* o = out + j * a.set_stride + (int64_t)sh.row0 * a.ld + i;
+        for (int r = 0; r < sh.nrow; ++r) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) v = fma(T[r * NC + k], c[k], v);
+            if (act) o[(int64_t)r * a.ld] = (OutT)v;
+        }
+    }
+}
+
+template <typename OutT>
+__global__ __launch_bounds__(256) void k_ao_eval(AoArgs a, OutT* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < a.np;
+    double px, py, pz;
+    grid_point(a.grid, a.p0 + (act ? i : 0), px, py, pz);
+    for (int s = 0; s < a.nshell; ++s) {
+        const ShellRec sh = a.shells[s];
+        switch (sh.l) {
+        case 0: shell_out<0, OutT>(a, sh, px, py, pz, i, act, out); break;
+        case 1: shell_out<1, OutT>(a, sh, px, py, pz, i, act, out); break;
+        case 2: shell_out<2, OutT>(a, sh, px, py, pz, i, act, out); break;
+        case 3: shell_out<3, OutT>(a, sh, px, py, pz, i, act, out); break;
+        default: shell_out<4, OutT>(a, sh, px, py, pz, i, act, out); break;
+        }
+    }
+}
+
+}  // namespace
+
+int okmi_launch_ao_eval(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const int* drv, int ndrv,
+                        int out_dtype, void* out, int64_t ld, int64_t set_stride, hipStream_t st)
+{
+    if (np <= 0) return 0;
+    AoArgs a{};
+    a.shells = b->d_shells;
+    a.prims = b->d_prims;
+    a.T = b->d_T;
+    a.grid = g;
+    a.p0 = p0;
+    a.np = np;
+    a.ld = ld;
+    a.set_stride = set_stride;
+    a.nshell = b->nshell;
+    a.ndrv = ndrv;
+    a.need1 = a.need2 = 0;
+    for (int j = 0; j < ndrv; ++j) {
+        a.drv[j] = drv[j];
+        if (drv[j] >= 1) a.need1 = 1;
+        if (drv[j] >= 4) a.need2 = 1;
+    }
+    dim3 blk(256), grd((unsigned)((np + 255) / 256));
+    if (out_dtype == OKMI_F64)
+        hipLaunchKernelGGL(k_ao_eval<double>, grd, blk, 0, st, a, (double*)out);
+    else
+        hipLaunchKernelGGL(k_ao_eval<float>, grd, blk, 0, st, a, (float*)out);
+    return okmi_check_launch("ao_eval");
+}

@@ -1,0 +1,48 @@
+// Library-internal state and helpers (not part of the C ABI).
+#pragma once
+#include "okmi_common.h"
+#include "../../include/orbkit_amd.h"
+
+struct OkmiBasis {
+    int ncont = 0;
+    int nshell = 0;
+    int nao_cart = 0;      // Cartesian rows (before transform)
+    int nao = 0;           // output rows (Cartesian or spherical)
+    int nprim = 0;
+    int ntab = 0;          // doubles in T
+    int max_l = 0;
+    int natoms = 0;
+    int device = -1;
+    ShellRec* d_shells = nullptr;
+    PrimRec* d_prims = nullptr;
+    double* d_T = nullptr;
+    // host copies (kept for rebuilds and for the fused kernels' chunk plans)
+    ShellRec* h_shells = nullptr;
+    PrimRec* h_prims = nullptr;
+    double* h_T = nullptr;
+    double* h_c = nullptr;      // raw contraction coefficients per primitive
+    int* h_prim_l = nullptr;    // l per primitive
+    int is_normalized = 0;
+};
+
+void okmi_set_error(const char* fmt, ...);
+int okmi_check_launch(const char* what);
+int okmi_hip_fail(hipError_t e, const char* what);
+
+int okmi_launch_ao_eval(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const int* drv, int ndrv,
+                        int out_dtype, void* out, int64_t ld, int64_t set_stride, hipStream_t st);
+
+// MO values Phi_s = C A_s for NS sets stored [set][mo][point] (FP64 MFMA)
+int okmi_launch_gemm_f64(const double* A, int64_t ld, int64_t set_stride, int nsets, int nao, const double* C,
+                         int nmo, int64_t np, double* mo_out, int64_t ld_mo, hipStream_t st);
+
+// Density and derivative outputs from AO sets indexed by derivative code:
+// set_off[c] is the offset of code c's AO set in A (codes in set_mask),
+// out_code[r] the code of output row r.
+int okmi_launch_rho_f64(const double* A, int64_t ld, const int64_t* set_off, int set_mask, int nao,
+                        const double* C, int nmo, const double* occ, int64_t np, const int* out_code, int nout,
+                        double* rho, double* delta, int64_t ld_delta, double* mo_norm, hipStream_t st);
+
+// codes whose AO sets a density request needs: 0, the requested codes and
+// the first derivatives entering second-derivative product terms
+int okmi_rho_sets(const int* drv_out, int nout, int* codes, int* ncodes);

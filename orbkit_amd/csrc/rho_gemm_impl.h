@@ -1,0 +1,282 @@
+// FP64 MO contraction on MFMA (v_mfma_f64_16x16x4f64) with the density
+// assembly in the epilogue.
+//
+//   Phi_s[m][p] = sum_k C[m][k] A_s[k][p]         (orbkit/cy_core.pyx:80-101)
+//   rho[p]      = sum_m occ_m Phi_0[m][p]^2       (orbkit/core.py:266-274)
+//   delta_r[p]  = 2 sum_m occ_m (Phi_0 Phi_d + Phi_a Phi_b)   (orbkit/core.py:280-304)
+//
+// A workgroup is 4 waves; each wave owns 16 points and loops over MO tiles
+// of 16 in groups of MG, accumulating every AO set (NS) for those tiles in
+// registers.  In density mode the MOs are reduced in registers and never
+// written; in MO mode Phi is stored [set][mo][point].
+#pragma once
+#include "okmi_internal.h"
+
+namespace okmi_gemm {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+struct GemmArgs {
+    const double* A;
+    int64_t ld, set_stride;
+    int nao;
+    const double* C;
+    int nmo;
+    const double* occ;
+    int64_t np;
+    double* mo_out;
+    int64_t ld_mo;
+    double* rho;
+    double* delta;
+    int64_t ld_delta;
+    double* mo_norm;
+    // density mode: accumulators are indexed by derivative code
+    int set_mask;            // codes whose AO set is present
+    int out_mask;            // codes with a density output
+    int64_t set_off[OKMI_MAX_SETS];   // offset of code's AO set in A
+    int nout;
+    int out_code[16];        // derivative code of each output row
+};
+
+template <int MG, int NS>
+__global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs a)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 15, kq = lane >> 4;
+    const int64_t p = (int64_t)blockIdx.x * 64 + wave * 16 + n;
+    const bool pin = p < a.np;
+    const int nmt = (a.nmo + 15) >> 4;
+    const int ngroups = (nmt + MG - 1) / MG;
+
+    for (int g = 0; g < ngroups; ++g) {
+        d4 acc[MG][NS];
+#pragma unroll
+        for (int mt = 0; mt < MG; ++mt)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) acc[mt][s] = d4{0.0, 0.0, 0.0, 0.0};
+
+        const double* cbase[MG];
+        bool min_[MG];
+#pragma unroll
+        for (int mt = 0; mt < MG; ++mt) {
+            const int m = (g * MG + mt) * 16 + n;
+            min_[mt] = m < a.nmo;
+            cbase[mt] = a.C + (int64_t)(min_[mt] ? m : 0) * a.nao;
+        }
+        const double* abase = a.A + (pin ? p : 0);
+
+        for (int k0 = 0; k0 < a.nao; k0 += 4) {
+            const int k = k0 + kq;
+            const bool kin = k < a.nao;
+            double av[MG], bv[NS];
+#pragma unroll
+            for (int mt = 0; mt < MG; ++mt) av[mt] = (kin && min_[mt]) ? cbase[mt][k] : 0.0;
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                bv[s] = (kin && pin) ? abase[s * a.set_stride + (int64_t)k * a.ld] : 0.0;
+#pragma unroll
+            for (int mt = 0; mt < MG; ++mt)
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    acc[mt][s] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bv[s], acc[mt][s], 0, 0, 0);
+        }
+
+        // v_mfma_f64_16x16x4f64 leaves D[4*i + lane/16][lane%16] in item i,
+        // so this lane holds Phi_s[m = tile*16 + 4*i + kq][p]
+#pragma unroll
+        for (int mt = 0; mt < MG; ++mt) {
+            const int mb = (g * MG + mt) * 16 + kq;
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (pin && mb + 4 * i < a.nmo)
+                        a.mo_out[(int64_t)s * a.nmo * a.ld_mo + (int64_t)(mb + 4 * i) * a.ld_mo + p] = acc[mt][s][i];
+        }
+    }
+}
+
+template <int MG>
+int launch_ns(const GemmArgs& a, int nsets, hipStream_t st)
+{
+    dim3 blk(256), grd((unsigned)((a.np + 63) / 64));
+    switch (nsets) {
+#define OKMI_NS_CASE(NS)                                                          \
+    case NS:                                                                      \
+        if constexpr (MG * NS <= 32) {                                            \
+            hipLaunchKernelGGL((k_gemm_f64<MG, NS>), grd, blk, 0, st, a);   \
+            return 0;                                                             \
+        }                                                                         \
+        break;
+        OKMI_NS_CASE(1) OKMI_NS_CASE(2) OKMI_NS_CASE(3) OKMI_NS_CASE(4) OKMI_NS_CASE(5)
+        OKMI_NS_CASE(6) OKMI_NS_CASE(7) OKMI_NS_CASE(8) OKMI_NS_CASE(9) OKMI_NS_CASE(10)
+#undef OKMI_NS_CASE
+    default: break;
+    }
+    okmi_set_error("gemm_f64: unsupported set count %d for MO group %d", nsets, MG);
+    return 1;
+}
+
+inline int launch_mg(const GemmArgs& a, int nsets, hipStream_t st)
+{
+    const int nmt = (a.nmo + 15) / 16;
+    int mg = nsets <= 8 ? 4 : 3;
+    if (nmt < mg) mg = nmt;
+    switch (mg) {
+    case 1: return launch_ns<1>(a, nsets, st);
+    case 2: return launch_ns<2>(a, nsets, st);
+    case 3: return launch_ns<3>(a, nsets, st);
+    default: return launch_ns<4>(a, nsets, st);
+    }
+}
+
+
+// dsum[D] += 2 w (f0 f_D + e_D) for every output code D >= 1, unrolled at
+// compile time (e_D: f_a^2 for D = aa, f_a f_b for D = ab)
+template <int D, int NSC>
+__device__ __forceinline__ void rho_terms(const d4 (&acc)[NSC], int i, double f0, double w, int out_mask,
+                                          double (&dsum)[NSC])
+{
+    if constexpr (D < NSC) {
+        if (out_mask & (1 << D)) {
+            double t = f0 * acc[D][i];
+            if constexpr (D >= 4 && D <= 6) t += acc[D - 3][i] * acc[D - 3][i];
+            if constexpr (D == 7) t += acc[1][i] * acc[2][i];
+            if constexpr (D == 8) t += acc[1][i] * acc[3][i];
+            if constexpr (D == 9) t += acc[2][i] * acc[3][i];
+            dsum[D] += 2.0 * w * t;
+        }
+        rho_terms<D + 1, NSC>(acc, i, f0, w, out_mask, dsum);
+    }
+}
+
+// Density mode.  Accumulators are indexed by derivative code (NSC = 1, 4, 7
+// or 10 codes 0..NSC-1); codes absent from set_mask issue no loads and no
+// MFMAs (uniform branches).  Per point, with w = occ:
+//   rho   += w f0^2
+//   d_c   += 2 w (f0 f_c + e_c),  e_c = f_a^2 (c = aa), f_a f_b (c = ab), 0 (c = a)
+template <int MG, int NSC>
+__global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 15, kq = lane >> 4;
+    const int64_t p = (int64_t)blockIdx.x * 64 + wave * 16 + n;
+    const bool pin = p < a.np;
+    const int nmt = (a.nmo + 15) >> 4;
+    const int ngroups = (nmt + MG - 1) / MG;
+
+    double rho_acc = 0.0;
+    double dsum[NSC];
+#pragma unroll
+    for (int d = 0; d < NSC; ++d) dsum[d] = 0.0;
+
+    for (int g = 0; g < ngroups; ++g) {
+        d4 acc[MG][NSC];
+#pragma unroll
+        for (int mt = 0; mt < MG; ++mt)
+#pragma unroll
+            for (int d = 0; d < NSC; ++d) acc[mt][d] = d4{0.0, 0.0, 0.0, 0.0};
+        const double* cbase[MG];
+        bool min_[MG];
+#pragma unroll
+        for (int mt = 0; mt < MG; ++mt) {
+            const int m = (g * MG + mt) * 16 + n;
+            min_[mt] = m < a.nmo;
+            cbase[mt] = a.C + (int64_t)(min_[mt] ? m : 0) * a.nao;
+        }
+        const double* abase = a.A + (pin ? p : 0);
+        for (int k0 = 0; k0 < a.nao; k0 += 4) {
+            const int k = k0 + kq;
+            const bool kin = k < a.nao;
+            double av[MG];
+#pragma unroll
+            for (int mt = 0; mt < MG; ++mt) av[mt] = (kin && min_[mt]) ? cbase[mt][k] : 0.0;
+#pragma unroll
+            for (int d = 0; d < NSC; ++d) {
+                if (a.set_mask & (1 << d)) {
+                    const double bv = (kin && pin) ? abase[a.set_off[d] + (int64_t)k * a.ld] : 0.0;
+#pragma unroll
+                    for (int mt = 0; mt < MG; ++mt)
+                        acc[mt][d] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bv, acc[mt][d], 0, 0, 0);
+                }
+            }
+        }
+        // item i of this lane is Phi[m = tile*16 + 4*i + kq][p]
+#pragma unroll
+        for (int mt = 0; mt < MG; ++mt) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = (g * MG + mt) * 16 + kq + 4 * i;
+                const double w = m < a.nmo ? a.occ[m] : 0.0;
+                const double f0 = acc[mt][0][i];
+                rho_acc += w * (f0 * f0);
+                rho_terms<1, NSC>(acc[mt], i, f0, w, a.out_mask, dsum);
+                if (a.mo_norm) {
+                    double q = f0 * f0;
+                    q += __shfl_xor(q, 1);
+                    q += __shfl_xor(q, 2);
+                    q += __shfl_xor(q, 4);
+                    q += __shfl_xor(q, 8);
+                    if (n == 0 && m < a.nmo) atomicAdd(a.mo_norm + m, q);
+                }
+            }
+        }
+    }
+    rho_acc += __shfl_xor(rho_acc, 16);
+    rho_acc += __shfl_xor(rho_acc, 32);
+#pragma unroll
+    for (int d = 1; d < NSC; ++d) {
+        if (a.out_mask & (1 << d)) {
+            dsum[d] += __shfl_xor(dsum[d], 16);
+            dsum[d] += __shfl_xor(dsum[d], 32);
+        }
+    }
+    if (kq == 0 && pin) {
+        a.rho[p] = rho_acc;
+        for (int r = 0; r < a.nout; ++r) {
+            const int c = a.out_code[r];
+            double v = 0.0;
+#pragma unroll
+            for (int d = 0; d < NSC; ++d)
+                if (d == c) v = d == 0 ? 2.0 * rho_acc : dsum[d];
+            a.delta[(int64_t)r * a.ld_delta + p] = v;
+        }
+    }
+}
+
+template <int MG>
+int launch_rho_nsc(const GemmArgs& a, int nsc, hipStream_t st)
+{
+    dim3 blk(256), grd((unsigned)((a.np + 63) / 64));
+    switch (nsc) {
+    case 1: hipLaunchKernelGGL((k_rho_f64<MG, 1>), grd, blk, 0, st, a); return 0;
+    case 4: hipLaunchKernelGGL((k_rho_f64<MG, 4>), grd, blk, 0, st, a); return 0;
+    case 7: hipLaunchKernelGGL((k_rho_f64<MG, 7>), grd, blk, 0, st, a); return 0;
+    default:
+        if constexpr (MG <= 3) {
+            hipLaunchKernelGGL((k_rho_f64<MG, 10>), grd, blk, 0, st, a);
+            return 0;
+        }
+        break;
+    }
+    okmi_set_error("rho_f64: unsupported code range %d for MO group %d", nsc, MG);
+    return 1;
+}
+
+inline int launch_rho(const GemmArgs& a, int nsc, hipStream_t st)
+{
+    const int nmt = (a.nmo + 15) / 16;
+    int mg = nsc >= 10 ? 3 : 4;
+    if (nmt < mg) mg = nmt;
+    switch (mg) {
+    case 1: return launch_rho_nsc<1>(a, nsc, st);
+    case 2: return launch_rho_nsc<2>(a, nsc, st);
+    case 3: return launch_rho_nsc<3>(a, nsc, st);
+    default: return launch_rho_nsc<4>(a, nsc, st);
+    }
+}
+}  // namespace okmi_gemm
+
+// one translation unit per kernel family (compiled in parallel)
+int okmi_gemm_f64_mode0(const okmi_gemm::GemmArgs& a, int nsets, hipStream_t st);
+int okmi_gemm_f64_mode1(const okmi_gemm::GemmArgs& a, int nsc, hipStream_t st);
