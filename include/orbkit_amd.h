@@ -123,6 +123,38 @@ int okmi_rho(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np,
              const int* drv_out, int nout, double* d_rho, double* d_delta, int64_t ld_delta,
              double* d_mo_norm, void* d_ws, size_t ws_bytes, void* stream);
 
+/*
+ * Workspace bytes okmi_rho_fused needs to process `chunk` points per pass.
+ */
+size_t okmi_rho_fused_workspace_bytes(const okmi_basis* b, int nmo, int precision, int64_t chunk);
+
+/*
+ * Density and derivatives in one fused launch per pass: AO values are
+ * evaluated per 64-point tile into LDS (FP64 coordinates and exponent range
+ * reduction, FP32 values) and contracted with C on the matrix cores
+ * (precision OKMI_PREC_F32: FP32 MFMA; OKMI_PREC_BF16X3 / _BF16X6: C and AO
+ * split into two / three bf16 parts, 3 / 6 bf16 MFMA products, FP32
+ * accumulation); the density terms are assembled in FP64.  MOs are never
+ * written to memory.  Up to 64 MOs.  Outputs as okmi_rho.
+ * Replaces slice_rho's density branch (orbkit/core.py:179-308) with the
+ * reduced-precision contraction; okmi_rho is the FP64 form.
+ */
+int okmi_rho_fused(okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np,
+                   const double* d_c, const double* d_occ, int nmo,
+                   const int* drv_out, int nout, int precision,
+                   double* d_rho, double* d_delta, int64_t ld_delta,
+                   void* d_ws, size_t ws_bytes, void* stream);
+
+/*
+ * Kernel timing for measurement tools: while enabled, every kernel launch
+ * records a pair of HIP events on its stream.  okmi_profile_query sums the
+ * elapsed times of the launches of one kernel family ("ao_eval", "gemm_f64",
+ * "rho_f64", "rho_fused") since the last okmi_profile_enable call.
+ * No counterpart in the reference (orbkit has no kernel timers).
+ */
+int okmi_profile_enable(int on);
+int okmi_profile_query(const char* kernel, int* count, double* total_ms);
+
 #ifdef __cplusplus
 }
 #endif

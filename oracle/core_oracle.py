@@ -60,10 +60,10 @@ def clib():
     l.orc_lcreator.argtypes = [_P_d, ctypes.c_int64, _P_i, _P_d, _P_d, _P_d, _P_d, _P_d,
                                ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     l.orc_rho_slices.restype = ctypes.c_int
-    l.orc_rho_slices.argtypes = [_P_d, _P_d, _P_i, _P_i, _P_d, _P_i, _P_d, _P_i, ctypes.c_int,
-                                 ctypes.c_int, _P_d, _P_d, ctypes.c_int, _P_d, _P_d, ctypes.c_int,
-                                 _P_d, _P_d, _P_d, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
-                                 ctypes.c_int]
+    l.orc_rho_slices.argtypes = [_P_d, _P_d, _P_d, _P_i, ctypes.c_int, _P_i, _P_i, _P_d, _P_i, _P_d, _P_i,
+                                 ctypes.c_int, ctypes.c_int, _P_d, _P_d, ctypes.c_int, _P_d, _P_d,
+                                 ctypes.c_int, _P_d, _P_d, _P_d, ctypes.c_int64, ctypes.c_int64,
+                                 ctypes.c_int, ctypes.c_int]
     l.orc_max_threads.restype = ctypes.c_int
     _lib = l
   return _lib
@@ -281,3 +281,40 @@ def rho_compute(qc, x, y, z, is_vector, calc_ao=False, calc_mo=False, drv=None, 
   if laplacian:
     return rho, delta, delta.sum(axis=0)
   return rho, delta
+
+
+def rho_slices_c(qc, x, y, z, drv_codes=(), slice_length=10000, nthreads=1):
+  """Density (and first / diagonal second derivatives, codes 1-6) on a
+  vector grid with the C oracle's sliced, OpenMP-parallel driver.  Returns
+  (rho, delta (len(codes), n), mo_norm)."""
+  from orbkit_amd.tools import get_cart2sph
+  L = clib()
+  geo, ao_spec, mo_spec = qc.geo_spec, qc.ao_spec, qc.mo_spec
+  f = flat_basis(geo, ao_spec)
+  nao_cart = int(numpy.sum(f['ncart']))
+  scale = numpy.ascontiguousarray(ao_spec[0]['N'], dtype=float).reshape(-1) if 'N' in ao_spec[0] else None
+  if ao_spec.spherical:
+    eye = numpy.eye(nao_cart)
+    T = cartesian2spherical(eye, ao_spec, get_cart2sph)   # (nao_sph, nao_cart)
+  else:
+    T = None
+  nao_out = T.shape[0] if T is not None else nao_cart
+  C = numpy.ascontiguousarray(mo_spec.get_coeffs())
+  occ = numpy.ascontiguousarray(mo_spec.get_occ())
+  x, px = _d(x); y, py = _d(y); z, pz = _d(z)
+  n = len(x)
+  codes = numpy.ascontiguousarray(drv_codes, dtype=numpy.intc)
+  rho = numpy.zeros(n)
+  delta = numpy.zeros((max(len(codes), 1), n))
+  mo_norm = numpy.zeros(len(C))
+  keep = [_i(f['lxlylz']), _i(f['ncart']), _d(f['prims']), _i(f['nprim']), _d(f['geo']), _i(f['atoms'])]
+  sc = _d(scale) if scale is not None else (None, None)
+  tt = _d(T) if T is not None else (None, None)
+  rc = L.orc_rho_slices(rho.ctypes.data_as(_P_d), mo_norm.ctypes.data_as(_P_d), delta.ctypes.data_as(_P_d),
+                        codes.ctypes.data_as(_P_i), len(codes), keep[0][1], keep[1][1], keep[2][1], keep[3][1],
+                        keep[4][1], keep[5][1], len(f['ncart']), nao_cart, sc[1], tt[1], nao_out,
+                        _d(C)[1], _d(occ)[1], len(C), px, py, pz, n, int(slice_length), f['normalized'],
+                        int(nthreads))
+  if rc:
+    raise RuntimeError('orc_rho_slices failed (%d)' % rc)
+  return rho, delta[:len(codes)], mo_norm

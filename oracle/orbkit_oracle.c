@@ -13,9 +13,9 @@
  *   orc_lcreator            orbkit/c_grid-based.c:9-79 (one contraction)
  *   orc_aocreator           orbkit/cy_core.pyx:49-78 (loop over contractions)
  *   orc_mocreator           orbkit/cy_core.pyx:80-101 (MO[i,j] = sum_k C[i,k] AO[k,j])
- *   orc_rho_slices          orbkit/core.py:179-308 + :504-561, density-only
- *                           workload (rho, mo_norm) over slices with OpenMP
- *                           standing in for multiprocessing.Pool.
+ *   orc_rho_slices          orbkit/core.py:179-308 + :504-561, density and
+ *                           first/diagonal-second derivatives over slices
+ *                           with OpenMP standing in for multiprocessing.Pool.
  *
  * Summation order follows the reference: per Cartesian component a sum over
  * primitives in file order, the monomial multiplied after the sum for drv 0
@@ -208,14 +208,33 @@ void orc_transform(double *out, const double *T, const double *in, int nout, int
     }
 }
 
+/* One AO set of a slice as orbkit's ao_creator forms it: Cartesian rows,
+ * optional per-row factor, optional transform T (spherical). */
+static const double *slice_aos(double *ao, double *aos, const int *lxlylz, const int *ncart_per_cont,
+                               const double *prim_coeffs, const int *nprim_per_cont, const double *geo,
+                               const int *cont_atom, int ncont, int nao_cart, const double *row_scale,
+                               const double *T, int nao_out, const double *x, const double *y,
+                               const double *z, int64_t n, int drv, int is_normalized)
+{
+    orc_aocreator(ao, lxlylz, ncart_per_cont, prim_coeffs, nprim_per_cont, geo, cont_atom, ncont, x, y, z, n,
+                  drv, is_normalized);
+    if (row_scale)
+        for (int r = 0; r < nao_cart; r++)
+            for (int64_t j = 0; j < n; j++) ao[(int64_t)r * n + j] *= row_scale[r];
+    if (!T) return ao;
+    orc_transform(aos, T, ao, nao_out, nao_cart, n);
+    return aos;
+}
+
 /*
- * Density-only workload, sliced like rho_compute: for each slice of
- * slice_len points, AO (Cartesian) -> optional per-row scale -> optional
- * transform T (spherical) -> MOs -> rho += occ*phi^2, mo_norm += phi^2.
- * Slices run in parallel (OpenMP dynamic schedule) when nthreads > 1.
- * Returns 0 on success.
+ * rho_compute's density workload over slices of slice_len points, with
+ * orbkit's loop structure (orbkit/core.py:244-304): AO(0) -> MOs -> rho and
+ * mo_norm; then for each derivative code d (1-6) AO(d) -> MOs -> 2 occ phi_d phi,
+ * and for d = 4..6 a further AO(d-3) -> MOs -> 2 occ phi_a^2.
+ * delta[r][p] for r < ndrv.  Slices run in parallel (OpenMP dynamic schedule)
+ * standing in for multiprocessing.Pool.  Returns 0 on success.
  */
-int orc_rho_slices(double *rho, double *mo_norm,
+int orc_rho_slices(double *rho, double *mo_norm, double *delta, const int *drv, int ndrv,
                    const int *lxlylz, const int *ncart_per_cont, const double *prim_coeffs,
                    const int *nprim_per_cont, const double *geo, const int *cont_atom, int ncont,
                    int nao_cart, const double *row_scale, const double *T, int nao_out,
@@ -226,6 +245,8 @@ int orc_rho_slices(double *rho, double *mo_norm,
     if (slice_len <= 0) slice_len = npts;
     int64_t nslices = (npts + slice_len - 1) / slice_len;
     for (int i = 0; i < nmo; i++) mo_norm[i] = 0.0;
+    for (int r = 0; r < ndrv; r++)
+        if (drv[r] < 1 || drv[r] > 6) return 2;
     int fail = 0;
 #ifdef _OPENMP
     if (nthreads < 1) nthreads = 1;
@@ -236,7 +257,8 @@ int orc_rho_slices(double *rho, double *mo_norm,
         double *ao = (double *)malloc(sizeof(double) * (size_t)nao_cart * (size_t)slice_len);
         double *aos = T ? (double *)malloc(sizeof(double) * (size_t)nao_out * (size_t)slice_len) : NULL;
         double *mo = (double *)malloc(sizeof(double) * (size_t)nmo * (size_t)slice_len);
-        if (!part || !ao || !mo || (T && !aos)) {
+        double *md = (double *)malloc(sizeof(double) * (size_t)nmo * (size_t)slice_len);
+        if (!part || !ao || !mo || !md || (T && !aos)) {
 #ifdef _OPENMP
 #pragma omp atomic write
 #endif
@@ -247,16 +269,9 @@ int orc_rho_slices(double *rho, double *mo_norm,
 #endif
             for (int64_t s = 0; s < nslices; s++) {
                 int64_t p0 = s * slice_len, n = npts - p0 < slice_len ? npts - p0 : slice_len;
-                orc_aocreator(ao, lxlylz, ncart_per_cont, prim_coeffs, nprim_per_cont, geo, cont_atom,
-                              ncont, x + p0, y + p0, z + p0, n, 0, is_normalized);
-                if (row_scale)
-                    for (int r = 0; r < nao_cart; r++)
-                        for (int64_t j = 0; j < n; j++) ao[(int64_t)r * n + j] *= row_scale[r];
-                const double *a = ao;
-                if (T) {
-                    orc_transform(aos, T, ao, nao_out, nao_cart, n);
-                    a = aos;
-                }
+                const double *a = slice_aos(ao, aos, lxlylz, ncart_per_cont, prim_coeffs, nprim_per_cont, geo,
+                                            cont_atom, ncont, nao_cart, row_scale, T, nao_out, x + p0, y + p0,
+                                            z + p0, n, 0, is_normalized);
                 orc_mocreator(mo, a, C, nao_out, n, nmo);
                 for (int64_t j = 0; j < n; j++) rho[p0 + j] = 0.0;
                 for (int i = 0; i < nmo; i++) {
@@ -268,13 +283,35 @@ int orc_rho_slices(double *rho, double *mo_norm,
                     }
                     part[i] += nn;
                 }
+                for (int r = 0; r < ndrv; r++) {
+                    double *dr = delta + (int64_t)r * npts + p0;
+                    for (int64_t j = 0; j < n; j++) dr[j] = 0.0;
+                    a = slice_aos(ao, aos, lxlylz, ncart_per_cont, prim_coeffs, nprim_per_cont, geo, cont_atom,
+                                  ncont, nao_cart, row_scale, T, nao_out, x + p0, y + p0, z + p0, n, drv[r],
+                                  is_normalized);
+                    orc_mocreator(md, a, C, nao_out, n, nmo);
+                    for (int i = 0; i < nmo; i++)
+                        for (int64_t j = 0; j < n; j++)
+                            dr[j] += occ[i] * 2 * md[(int64_t)i * n + j] * mo[(int64_t)i * n + j];
+                    if (drv[r] >= 4) {
+                        a = slice_aos(ao, aos, lxlylz, ncart_per_cont, prim_coeffs, nprim_per_cont, geo,
+                                      cont_atom, ncont, nao_cart, row_scale, T, nao_out, x + p0, y + p0, z + p0,
+                                      n, drv[r] - 3, is_normalized);
+                        orc_mocreator(md, a, C, nao_out, n, nmo);
+                        for (int i = 0; i < nmo; i++)
+                            for (int64_t j = 0; j < n; j++) {
+                                double v = md[(int64_t)i * n + j];
+                                dr[j] += occ[i] * 2 * (v * v);
+                            }
+                    }
+                }
             }
 #ifdef _OPENMP
 #pragma omp critical
 #endif
             for (int i = 0; i < nmo; i++) mo_norm[i] += part[i];
         }
-        free(part); free(ao); free(aos); free(mo);
+        free(part); free(ao); free(aos); free(mo); free(md);
     }
     return fail;
 }

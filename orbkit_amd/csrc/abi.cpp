@@ -105,6 +105,61 @@ int okmi_check_launch(const char* what)
 
 extern "C" const char* okmi_last_error(void) { return g_err.c_str(); }
 
+namespace {
+struct ProfEvent {
+    std::string name;
+    hipEvent_t start, stop;
+};
+bool g_prof = false;
+std::vector<ProfEvent> g_prof_events;
+}  // namespace
+
+OkmiProfScope::OkmiProfScope(const char* name, hipStream_t s) : st(s)
+{
+    if (!g_prof) return;
+    ProfEvent e{name, nullptr, nullptr};
+    if (hipEventCreate(&e.start) != hipSuccess || hipEventCreate(&e.stop) != hipSuccess) return;
+    (void)hipEventRecord(e.start, st);
+    idx = (int)g_prof_events.size();
+    g_prof_events.push_back(e);
+}
+
+OkmiProfScope::~OkmiProfScope()
+{
+    if (idx >= 0) (void)hipEventRecord(g_prof_events[idx].stop, st);
+}
+
+extern "C" int okmi_profile_enable(int on)
+{
+    for (auto& e : g_prof_events) {
+        (void)hipEventSynchronize(e.stop);
+        (void)hipEventDestroy(e.start);
+        (void)hipEventDestroy(e.stop);
+    }
+    g_prof_events.clear();
+    g_prof = on != 0;
+    return 0;
+}
+
+extern "C" int okmi_profile_query(const char* name, int* count, double* total_ms)
+{
+    int n = 0;
+    double t = 0.0;
+    for (auto& e : g_prof_events) {
+        if (e.name != name) continue;
+        hipError_t r = hipEventSynchronize(e.stop);
+        if (r != hipSuccess) return okmi_hip_fail(r, "profile_query");
+        float ms = 0.f;
+        r = hipEventElapsedTime(&ms, e.start, e.stop);
+        if (r != hipSuccess) return okmi_hip_fail(r, "profile_query");
+        t += ms;
+        ++n;
+    }
+    if (count) *count = n;
+    if (total_ms) *total_ms = t;
+    return 0;
+}
+
 extern "C" int okmi_basis_create(int ncont, const int* ncart_per_cont, const int* lxlylz, const int* nprim_per_cont,
                                  const double* prim_coeffs, int natoms, const double* geo, const int* cont_atom,
                                  int is_normalized, const double* cart_scale, const int* nout_per_cont,
@@ -215,6 +270,8 @@ extern "C" int okmi_basis_destroy(okmi_basis* b)
     if (b->d_shells) (void)hipFree(b->d_shells);
     if (b->d_prims) (void)hipFree(b->d_prims);
     if (b->d_T) (void)hipFree(b->d_T);
+    if (b->d_T32) (void)hipFree(b->d_T32);
+    if (b->d_chunks) (void)hipFree(b->d_chunks);
     delete[] b->h_shells;
     delete[] b->h_prims;
     delete[] b->h_T;
@@ -369,4 +426,25 @@ extern "C" int okmi_rho(const okmi_basis* b, const okmi_grid* g, int64_t p0, int
         if (rc) return rc;
     }
     return 0;
+}
+
+extern "C" size_t okmi_rho_fused_workspace_bytes(const okmi_basis* b, int nmo, int precision, int64_t chunk)
+{
+    if (!b || nmo <= 0 || chunk <= 0) return 0;
+    return okmi_fused_ws_bytes(b, nmo, precision, chunk);
+}
+
+extern "C" int okmi_rho_fused(okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np, const double* d_c,
+                              const double* d_occ, int nmo, const int* drv_out, int nout, int precision,
+                              double* d_rho, double* d_delta, int64_t ld_delta, void* d_ws, size_t ws_bytes,
+                              void* stream)
+{
+    g_err.clear();
+    if (!b || !d_c || !d_occ || !d_rho || nmo <= 0 || (nout > 0 && (!d_delta || ld_delta < np))) {
+        okmi_set_error("rho_fused: invalid arguments");
+        return 1;
+    }
+    if (check_grid(g, p0, np)) return 1;
+    return okmi_launch_rho_fused(b, to_desc(g), p0, np, d_c, d_occ, nmo, precision, drv_out, nout, d_rho, d_delta,
+                                 ld_delta, d_ws, ws_bytes, (hipStream_t)stream);
 }

@@ -114,3 +114,17 @@ __device__ __forceinline__ double gauss_f64(double t)
 {
     return t < 746.0 ? exp(-t) : 0.0;
 }
+
+// exp(-t) to FP32 accuracy for any t >= 0 given in FP64: the range reduction
+// t*log2(e) = n + f (n integer, 0 <= f < 1) is done in FP64 so that large
+// arguments (tight exponents far from their centre) keep a full FP32
+// mantissa; 2^-f uses the hardware exp2 (v_exp_f32) and 2^-n an exponent
+// adjustment.  Results below FP32's range flush to +0.
+__device__ __forceinline__ float gauss_f32(double t)
+{
+    const double u = t * 1.4426950408889634;   // log2(e)
+    if (u >= 150.0) return 0.0f;
+    const double n = floor(u);
+    const float f = (float)(u - n);
+    return __builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f(-f), -(int)n);
+}

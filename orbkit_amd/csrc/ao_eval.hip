@@ -98,6 +98,7 @@ int okmi_launch_ao_eval(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64
         if (drv[j] >= 4) a.need2 = 1;
     }
     dim3 blk(256), grd((unsigned)((np + 255) / 256));
+    OkmiProfScope prof("ao_eval", st);
     if (out_dtype == OKMI_F64)
         hipLaunchKernelGGL(k_ao_eval<double>, grd, blk, 0, st, a, (double*)out);
     else

@@ -3,6 +3,9 @@
 #include "okmi_common.h"
 #include "../../include/orbkit_amd.h"
 
+#include <algorithm>
+#include <vector>
+
 struct OkmiBasis {
     int ncont = 0;
     int nshell = 0;
@@ -22,6 +25,10 @@ struct OkmiBasis {
     double* h_T = nullptr;
     double* h_c = nullptr;      // raw contraction coefficients per primitive
     int* h_prim_l = nullptr;    // l per primitive
+    // fused-kernel tables (built on first use)
+    float* d_T32 = nullptr;
+    int2* d_chunks = nullptr;
+    int nchunk = 0;
     int is_normalized = 0;
 };
 
@@ -46,3 +53,18 @@ int okmi_launch_rho_f64(const double* A, int64_t ld, const int64_t* set_off, int
 // codes whose AO sets a density request needs: 0, the requested codes and
 // the first derivatives entering second-derivative product terms
 int okmi_rho_sets(const int* drv_out, int nout, int* codes, int* ncodes);
+
+// fused AO -> MFMA -> density (reduced-precision contraction)
+size_t okmi_fused_ws_bytes(const OkmiBasis* b, int nmo, int prec, int64_t np);
+int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const double* d_c,
+                          const double* d_occ, int nmo, int prec, const int* drv_out, int nout, double* rho,
+                          double* delta, int64_t ld_delta, void* ws, size_t ws_bytes, hipStream_t st);
+
+// Optional per-kernel timing with HIP events recorded on the launch stream
+// (okmi_profile_enable / okmi_profile_query).  Scope object around a launch.
+struct OkmiProfScope {
+    int idx = -1;
+    hipStream_t st = nullptr;
+    OkmiProfScope(const char* name, hipStream_t s);
+    ~OkmiProfScope();
+};

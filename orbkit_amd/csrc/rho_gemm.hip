@@ -13,6 +13,7 @@ int okmi_launch_gemm_f64(const double* A, int64_t ld, int64_t set_stride, int ns
     a.A = A; a.ld = ld; a.set_stride = set_stride; a.nao = nao;
     a.C = C; a.nmo = nmo; a.np = np;
     a.mo_out = mo_out; a.ld_mo = ld_mo;
+    OkmiProfScope prof("gemm_f64", st);
     int rc = okmi_gemm_f64_mode0(a, nsets, st);
     if (rc) return rc;
     return okmi_check_launch("gemm_f64");
@@ -43,6 +44,7 @@ int okmi_launch_rho_f64(const double* A, int64_t ld, const int64_t* set_off, int
         if (out_code[r] > 0) a.out_mask |= 1 << out_code[r];
     }
     const int nsc = maxc == 0 ? 1 : maxc <= 3 ? 4 : maxc <= 6 ? 7 : 10;
+    OkmiProfScope prof("rho_f64", st);
     int rc = okmi_gemm_f64_mode1(a, nsc, st);
     if (rc) return rc;
     return okmi_check_launch("rho_f64");

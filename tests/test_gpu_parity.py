@@ -239,3 +239,26 @@ def test_slice_rho_direct(ok):
   r = O.slice_rho(qc.geo_spec, qc.ao_spec, qc.mo_spec, grid.x[100:900], grid.y[100:900], grid.z[100:900],
                   ['x', 'yy'], False, False, ok['tools'].get_cart2sph)
   assert rel_err(rho, r[0]) <= TOL and rel_err(mo_norm, r[1]) <= TOL and rel_err(delta, r[2]) <= TOL
+
+
+@pytest.mark.parametrize('prec,tol', [('f32', 1e-5), ('bf16x6', 1e-5), ('bf16x3', 1e-3)])
+def test_fused_precision(ok, prec, tol):
+  '''Reduced-precision fused path vs the FP64 path: norm-wise relative error
+  of rho, grad rho and the Laplacian (first bar; measured values printed).'''
+  core, grid = ok['core'], ok['grid']
+  qc, axes, _ = ok['W'].make('medium', npts_side=24)
+  grid.set_grid(*axes, is_vector=False)
+  print('case fused', prec, flush=True)
+  rho, d, lap = core.rho_compute(qc, laplacian=True)
+  frho, fd, flap = core.rho_compute(qc, laplacian=True, precision=prec)
+  rho1, g1 = core.rho_compute(qc, drv=['x', 'y', 'z', 'xy'])
+  frho1, fg1 = core.rho_compute(qc, drv=['x', 'y', 'z', 'xy'], precision=prec)
+  errs = {'rho': numpy.linalg.norm(frho - rho) / numpy.linalg.norm(rho),
+          'dd': numpy.linalg.norm(fd - d) / numpy.linalg.norm(d),
+          'lap': numpy.linalg.norm(flap - lap) / numpy.linalg.norm(lap),
+          'grad': numpy.linalg.norm(fg1 - g1) / numpy.linalg.norm(g1),
+          'rho_max': float(numpy.max(numpy.abs(frho - rho)[rho > 1e-10 * rho.max()] /
+                                     rho[rho > 1e-10 * rho.max()]))}
+  print(prec, errs)
+  for k in ('rho', 'dd', 'lap', 'grad'):
+    assert errs[k] <= tol, (k, errs[k])
