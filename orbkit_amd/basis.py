@@ -130,6 +130,9 @@ def _key(flat):
 def device_basis(geo_spec, ao_spec):
   '''Cached DeviceBasis for this geometry and basis (keyed by content).'''
   import torch
+  if not torch.cuda.is_available():
+    raise RuntimeError('orbkit_amd needs a HIP device (torch.cuda.is_available() is False); '
+                       'it has no CPU compute path')
   flat = flatten(geo_spec, ao_spec)
   key = (_key(flat), torch.cuda.current_device())
   b = _cache.get(key)

@@ -35,19 +35,20 @@ def grid_init(is_vector=False, force=False):
   if is_initialized and not force:
     return 0
   axes = [None, None, None]
+  step = lambda i: float(numpy.asarray(delta_[i]).reshape(-1)[0])
   for i in range(3):
     if max_[i] == min_[i]:
       axes[i] = numpy.array([min_[i]], dtype=numpy.float64)
       delta_[i] = 1
       N_[i] = 1
-    elif delta_[i]:
-      axes[i] = numpy.arange(min_[i], max_[i] + delta_[i], delta_[i], dtype=numpy.float64)
+    elif step(i):
+      axes[i] = numpy.arange(min_[i], max_[i] + step(i), step(i), dtype=numpy.float64)
       N_[i] = len(axes[i])
     else:
       axes[i] = numpy.array(numpy.linspace(min_[i], max_[i], N_[i]), dtype=numpy.float64)
       delta_[i] = axes[i][1] - axes[i][0]
   x, y, z = axes
-  d3r = float(numpy.prod(delta_))
+  d3r = float(numpy.prod([step(i) for i in range(3)]))
   is_initialized = True
   is_regular = True
   if is_vector:
