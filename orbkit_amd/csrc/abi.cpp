@@ -1,5 +1,6 @@
 // extern "C" entry points of liborbkit_amd.so (declared in include/orbkit_amd.h).
 #include "okmi_internal.h"
+#include "ao_device.h"
 
 #include <cmath>
 #include <cstdarg>
@@ -16,6 +17,58 @@ int dfact(int n)
     int r = 1;
     for (; n > 0; n -= 2) r *= n;
     return r;
+}
+
+// Smallest R (conservative) beyond which every primitive term of a shell,
+// with the polynomial and up to second-derivative factors, is below 1e-40:
+// max_p |d_p| (2l+2) (1+r)^(l+2) (1+2 a_p r)^2 exp(-a_p r^2) < 1e-40.
+double shell_rcut(int l, const double* alpha, const double* d, int n)
+{
+    const double cut = std::log(1e-40);
+    auto logf = [&](double r) {
+        double m = -1e300;
+        for (int p = 0; p < n; ++p) {
+            if (d[p] == 0.0) continue;
+            const double v = std::log(std::fabs(d[p])) + std::log(2.0 * l + 2.0) + (l + 2) * std::log1p(r) +
+                             2.0 * std::log1p(2.0 * alpha[p] * r) - alpha[p] * r * r;
+            m = std::max(m, v);
+        }
+        return m;
+    };
+    double lo = 0.0;
+    for (int p = 0; p < n; ++p) lo = std::max(lo, std::sqrt((l + 4.0) / (2.0 * alpha[p])) + 1.0);
+    if (logf(lo) < cut) return lo;
+    double hi = std::max(2.0 * lo, 1e4);
+    if (logf(hi) >= cut) return 1e300;
+    for (int it = 0; it < 200 && hi - lo > 1e-9 * hi; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        (logf(mid) < cut ? hi : lo) = mid;
+    }
+    return hi;
+}
+
+// T block kind: standard spherical rows (sph_row table), standard Cartesian
+// (identity with the component norm factors) or general.
+int shell_kind(int l, int nrow, const double* blk, bool spherical)
+{
+    const int NC = ncart(l);
+    auto close = [](double a, double b) { return std::fabs(a - b) <= 1e-14 * std::max(1.0, std::fabs(b)); };
+    if (spherical) {
+        if (l > 3 || nrow != 2 * l + 1) return SHELL_T_GENERAL;
+        for (int r = 0; r < nrow; ++r) {
+            double ref[15] = {0};
+            const SphRow row = sph_row(l, r);
+            for (int t = 0; t < row.n; ++t) ref[row.t[t].c] += row.t[t].v;
+            for (int c = 0; c < NC; ++c)
+                if (!close(blk[r * NC + c], ref[c])) return SHELL_T_GENERAL;
+        }
+        return SHELL_T_SPHERICAL;
+    }
+    if (nrow != NC) return SHELL_T_GENERAL;
+    for (int r = 0; r < nrow; ++r)
+        for (int c = 0; c < NC; ++c)
+            if (!close(blk[r * NC + c], r == c ? cart_norm(l, c) : 0.0)) return SHELL_T_GENERAL;
+    return SHELL_T_CARTESIAN;
 }
 
 int canon_index(int l, int lx, int ly, int lz)
@@ -228,14 +281,23 @@ extern "C" int okmi_basis_create(int ncont, const int* ncart_per_cont, const int
         s.cx = geo[3 * at]; s.cy = geo[3 * at + 1]; s.cz = geo[3 * at + 2];
         s.l = l; s.nprim = np; s.prim0 = prow; s.row0 = orow; s.nrow = nrow;
         s.t0 = (int)T.size(); s.atom = at;
+        s.kind = shell_kind(l, nrow, blk.data(), nout_per_cont != nullptr);
         T.insert(T.end(), blk.begin(), blk.end());
+        std::vector<double> al(np), dd(np);
         for (int p = 0; p < np; ++p) {
             const double a = prim_coeffs[2 * (prow + p)], cc = prim_coeffs[2 * (prow + p) + 1];
             const double nh = is_normalized ? 1.0 : pref * std::pow(2.0, l) * std::pow(a, (2.0 * l + 3.0) / 4.0);
             prims.push_back(PrimRec{a, cc * nh});
             craw.push_back(cc);
             pl.push_back(l);
+            al[p] = a;
+            // the largest T entry bounds the output rows' magnitude
+            double tmax = 0.0;
+            for (double t : blk) tmax = std::max(tmax, std::fabs(t));
+            dd[p] = cc * nh * std::max(1.0, tmax) * nc;
         }
+        const double rc = shell_rcut(l, al.data(), dd.data(), np);
+        s.rcut2 = rc >= 1e150 ? 1e300 : rc * rc;
         if (l > max_l) max_l = l;
         crow += nc; prow += np; orow += nrow;
     }
@@ -270,8 +332,8 @@ extern "C" int okmi_basis_destroy(okmi_basis* b)
     if (b->d_shells) (void)hipFree(b->d_shells);
     if (b->d_prims) (void)hipFree(b->d_prims);
     if (b->d_T) (void)hipFree(b->d_T);
-    if (b->d_T32) (void)hipFree(b->d_T32);
     if (b->d_chunks) (void)hipFree(b->d_chunks);
+    if (b->d_work) (void)hipFree(b->d_work);
     delete[] b->h_shells;
     delete[] b->h_prims;
     delete[] b->h_T;

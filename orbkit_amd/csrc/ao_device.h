@@ -128,3 +128,69 @@ __device__ __forceinline__ float gauss_f32(double t)
     const float f = (float)(u - n);
     return __builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f(-f), -(int)n);
 }
+
+// exp(-t) for t >= 0 in FP64 with ~1e-11 relative error, for the paths whose
+// AO values end in FP32 or narrower.  Cody-Waite reduction t = k ln2 + r,
+// |r| <= ln2/2 (ln2 split in a high part exact in 32 bits and a low part),
+// then the degree-9 Taylor polynomial of exp(-r) (truncation error
+// r^10/10! < 7e-12) and 2^-k by exponent adjustment (underflows to +0).
+__device__ __forceinline__ double gauss_f64_fast(double t)
+{
+    if (t > 1.0e4) return 0.0;
+    const double k = __builtin_rint(t * 1.4426950408889634);
+    double r = fma(-k, 6.93147180369123816490e-01, t);
+    r = fma(-k, 1.90821492927058770002e-10, r);
+    const double s = -r;
+    double p = 2.7557319223985893e-06;             // 1/9!
+    p = fma(p, s, 2.4801587301587302e-05);         // 1/8!
+    p = fma(p, s, 1.9841269841269841e-04);         // 1/7!
+    p = fma(p, s, 1.3888888888888889e-03);         // 1/6!
+    p = fma(p, s, 8.3333333333333333e-03);         // 1/5!
+    p = fma(p, s, 4.1666666666666667e-02);         // 1/4!
+    p = fma(p, s, 1.6666666666666667e-01);         // 1/3!
+    p = fma(p, s, 0.5);
+    p = fma(p, s, 1.0);
+    p = fma(p, s, 1.0);
+    return ldexp(p, -(int)k);
+}
+
+// Standard real-solid-harmonic rows in orbkit's (l, m) order
+// (m = 0, 1, -1, 2, -2, ...; p: x, y, z), each as at most three canonical
+// Cartesian components with orbkit's cart2sph coefficient (orbkit/tools.py:
+// 148-191) times the 1/sqrt((2lx-1)!!(2ly-1)!!(2lz-1)!!) normalisation
+// factor.  basis_create checks each spherical shell's transform against this
+// table before it is evaluated through it.
+struct SphTerm { int c; double v; };
+struct SphRow { int n; SphTerm t[3]; };
+__host__ __device__ constexpr SphRow sph_row(int l, int r)
+{
+    constexpr SphRow L0[1] = {{1, {{0, 1.0}, {0, 0.0}, {0, 0.0}}}};
+    constexpr SphRow L1[3] = {{1, {{0, 1.0}, {0, 0.0}, {0, 0.0}}},
+                              {1, {{1, 1.0}, {0, 0.0}, {0, 0.0}}},
+                              {1, {{2, 1.0}, {0, 0.0}, {0, 0.0}}}};
+    constexpr SphRow L2[5] = {{3, {{0, -0.28867513459481292}, {1, -0.28867513459481292}, {2, 0.57735026918962584}}},
+                              {1, {{4, 1.0}, {0, 0.0}, {0, 0.0}}},
+                              {1, {{5, 1.0}, {0, 0.0}, {0, 0.0}}},
+                              {2, {{0, 0.5}, {1, -0.5}, {0, 0.0}}},
+                              {1, {{3, 1.0}, {0, 0.0}, {0, 0.0}}}};
+    constexpr SphRow L3[7] = {{3, {{2, 0.2581988897471611}, {5, -0.3872983346207417}, {8, -0.3872983346207417}}},
+                              {3, {{0, -0.15811388300841897}, {3, -0.15811388300841897}, {6, 0.63245553203367599}}},
+                              {3, {{1, -0.15811388300841897}, {4, -0.15811388300841897}, {7, 0.63245553203367599}}},
+                              {2, {{5, 0.5}, {8, -0.5}, {0, 0.0}}},
+                              {1, {{9, 1.0}, {0, 0.0}, {0, 0.0}}},
+                              {2, {{0, 0.20412414523193148}, {3, -0.61237243569579447}, {0, 0.0}}},
+                              {2, {{1, -0.20412414523193148}, {4, 0.61237243569579447}, {0, 0.0}}}};
+    return l == 0 ? L0[r] : l == 1 ? L1[r] : l == 2 ? L2[r] : L3[r];
+}
+
+// 1/sqrt((2lx-1)!!(2ly-1)!!(2lz-1)!!) of canonical component c
+__host__ __device__ constexpr double cart_norm(int l, int c)
+{
+    int d = 1;
+    for (int a = 0; a < 3; ++a) {
+        int e = cart_exp(l, c, a);
+        for (int n = 2 * e - 1; n > 0; n -= 2) d *= n;
+    }
+    return d == 1 ? 1.0 : d == 3 ? 0.57735026918962584 : d == 9 ? 0.33333333333333333
+         : d == 15 ? 0.2581988897471611 : d == 105 ? 0.09759000729485333 : 0.0;
+}

@@ -32,7 +32,7 @@ __device__ __forceinline__ void shell_out(const AoArgs& a, const ShellRec& sh, d
     const double rr = X * X + Y * Y + Z * Z;
     double S0 = 0.0, S1 = 0.0, S2 = 0.0;
     for (int p = 0; p < sh.nprim; ++p) {
-        const PrimRec pr = a.prims[sh.prim0 + p];
+        const PrimRec pr = load_prim(a.prims, sh.prim0 + p);
         const double de = pr.d * gauss_f64(pr.alpha * rr);
         S0 += de;
         if (a.need1) S1 += (-2.0 * pr.alpha) * de;
@@ -41,7 +41,7 @@ __device__ __forceinline__ void shell_out(const AoArgs& a, const ShellRec& sh, d
     ShellPows<L, double> P;
     P.init(X, Y, Z);
     constexpr int NC = ncart(L);
-    const double* T = a.T + sh.t0;
+    const OKMI_CONST double* T = cptr(a.T) + sh.t0;
     for (int j = 0; j < a.ndrv; ++j) {
         double c[NC];
         cart_drv<L, double>(P, a.drv[j], S0, S1, S2, c);
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_ao_eval(AoArgs a, OutT* __restrict__ ou
     double px, py, pz;
     grid_point(a.grid, a.p0 + (act ? i : 0), px, py, pz);
     for (int s = 0; s < a.nshell; ++s) {
-        const ShellRec sh = a.shells[s];
+        const ShellRec sh = load_shell(a.shells, s);
         switch (sh.l) {
         case 0: shell_out<0, OutT>(a, sh, px, py, pz, i, act, out); break;
         case 1: shell_out<1, OutT>(a, sh, px, py, pz, i, act, out); break;

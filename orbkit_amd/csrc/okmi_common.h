@@ -16,6 +16,8 @@
 // in column-wise.
 struct ShellRec {
     double cx, cy, cz;   // centre (bohr)
+    double rcut2;        // beyond this squared distance every value and
+                         // derivative of the shell is below 1e-40
     int l;               // angular momentum (all components share it)
     int nprim;           // primitives
     int prim0;           // first primitive in the primitive table
@@ -23,9 +25,11 @@ struct ShellRec {
     int nrow;            // output rows of this shell
     int t0;              // offset of this shell's T block
     int atom;            // nucleus index (for nuclear-position gradients)
-    int pad[3];
+    int kind;            // T is: 0 general, 1 standard spherical, 2 standard Cartesian
 };
 static_assert(sizeof(ShellRec) == 64, "ShellRec layout");
+
+enum { SHELL_T_GENERAL = 0, SHELL_T_SPHERICAL = 1, SHELL_T_CARTESIAN = 2 };
 
 // Primitive: exponent alpha and d = c * (2/pi)^(3/4) 2^l alpha^((2l+3)/4)
 // (or c alone for pre-normalised bases, orbkit/c_support.c:182).
@@ -57,6 +61,41 @@ __device__ __forceinline__ void grid_point(const GridDesc& g, int64_t p, double&
     } else {
         px = g.x[p]; py = g.y[p]; pz = g.z[p];
     }
+}
+
+// Read-only tables are read through the constant address space: loads with
+// wave-uniform addresses then become scalar loads (the compiler cannot prove
+// on its own that the kernel's stores leave the tables untouched).
+#define OKMI_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const OKMI_CONST T* cptr(const T* p)
+{
+    return (const OKMI_CONST T*)p;
+}
+
+__device__ __forceinline__ ShellRec load_shell(const ShellRec* base, int i)
+{
+    const OKMI_CONST ShellRec* p = cptr(base) + i;
+    ShellRec s;
+    s.cx = p->cx; s.cy = p->cy; s.cz = p->cz; s.rcut2 = p->rcut2;
+    s.l = p->l; s.nprim = p->nprim; s.prim0 = p->prim0; s.row0 = p->row0;
+    s.nrow = p->nrow; s.t0 = p->t0; s.atom = p->atom; s.kind = p->kind;
+    return s;
+}
+
+__device__ __forceinline__ PrimRec load_prim(const PrimRec* base, int i)
+{
+    const OKMI_CONST PrimRec* p = cptr(base) + i;
+    PrimRec r;
+    r.alpha = p->alpha;
+    r.d = p->d;
+    return r;
+}
+
+__device__ __forceinline__ int2 load_int2(const int2* base, int i)
+{
+    const OKMI_CONST int* p = (const OKMI_CONST int*)cptr(base) + 2 * i;
+    return make_int2(p[0], p[1]);
 }
 
 __host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }

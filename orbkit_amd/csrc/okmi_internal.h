@@ -26,8 +26,8 @@ struct OkmiBasis {
     double* h_c = nullptr;      // raw contraction coefficients per primitive
     int* h_prim_l = nullptr;    // l per primitive
     // fused-kernel tables (built on first use)
-    float* d_T32 = nullptr;
     int2* d_chunks = nullptr;
+    int* d_work = nullptr;
     int nchunk = 0;
     int is_normalized = 0;
 };

@@ -1,15 +1,24 @@
 // Fused density kernel: AO evaluation -> LDS -> MFMA contraction with the MO
 // coefficients -> density terms, in one launch.  MOs never leave registers.
 //
-// One wave owns 64 grid points (lane = point while evaluating AOs).  The AO
-// rows are processed in chunks of 32 (one K step of the bf16 MFMA); for each
-// chunk the wave evaluates the shells overlapping it (FP64 coordinates and
-// exponent range reduction, FP32 values), stores the rows in LDS [row][point],
-// then multiplies them with the matching 32 columns of C:
+// A workgroup of four waves owns a tile of 64 grid points: a 4x4x4 brick of a
+// regular grid (compact in space) or 64 consecutive points of a vector grid.
+// The AO rows are processed in chunks of 32 (one K step of the bf16 MFMA).
+// For each chunk the four waves share out the shells overlapping it (a
+// host-balanced work list; lane = point) and evaluate them in FP64
+// (coordinates, exponentials, polynomials, the Cartesian-to-spherical map),
+// storing each value rounded once to FP32 in LDS [row][point] (double
+// buffered, one barrier per chunk).  Each wave then multiplies the rows of its
+// 16-point column with the matching 32 columns of C:
 //   F32     v_mfma_f32_16x16x4_f32, C and AO in FP32
 //   BF16X3  v_mfma_f32_16x16x32_bf16 on (hi, lo) splits, 3 products
 //   BF16X6  (hi, mid, lo) splits, 6 products (terms below 2^-24 dropped)
 // accumulating in FP32.  The epilogue forms the density terms in FP64.
+//
+// Screening: a shell whose screening radius (every value and derivative
+// below 1e-40 beyond it) does not reach the tile's bounding box is not
+// evaluated (its rows are zero); a chunk with no such shell is skipped,
+// contraction included.
 //
 // Derivative codes are handled in passes of at most 4 AO sets each:
 //   PASS 0  {0}          rho
@@ -34,8 +43,9 @@ constexpr int PTS = 64;       // points per wave
 struct FusedArgs {
     const ShellRec* shells;
     const PrimRec* prims;
-    const float* T;           // FP32 copy of the transform table
-    const int2* chunks;       // first/last shell overlapping each chunk
+    const double* T;          // general transform blocks (kind 0 shells)
+    const int2* chunks;       // per chunk: first entry in `work`, entry count
+    const int* work;          // shell index | (evaluating wave << 24), per chunk
     GridDesc grid;
     int64_t p0, np;
     int nao, nchunk;
@@ -47,6 +57,8 @@ struct FusedArgs {
     double* term2;            // [10][ld_term]: f_a f_b halves (codes 4..9)
     int64_t ld_term;
     int need_mask;            // codes whose terms are wanted
+    int brick;                // regular grid tiled in 4x4x4 bricks (p0 = 0, all points)
+    int64_t nbx, nby, nbz;
 };
 
 template <int PASS> struct PassCodes;
@@ -55,6 +67,95 @@ template <> struct PassCodes<1> { static constexpr int n = 4; static constexpr i
 template <> struct PassCodes<2> { static constexpr int n = 4; static constexpr int c[4] = {0, 4, 5, 6}; };
 template <> struct PassCodes<3> { static constexpr int n = 4; static constexpr int c[4] = {0, 7, 8, 9}; };
 
+// point of lane `ln` of this block's tile: index relative to p0, validity,
+// and the (clamped) coordinates
+__device__ __forceinline__ void tile_point(const FusedArgs& a, int ln, int64_t& p, bool& act, double& px, double& py,
+                                           double& pz)
+{
+    if (a.brick) {
+        int64_t t = blockIdx.x;
+        const int64_t bz = t % a.nbz;
+        t /= a.nbz;
+        const int64_t by = t % a.nby, bx = t / a.nby;
+        int64_t ix = 4 * bx + (ln >> 4), iy = 4 * by + ((ln >> 2) & 3), iz = 4 * bz + (ln & 3);
+        act = ix < a.grid.nx && iy < a.grid.ny && iz < a.grid.nz;
+        ix = ix < a.grid.nx ? ix : a.grid.nx - 1;
+        iy = iy < a.grid.ny ? iy : a.grid.ny - 1;
+        iz = iz < a.grid.nz ? iz : a.grid.nz - 1;
+        p = (ix * a.grid.ny + iy) * a.grid.nz + iz;
+        px = a.grid.x[ix]; py = a.grid.y[iy]; pz = a.grid.z[iz];
+    } else {
+        const int64_t pl = (int64_t)blockIdx.x * PTS + ln;
+        act = pl < a.np;
+        p = act ? pl : a.np - 1;
+        grid_point(a.grid, a.p0 + p, px, py, pz);
+    }
+}
+
+// value known to be equal in all lanes -> scalar register, so that the
+// compiler treats everything derived from it as wave-uniform (scalar loads
+// of the shell tables, scalar branches)
+__device__ __forceinline__ double uniform(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) v = fmin(v, __shfl_xor(v, o));
+    return uniform(v);
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) v = fmax(v, __shfl_xor(v, o));
+    return uniform(v);
+}
+
+struct Box {
+    double lo[3], hi[3];
+    __device__ __forceinline__ bool reaches(const ShellRec sh) const
+    {
+        const double c[3] = {sh.cx, sh.cy, sh.cz};
+        double d2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double d = fmax(fmax(lo[k] - c[k], c[k] - hi[k]), 0.0);
+            d2 = fma(d, d, d2);
+        }
+        return d2 < sh.rcut2;
+    }
+};
+
+template <int L, int R>
+__device__ __forceinline__ void sph_rows(const double* c, float* dst, int rlo, int rhi)
+{
+    if constexpr (L <= 3 && R < 2 * L + 1) {
+        if (R >= rlo && R < rhi) {
+            constexpr SphRow row = sph_row(L, R);
+            double v = row.t[0].v * c[row.t[0].c];
+            if constexpr (row.n > 1) v = fma(row.t[1].v, c[row.t[1].c], v);
+            if constexpr (row.n > 2) v = fma(row.t[2].v, c[row.t[2].c], v);
+            dst[R * LDS_LD] = (float)v;
+        }
+        sph_rows<L, R + 1>(c, dst, rlo, rhi);
+    }
+}
+
+template <int L, int R>
+__device__ __forceinline__ void cart_rows(const double* c, float* dst, int rlo, int rhi)
+{
+    if constexpr (R < ncart(L)) {
+        if (R >= rlo && R < rhi) dst[R * LDS_LD] = (float)(c[R] * cart_norm(L, R));
+        cart_rows<L, R + 1>(c, dst, rlo, rhi);
+    }
+}
+
 template <int L, int PASS>
 __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const ShellRec& sh, double px, double py,
                                                  double pz, int k0, float* lds, int lane)
@@ -62,42 +163,48 @@ __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const Shell
     constexpr int NS = PassCodes<PASS>::n;
     constexpr bool need1 = PASS == 1 || PASS == 2;
     constexpr bool need2 = PASS >= 2;
-    const double Xd = px - sh.cx, Yd = py - sh.cy, Zd = pz - sh.cz;
-    const double rr = Xd * Xd + Yd * Yd + Zd * Zd;
-    float S0 = 0.f, S1 = 0.f, S2 = 0.f;
+    const double X = px - sh.cx, Y = py - sh.cy, Z = pz - sh.cz;
+    const double rr = X * X + Y * Y + Z * Z;
+    double S0 = 0.0, S1 = 0.0, S2 = 0.0;
     for (int p = 0; p < sh.nprim; ++p) {
-        const PrimRec pr = a.prims[sh.prim0 + p];
-        const float de = (float)pr.d * gauss_f32(pr.alpha * rr);
+        const PrimRec pr = load_prim(a.prims, sh.prim0 + p);
+        const double de = pr.d * gauss_f64_fast(pr.alpha * rr);
         S0 += de;
-        if constexpr (need1) S1 += (float)(-2.0 * pr.alpha) * de;
-        if constexpr (need2) S2 += (float)(4.0 * pr.alpha * pr.alpha) * de;
+        if constexpr (need1) S1 = fma(-2.0 * pr.alpha, de, S1);
+        if constexpr (need2) S2 = fma(4.0 * pr.alpha * pr.alpha, de, S2);
     }
-    ShellPows<L, float> P;
-    P.init((float)Xd, (float)Yd, (float)Zd);
+    ShellPows<L, double> P;
+    P.init(X, Y, Z);
     constexpr int NC = ncart(L);
-    const float* T = a.T + sh.t0;
     const int rlo = max(0, k0 - sh.row0), rhi = min(sh.nrow, k0 + KC - sh.row0);
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        float c[NC];
+        double c[NC];
         switch (PassCodes<PASS>::c[s]) {
-        case 0: cart_all<L, 0, float>(P, S0, S1, S2, c); break;
-        case 1: cart_all<L, 1, float>(P, S0, S1, S2, c); break;
-        case 2: cart_all<L, 2, float>(P, S0, S1, S2, c); break;
-        case 3: cart_all<L, 3, float>(P, S0, S1, S2, c); break;
-        case 4: cart_all<L, 4, float>(P, S0, S1, S2, c); break;
-        case 5: cart_all<L, 5, float>(P, S0, S1, S2, c); break;
-        case 6: cart_all<L, 6, float>(P, S0, S1, S2, c); break;
-        case 7: cart_all<L, 7, float>(P, S0, S1, S2, c); break;
-        case 8: cart_all<L, 8, float>(P, S0, S1, S2, c); break;
-        default: cart_all<L, 9, float>(P, S0, S1, S2, c); break;
+        case 0: cart_all<L, 0, double>(P, S0, S1, S2, c); break;
+        case 1: cart_all<L, 1, double>(P, S0, S1, S2, c); break;
+        case 2: cart_all<L, 2, double>(P, S0, S1, S2, c); break;
+        case 3: cart_all<L, 3, double>(P, S0, S1, S2, c); break;
+        case 4: cart_all<L, 4, double>(P, S0, S1, S2, c); break;
+        case 5: cart_all<L, 5, double>(P, S0, S1, S2, c); break;
+        case 6: cart_all<L, 6, double>(P, S0, S1, S2, c); break;
+        case 7: cart_all<L, 7, double>(P, S0, S1, S2, c); break;
+        case 8: cart_all<L, 8, double>(P, S0, S1, S2, c); break;
+        default: cart_all<L, 9, double>(P, S0, S1, S2, c); break;
         }
         float* dst = lds + s * KC * LDS_LD + (sh.row0 - k0) * LDS_LD + lane;
-        for (int r = rlo; r < rhi; ++r) {
-            float v = 0.f;
+        if (sh.kind == SHELL_T_SPHERICAL) {
+            sph_rows<L, 0>(c, dst, rlo, rhi);
+        } else if (sh.kind == SHELL_T_CARTESIAN) {
+            cart_rows<L, 0>(c, dst, rlo, rhi);
+        } else {
+            const OKMI_CONST double* T = cptr(a.T) + sh.t0;
+            for (int r = rlo; r < rhi; ++r) {
+                double v = 0.0;
 #pragma unroll
-            for (int k = 0; k < NC; ++k) v = fmaf(T[r * NC + k], c[k], v);
-            dst[r * LDS_LD] = v;
+                for (int k = 0; k < NC; ++k) v = fma(T[r * NC + k], c[k], v);
+                dst[r * LDS_LD] = (float)v;
+            }
         }
     }
 }
@@ -140,28 +247,36 @@ __device__ __forceinline__ f4 mfma_split(const Split<PREC>& A, const Split<PREC>
 }
 
 template <int PREC, int PASS, int MT>
-__global__ __launch_bounds__(64) void k_rho_fused(FusedArgs a)
+__global__ __launch_bounds__(256, 2) void k_rho_fused(FusedArgs a)
 {
     constexpr int NS = PassCodes<PASS>::n;
-    constexpr int NT = PTS / 16;
-    __shared__ float lds[NS * KC * LDS_LD];
-    const int lane = threadIdx.x;
-    const int64_t tile0 = (int64_t)blockIdx.x * PTS;
-    const int64_t pl = tile0 + lane;
+    constexpr int BUF = NS * KC * LDS_LD;
+    __shared__ float lds[2 * BUF];
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int64_t pidx;
+    bool pact;
     double px, py, pz;
-    grid_point(a.grid, a.p0 + (pl < a.np ? pl : 0), px, py, pz);
+    tile_point(a, lane, pidx, pact, px, py, pz);   // lane l of every wave: point l of the tile
+    Box box;
+    box.lo[0] = wave_min(px); box.hi[0] = wave_max(px);
+    box.lo[1] = wave_min(py); box.hi[1] = wave_max(py);
+    box.lo[2] = wave_min(pz); box.hi[2] = wave_max(pz);
 
-    f4 acc[MT][NT][NS];
+    f4 acc[MT][NS];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-            for (int s = 0; s < NS; ++s) acc[mt][nt][s] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int s = 0; s < NS; ++s) acc[mt][s] = f4{0.f, 0.f, 0.f, 0.f};
 
     const int g = lane >> 4, n16 = lane & 15;
+    int buf = 0;
     for (int ch = 0; ch < a.nchunk; ++ch) {
         const int k0 = ch * KC;
+        const int2 cw = load_int2(a.chunks, ch);
+        bool any = false;
+        for (int e = cw.x; e < cw.x + cw.y; ++e) any |= box.reaches(load_shell(a.shells, cptr(a.work)[e] & 0xffffff));
+        if (!any) continue;   // every AO row of this chunk is zero on the tile (same decision in all waves)
+        float* L = lds + buf * BUF;
         // operand A (MO coefficients) for this chunk
         Split<PREC> ca[MT];
         float cf[MT][KC / 4];
@@ -181,108 +296,114 @@ __global__ __launch_bounds__(64) void k_rho_fused(FusedArgs a)
                 for (int q = 0; q < NP; ++q)
                     ca[mt].p[q] = *(const bf8*)(C + ((int64_t)q * a.mpad + mt * 16 + n16) * a.kpad + k0 + 8 * g);
         }
-        // AO rows [k0, k0+KC) for this wave's 64 points
-        const int2 sr = a.chunks[ch];
-        if (k0 + KC > a.nao) {
+        // this wave's share of the AO rows [k0, k0+KC)
+        if (wave == 0 && k0 + KC > a.nao) {
             for (int s = 0; s < NS; ++s)
-                for (int r = a.nao - k0; r < KC; ++r) lds[s * KC * LDS_LD + r * LDS_LD + lane] = 0.f;
+                for (int r = a.nao - k0; r < KC; ++r) L[s * KC * LDS_LD + r * LDS_LD + lane] = 0.f;
         }
-        for (int si = sr.x; si <= sr.y; ++si) {
-            const ShellRec sh = a.shells[si];
-            switch (sh.l) {
-            case 0: eval_shell_chunk<0, PASS>(a, sh, px, py, pz, k0, lds, lane); break;
-            case 1: eval_shell_chunk<1, PASS>(a, sh, px, py, pz, k0, lds, lane); break;
-            case 2: eval_shell_chunk<2, PASS>(a, sh, px, py, pz, k0, lds, lane); break;
-            case 3: eval_shell_chunk<3, PASS>(a, sh, px, py, pz, k0, lds, lane); break;
-            default: eval_shell_chunk<4, PASS>(a, sh, px, py, pz, k0, lds, lane); break;
-            }
-        }
-        __syncthreads();   // the block is one wave: orders the LDS writes before the reads
-        // contraction
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                const float* src = lds + s * KC * LDS_LD + nt * 16 + n16;
-                if constexpr (PREC == OKMI_PREC_F32) {
-#pragma unroll
-                    for (int ks = 0; ks < KC / 4; ++ks) {
-                        const float b = src[(4 * ks + g) * LDS_LD];
-#pragma unroll
-                        for (int mt = 0; mt < MT; ++mt)
-                            acc[mt][nt][s] = __builtin_amdgcn_mfma_f32_16x16x4f32(cf[mt][ks], b, acc[mt][nt][s], 0, 0, 0);
-                    }
-                } else {
-                    float bx[8];
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) bx[j] = src[(8 * g + j) * LDS_LD];
-                    Split<PREC> bs;
-                    split8<PREC>(bx, bs);
-#pragma unroll
-                    for (int mt = 0; mt < MT; ++mt) acc[mt][nt][s] = mfma_split<PREC>(ca[mt], bs, acc[mt][nt][s]);
+        for (int e = cw.x; e < cw.x + cw.y; ++e) {
+            const int w = cptr(a.work)[e];
+            if ((w >> 24) != wave) continue;
+            const ShellRec sh = load_shell(a.shells, w & 0xffffff);
+            if (box.reaches(sh)) {
+                switch (sh.l) {
+                case 0: eval_shell_chunk<0, PASS>(a, sh, px, py, pz, k0, L, lane); break;
+                case 1: eval_shell_chunk<1, PASS>(a, sh, px, py, pz, k0, L, lane); break;
+                case 2: eval_shell_chunk<2, PASS>(a, sh, px, py, pz, k0, L, lane); break;
+                case 3: eval_shell_chunk<3, PASS>(a, sh, px, py, pz, k0, L, lane); break;
+                default: eval_shell_chunk<4, PASS>(a, sh, px, py, pz, k0, L, lane); break;
                 }
+            } else {
+                const int rlo = max(0, k0 - sh.row0), rhi = min(sh.nrow, k0 + KC - sh.row0);
+                for (int s = 0; s < NS; ++s)
+                    for (int r = rlo; r < rhi; ++r) L[s * KC * LDS_LD + (sh.row0 - k0 + r) * LDS_LD + lane] = 0.f;
             }
         }
+        // All rows of buffer `buf` written; the other buffer is free because
+        // every wave has passed the previous barrier after reading it.
         __syncthreads();
+        // contraction of this wave's 16-point column
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const float* src = L + s * KC * LDS_LD + wave * 16 + n16;
+            if constexpr (PREC == OKMI_PREC_F32) {
+#pragma unroll
+                for (int ks = 0; ks < KC / 4; ++ks) {
+                    const float b = src[(4 * ks + g) * LDS_LD];
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                        acc[mt][s] = __builtin_amdgcn_mfma_f32_16x16x4f32(cf[mt][ks], b, acc[mt][s], 0, 0, 0);
+                }
+            } else {
+                float bx[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) bx[j] = src[(8 * g + j) * LDS_LD];
+                Split<PREC> bs;
+                split8<PREC>(bx, bs);
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) acc[mt][s] = mfma_split<PREC>(ca[mt], bs, acc[mt][s]);
+            }
+        }
+        buf ^= 1;
     }
 
-    // epilogue: lane holds Phi[m = 16 mt + 4 g + i][point = 16 nt + n16]
+    // epilogue: lane holds Phi[m = 16 mt + 4 g + i][tile point 16 wave + n16]
+    double r0 = 0.0, t[NS > 1 ? NS - 1 : 1], u[6];
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-        double r0 = 0.0, t[NS > 1 ? NS - 1 : 1], u[6];
+    for (int s = 0; s < (NS > 1 ? NS - 1 : 1); ++s) t[s] = 0.0;
 #pragma unroll
-        for (int s = 0; s < (NS > 1 ? NS - 1 : 1); ++s) t[s] = 0.0;
+    for (int q = 0; q < 6; ++q) u[q] = 0.0;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) u[q] = 0.0;
+    for (int mt = 0; mt < MT; ++mt) {
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
+        for (int i = 0; i < 4; ++i) {
+            const double w = a.occ[mt * 16 + 4 * g + i];
+            const double f0 = acc[mt][0][i];
+            if constexpr (PASS <= 1) r0 += w * (f0 * f0);
+            if constexpr (NS > 1) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const double w = a.occ[mt * 16 + 4 * g + i];
-                const double f0 = acc[mt][nt][0][i];
-                if constexpr (PASS <= 1) r0 += w * (f0 * f0);
-                if constexpr (NS > 1) {
-#pragma unroll
-                    for (int s = 1; s < NS; ++s) t[s - 1] += 2.0 * w * (f0 * (double)acc[mt][nt][s][i]);
-                }
-                if constexpr (PASS == 1) {
-                    const double fx = acc[mt][nt][1][i], fy = acc[mt][nt][2][i], fz = acc[mt][nt][3][i];
-                    u[0] += 2.0 * w * (fx * fx);
-                    u[1] += 2.0 * w * (fy * fy);
-                    u[2] += 2.0 * w * (fz * fz);
-                    u[3] += 2.0 * w * (fx * fy);
-                    u[4] += 2.0 * w * (fx * fz);
-                    u[5] += 2.0 * w * (fy * fz);
-                }
+                for (int s = 1; s < NS; ++s) t[s - 1] += 2.0 * w * (f0 * (double)acc[mt][s][i]);
+            }
+            if constexpr (PASS == 1) {
+                const double fx = acc[mt][1][i], fy = acc[mt][2][i], fz = acc[mt][3][i];
+                u[0] += 2.0 * w * (fx * fx);
+                u[1] += 2.0 * w * (fy * fy);
+                u[2] += 2.0 * w * (fz * fz);
+                u[3] += 2.0 * w * (fx * fy);
+                u[4] += 2.0 * w * (fx * fz);
+                u[5] += 2.0 * w * (fy * fz);
             }
         }
-        r0 += __shfl_xor(r0, 16);
-        r0 += __shfl_xor(r0, 32);
+    }
+    r0 += __shfl_xor(r0, 16);
+    r0 += __shfl_xor(r0, 32);
 #pragma unroll
-        for (int s = 0; s < NS - 1; ++s) {
-            t[s] += __shfl_xor(t[s], 16);
-            t[s] += __shfl_xor(t[s], 32);
+    for (int s = 0; s < NS - 1; ++s) {
+        t[s] += __shfl_xor(t[s], 16);
+        t[s] += __shfl_xor(t[s], 32);
+    }
+    if constexpr (PASS == 1) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            u[q] += __shfl_xor(u[q], 16);
+            u[q] += __shfl_xor(u[q], 32);
+        }
+    }
+    int64_t p;
+    bool act;
+    double qx, qy, qz;
+    tile_point(a, wave * 16 + n16, p, act, qx, qy, qz);
+    if (g == 0 && act) {
+        if constexpr (PASS <= 1) a.rho[p] = r0;
+#pragma unroll
+        for (int s = 1; s < NS; ++s) {
+            const int code = PassCodes<PASS>::c[s];
+            if (a.need_mask >> code & 1) a.term[code * a.ld_term + p] = t[s - 1];
         }
         if constexpr (PASS == 1) {
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                u[q] += __shfl_xor(u[q], 16);
-                u[q] += __shfl_xor(u[q], 32);
-            }
-        }
-        const int64_t p = tile0 + nt * 16 + n16;
-        if (g == 0 && p < a.np) {
-            if constexpr (PASS <= 1) a.rho[p] = r0;
-#pragma unroll
-            for (int s = 1; s < NS; ++s) {
-                const int code = PassCodes<PASS>::c[s];
-                if (a.need_mask >> code & 1) a.term[code * a.ld_term + p] = t[s - 1];
-            }
-            if constexpr (PASS == 1) {
-#pragma unroll
-                for (int q = 0; q < 6; ++q)
-                    if (a.need_mask >> (4 + q) & 1) a.term2[(4 + q) * a.ld_term + p] = u[q];
-            }
+            for (int q = 0; q < 6; ++q)
+                if (a.need_mask >> (4 + q) & 1) a.term2[(4 + q) * a.ld_term + p] = u[q];
         }
     }
 }
@@ -315,7 +436,8 @@ __global__ void k_pad_occ(const double* occ, int nmo, int mpad, double* out)
 template <int PREC, int PASS>
 int launch_mt(const FusedArgs& a, int mt, hipStream_t st)
 {
-    dim3 blk(PTS), grd((unsigned)((a.np + PTS - 1) / PTS));
+    const int64_t ntile = a.brick ? a.nbx * a.nby * a.nbz : (a.np + PTS - 1) / PTS;
+    dim3 blk(256), grd((unsigned)ntile);
     OkmiProfScope prof("rho_fused", st);
     switch (mt) {
     case 1: hipLaunchKernelGGL((k_rho_fused<PREC, PASS, 1>), grd, blk, 0, st, a); return 0;
@@ -372,29 +494,43 @@ size_t fused_fixed_bytes(const OkmiBasis* b, int nmo, int prec)
     return align256(elem * mpad * kpad) + align256(mpad * 8);
 }
 
+// Per chunk of KC rows: the shells overlapping it, each assigned to one of
+// the four waves of a workgroup so that the estimated evaluation cost
+// (primitives, Cartesian components, output rows) is balanced.
 int prepare_fused_tables(OkmiBasis* b)
 {
-    if (b->d_T32) return 0;
-    std::vector<float> t32(b->ntab);
-    for (int i = 0; i < b->ntab; ++i) t32[i] = (float)b->h_T[i];
+    if (b->d_chunks) return 0;
     const int nchunk = (b->nao + KC - 1) / KC;
     std::vector<int2> ch(nchunk);
+    std::vector<int> work;
     int s = 0;
     for (int c = 0; c < nchunk; ++c) {
         const int r0 = c * KC, r1 = std::min(b->nao, r0 + KC) - 1;
         while (s < b->nshell && b->h_shells[s].row0 + b->h_shells[s].nrow <= r0) ++s;
         int e = s;
         while (e + 1 < b->nshell && b->h_shells[e + 1].row0 <= r1) ++e;
-        ch[c] = make_int2(s, e);
+        std::vector<std::pair<int, int>> cost;
+        for (int i = s; i <= e; ++i) {
+            const ShellRec& sh = b->h_shells[i];
+            cost.push_back({16 * sh.nprim + 8 * (ncart(sh.l) + sh.nrow), i});
+        }
+        std::sort(cost.begin(), cost.end(), [](auto& x, auto& y) { return x.first > y.first; });
+        int load[4] = {0, 0, 0, 0};
+        ch[c] = make_int2((int)work.size(), (int)cost.size());
+        for (auto& cs : cost) {
+            const int w = (int)(std::min_element(load, load + 4) - load);
+            load[w] += cs.first;
+            work.push_back(cs.second | (w << 24));
+        }
     }
-    hipError_t e = hipMalloc((void**)&b->d_T32, sizeof(float) * std::max(1, b->ntab));
-    if (e != hipSuccess) return okmi_hip_fail(e, "rho_fused: transform table");
-    e = hipMemcpy(b->d_T32, t32.data(), sizeof(float) * b->ntab, hipMemcpyHostToDevice);
-    if (e != hipSuccess) return okmi_hip_fail(e, "rho_fused: transform table");
-    e = hipMalloc((void**)&b->d_chunks, sizeof(int2) * nchunk);
+    hipError_t e = hipMalloc((void**)&b->d_chunks, sizeof(int2) * nchunk);
     if (e != hipSuccess) return okmi_hip_fail(e, "rho_fused: chunk map");
     e = hipMemcpy(b->d_chunks, ch.data(), sizeof(int2) * nchunk, hipMemcpyHostToDevice);
     if (e != hipSuccess) return okmi_hip_fail(e, "rho_fused: chunk map");
+    e = hipMalloc((void**)&b->d_work, sizeof(int) * work.size());
+    if (e != hipSuccess) return okmi_hip_fail(e, "rho_fused: work list");
+    e = hipMemcpy(b->d_work, work.data(), sizeof(int) * work.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return okmi_hip_fail(e, "rho_fused: work list");
     b->nchunk = nchunk;
     return 0;
 }
@@ -432,9 +568,10 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
         return 1;
     }
     int64_t chunk = (int64_t)((ws_bytes - fixed - 256) / (20 * 8));
+    // a whole regular grid whose term buffers fit is tiled in 4x4x4 bricks
+    const bool brick = g.regular && p0 == 0 && np == g.nx * g.ny * g.nz && chunk >= np;
     if (chunk >= np) chunk = np;
     else chunk &= ~(int64_t)63;
-
     const int64_t mpad = mt * 16, kpad = (int64_t)b->nchunk * KC;
     char* w = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
     void* cpl = w;
@@ -461,10 +598,12 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
     const bool p1 = (need & ~1) != 0, p2 = (need & (7 << 4)) != 0, p3 = (need & (7 << 7)) != 0;
 
     FusedArgs a{};
-    a.shells = b->d_shells; a.prims = b->d_prims; a.T = b->d_T32; a.chunks = b->d_chunks;
+    a.shells = b->d_shells; a.prims = b->d_prims; a.T = b->d_T; a.chunks = b->d_chunks; a.work = b->d_work;
     a.grid = g; a.nao = b->nao; a.nchunk = b->nchunk;
     a.C = cpl; a.kpad = kpad; a.mpad = mpad; a.occ = occp;
     a.term = term; a.term2 = term2; a.ld_term = chunk; a.need_mask = need;
+    a.brick = brick;
+    a.nbx = (g.nx + 3) / 4; a.nby = (g.ny + 3) / 4; a.nbz = (g.nz + 3) / 4;
     for (int64_t c0 = 0; c0 < np; c0 += chunk) {
         const int64_t n = np - c0 < chunk ? np - c0 : chunk;
         a.p0 = p0 + c0;
