@@ -287,7 +287,7 @@ extern "C" int okmi_basis_create(int ncont, const int* ncart_per_cont, const int
         for (int p = 0; p < np; ++p) {
             const double a = prim_coeffs[2 * (prow + p)], cc = prim_coeffs[2 * (prow + p) + 1];
             const double nh = is_normalized ? 1.0 : pref * std::pow(2.0, l) * std::pow(a, (2.0 * l + 3.0) / 4.0);
-            prims.push_back(PrimRec{a, cc * nh});
+            prims.push_back(PrimRec{a, cc * nh, 0.0});
             craw.push_back(cc);
             pl.push_back(l);
             al[p] = a;
@@ -298,6 +298,10 @@ extern "C" int okmi_basis_create(int ncont, const int* ncart_per_cont, const int
         }
         const double rc = shell_rcut(l, al.data(), dd.data(), np);
         s.rcut2 = rc >= 1e150 ? 1e300 : rc * rc;
+        for (int p = 0; p < np; ++p) {
+            const double rp = shell_rcut(l, &al[p], &dd[p], 1);
+            prims[prims.size() - np + p].rcut2 = rp >= 1e150 ? 1e300 : rp * rp;
+        }
         if (l > max_l) max_l = l;
         crow += nc; prow += np; orow += nrow;
     }

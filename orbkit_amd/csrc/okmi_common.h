@@ -36,6 +36,7 @@ enum { SHELL_T_GENERAL = 0, SHELL_T_SPHERICAL = 1, SHELL_T_CARTESIAN = 2 };
 struct PrimRec {
     double alpha;
     double d;
+    double rcut2;        // beyond this squared distance this primitive's terms are below 1e-40
 };
 
 // Points: either a vector grid (x[i], y[i], z[i]) or a regular grid given by
@@ -89,6 +90,7 @@ __device__ __forceinline__ PrimRec load_prim(const PrimRec* base, int i)
     PrimRec r;
     r.alpha = p->alpha;
     r.d = p->d;
+    r.rcut2 = p->rcut2;
     return r;
 }
 

@@ -119,7 +119,8 @@ __device__ __forceinline__ double wave_max(double v)
 
 struct Box {
     double lo[3], hi[3];
-    __device__ __forceinline__ bool reaches(const ShellRec sh) const
+    // squared distance from a point to the box (wave-uniform)
+    __device__ __forceinline__ double dist2(const ShellRec& sh) const
     {
         const double c[3] = {sh.cx, sh.cy, sh.cz};
         double d2 = 0.0;
@@ -128,8 +129,9 @@ struct Box {
             const double d = fmax(fmax(lo[k] - c[k], c[k] - hi[k]), 0.0);
             d2 = fma(d, d, d2);
         }
-        return d2 < sh.rcut2;
+        return d2;
     }
+    __device__ __forceinline__ bool reaches(const ShellRec sh) const { return dist2(sh) < sh.rcut2; }
 };
 
 template <int L, int R>
@@ -157,8 +159,8 @@ __device__ __forceinline__ void cart_rows(const double* c, float* dst, int rlo, 
 }
 
 template <int L, int PASS>
-__device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const ShellRec& sh, double px, double py,
-                                                 double pz, int k0, float* lds, int lane)
+__device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const ShellRec& sh, double d2, double px,
+                                                 double py, double pz, int k0, float* lds, int lane)
 {
     constexpr int NS = PassCodes<PASS>::n;
     constexpr bool need1 = PASS == 1 || PASS == 2;
@@ -168,6 +170,7 @@ __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const Shell
     double S0 = 0.0, S1 = 0.0, S2 = 0.0;
     for (int p = 0; p < sh.nprim; ++p) {
         const PrimRec pr = load_prim(a.prims, sh.prim0 + p);
+        if (d2 >= pr.rcut2) continue;   // below 1e-40 on the whole tile
         const double de = pr.d * gauss_f64_fast(pr.alpha * rr);
         S0 += de;
         if constexpr (need1) S1 = fma(-2.0 * pr.alpha, de, S1);
@@ -214,21 +217,44 @@ struct Split {
     bf8 p[PREC == OKMI_PREC_BF16X6 ? 3 : 2];
 };
 
+// x = hi + mid (+ lo) exactly up to the last part: hi and mid keep the top 16
+// bits of x and of the residual x - hi (a truncating split, so every
+// residual is exact in FP32); the last part is the residual rounded to
+// nearest.  Two bf16 of consecutive k share a dword (low half: even k).
 template <int PREC>
 __device__ __forceinline__ void split8(const float (&x)[8], Split<PREC>& o)
 {
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    u4 h, m, lo;
+    float r1[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        const __bf16 h = (__bf16)x[j];
-        const float r1 = x[j] - (float)h;
-        const __bf16 m = (__bf16)r1;
-        o.p[0][j] = h;
-        o.p[1][j] = m;
-        if constexpr (PREC == OKMI_PREC_BF16X6) {
-            const float r2 = r1 - (float)m;
-            o.p[2][j] = (__bf16)r2;
+        const unsigned xb = __float_as_uint(x[j]);
+        r1[j] = x[j] - __uint_as_float(xb & 0xffff0000u);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        h[i] = __builtin_amdgcn_perm(__float_as_uint(x[2 * i + 1]), __float_as_uint(x[2 * i]), 0x07060302u);
+    if constexpr (PREC == OKMI_PREC_BF16X6) {
+        float r2[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) r2[j] = r1[j] - __uint_as_float(__float_as_uint(r1[j]) & 0xffff0000u);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            m[i] = __builtin_amdgcn_perm(__float_as_uint(r1[2 * i + 1]), __float_as_uint(r1[2 * i]), 0x07060302u);
+            const __bf16 a = (__bf16)r2[2 * i], b = (__bf16)r2[2 * i + 1];
+            lo[i] = (unsigned)__builtin_bit_cast(unsigned short, a) | ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
+        }
+        o.p[2] = __builtin_bit_cast(bf8, lo);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const __bf16 a = (__bf16)r1[2 * i], b = (__bf16)r1[2 * i + 1];
+            m[i] = (unsigned)__builtin_bit_cast(unsigned short, a) | ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
         }
     }
+    o.p[0] = __builtin_bit_cast(bf8, h);
+    o.p[1] = __builtin_bit_cast(bf8, m);
 }
 
 template <int PREC>
@@ -252,7 +278,10 @@ __global__ __launch_bounds__(256, 2) void k_rho_fused(FusedArgs a)
     constexpr int NS = PassCodes<PASS>::n;
     constexpr int BUF = NS * KC * LDS_LD;
     __shared__ float lds[2 * BUF];
+    __shared__ int next[2];   // per buffer: next work entry to evaluate
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (threadIdx.x < 2) next[threadIdx.x] = 0;
+    __syncthreads();
     int64_t pidx;
     bool pact;
     double px, py, pz;
@@ -296,22 +325,30 @@ __global__ __launch_bounds__(256, 2) void k_rho_fused(FusedArgs a)
                 for (int q = 0; q < NP; ++q)
                     ca[mt].p[q] = *(const bf8*)(C + ((int64_t)q * a.mpad + mt * 16 + n16) * a.kpad + k0 + 8 * g);
         }
-        // this wave's share of the AO rows [k0, k0+KC)
+        // AO rows [k0, k0+KC): the waves take the chunk's shells one at a
+        // time from a shared counter (costliest first), so that screening
+        // cannot leave one wave with most of the work
         if (wave == 0 && k0 + KC > a.nao) {
             for (int s = 0; s < NS; ++s)
                 for (int r = a.nao - k0; r < KC; ++r) L[s * KC * LDS_LD + r * LDS_LD + lane] = 0.f;
         }
-        for (int e = cw.x; e < cw.x + cw.y; ++e) {
-            const int w = cptr(a.work)[e];
-            if ((w >> 24) != wave) continue;
-            const ShellRec sh = load_shell(a.shells, w & 0xffffff);
-            if (box.reaches(sh)) {
+        // the other buffer's counter: every wave finished taking from it
+        // before the previous barrier, the next chunk reads it after the next
+        if (threadIdx.x == 0) next[buf ^ 1] = 0;
+        for (;;) {
+            int e = 0;
+            if (lane == 0) e = atomicAdd(&next[buf], 1);
+            e = __builtin_amdgcn_readfirstlane(__shfl(e, 0));
+            if (e >= cw.y) break;
+            const ShellRec sh = load_shell(a.shells, cptr(a.work)[cw.x + e] & 0xffffff);
+            const double d2 = box.dist2(sh);
+            if (d2 < sh.rcut2) {
                 switch (sh.l) {
-                case 0: eval_shell_chunk<0, PASS>(a, sh, px, py, pz, k0, L, lane); break;
-                case 1: eval_shell_chunk<1, PASS>(a, sh, px, py, pz, k0, L, lane); break;
-                case 2: eval_shell_chunk<2, PASS>(a, sh, px, py, pz, k0, L, lane); break;
-                case 3: eval_shell_chunk<3, PASS>(a, sh, px, py, pz, k0, L, lane); break;
-                default: eval_shell_chunk<4, PASS>(a, sh, px, py, pz, k0, L, lane); break;
+                case 0: eval_shell_chunk<0, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
+                case 1: eval_shell_chunk<1, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
+                case 2: eval_shell_chunk<2, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
+                case 3: eval_shell_chunk<3, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
+                default: eval_shell_chunk<4, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
                 }
             } else {
                 const int rlo = max(0, k0 - sh.row0), rhi = min(sh.nrow, k0 + KC - sh.row0);
