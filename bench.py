@@ -35,17 +35,32 @@ def parse():
   ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
   ap.add_argument('--cpu-threads', type=int, default=None)
   ap.add_argument('--errors', action='store_true', help='also report FP64-relative errors')
+  ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
+                  help='before the timed steps, copy MIB MiB device-to-device once (a kernel of known '
+                       'traffic for scaling FETCH_SIZE / WRITE_SIZE counters)')
   return ap.parse_args()
 
 
-def cpu_baseline(qc, axes, threads, slice_length):
+def measured_traffic(kernel, workload, precision):
+  '''HBM bytes per launch from the committed counter summary (FETCH_SIZE +
+  WRITE_SIZE, scaled by a plain copy of known size), or None.'''
+  path = os.path.join(ROOT, 'profiles', 'traffic.json')
+  if not os.path.exists(path):
+    return None
+  with open(path) as f:
+    t = json.load(f)
+  v = t.get('%s/%s/%s' % (kernel, workload, precision))
+  return None if v is None else v['bytes_per_launch']
+
+
+def cpu_baseline(qc, axes, threads, slice_length, slices_per_thread=4):
   '''orbkit's algorithm (C restatement in oracle/, OpenMP over slices) on a
-  bounded sample: one slice of slice_length points per thread, taken as a
-  contiguous run of the flattened grid from its middle.'''
+  bounded sample: slices_per_thread slices of slice_length points per
+  thread, a contiguous run of the flattened grid from its middle.'''
   from oracle import core_oracle
   X, Y, Z = numpy.meshgrid(*axes, indexing='ij')
   P = numpy.stack([X.ravel(), Y.ravel(), Z.ravel()])
-  n = int(threads * slice_length)
+  n = int(threads * slice_length * slices_per_thread)
   a = max(0, P.shape[1] // 2 - n // 2)
   x, y, z = P[0, a:a + n].copy(), P[1, a:a + n].copy(), P[2, a:a + n].copy()
   core_oracle.rho_slices_c(qc, x[:64], y[:64], z[:64], [1, 2, 3], slice_length=64, nthreads=1)
@@ -53,7 +68,8 @@ def cpu_baseline(qc, axes, threads, slice_length):
   core_oracle.rho_slices_c(qc, x, y, z, [1, 2, 3], slice_length=slice_length, nthreads=threads)
   dt = time.perf_counter() - t
   return {'value': n / dt, 'unit': 'grid points/s', 'cores': threads, 'kind': 'port',
-          'sample': '%d points (%d slices of %d, rho + grad rho, FP64) in %.1f s' % (n, threads, slice_length, dt)}
+          'sample': '%d of the %d grid points (%d slices of %d, rho + grad rho, FP64) in %.1f s'
+                    % (n, P.shape[1], n // slice_length, slice_length, dt)}
 
 
 def main():
@@ -101,6 +117,12 @@ def main():
     kernels = ['rho_fused']
     result = plan.result
 
+  if args.calibrate_copy:
+    src = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device=dg.x.device)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
   for _ in range(args.warmup):
     step()
   torch.cuda.synchronize()
@@ -126,7 +148,8 @@ def main():
     t_launch = ms / 1e3 / args.steps
     achieved = gemm_flops / t_launch / 1e12
     roof = {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_TFLOPS['f64'], 'unit': 'TFLOP/s',
-            'frac': achieved / PEAK_TFLOPS['f64'], 'traffic': None, 'kernel': 'k_rho_f64',
+            'frac': achieved / PEAK_TFLOPS['f64'],
+            'traffic': measured_traffic('k_rho_f64', args.workload, args.precision), 'kernel': 'k_rho_f64',
             'launch_ms': t_launch * 1e3, 'other_kernels_ms': {'k_ao_eval': times['ao_eval'][1] / args.steps}}
     dtype = 'f64'
   else:
@@ -137,7 +160,8 @@ def main():
     unit_dtype = 'f32' if args.precision == 'f32' else 'bf16'
     peak = PEAK_TFLOPS[unit_dtype] / nprod
     roof = {'bound': 'mfma', 'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s', 'frac': achieved / peak,
-            'traffic': None, 'kernel': 'k_rho_fused', 'launch_ms': t_launch * 1e3,
+            'traffic': measured_traffic('k_rho_fused', args.workload, args.precision), 'kernel': 'k_rho_fused',
+            'launch_ms': t_launch * 1e3,
             'executed_tflops': achieved * nprod, 'mfma_dtype': unit_dtype, 'products': nprod}
     dtype = {'bf16x6': 'bf16x6 (fp32 acc, fp64 epilogue)', 'bf16x3': 'bf16x3 (fp32 acc, fp64 epilogue)',
              'f32': 'f32'}[args.precision]
@@ -164,7 +188,7 @@ def main():
         'electrons_f64': float(numpy.sum(rr) * numpy.prod([a[1] - a[0] for a in axes]))}
   if not args.no_cpu:
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    out['cpu_baseline'] = cpu_baseline(qc, axes, threads, 2000)
+    out['cpu_baseline'] = cpu_baseline(qc, axes, threads, int(info['slice_length']))
   print(json.dumps(out))
 
 
