@@ -124,7 +124,9 @@ int okmi_rho(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np,
              double* d_mo_norm, void* d_ws, size_t ws_bytes, void* stream);
 
 /*
- * Workspace bytes okmi_rho_fused needs to process `chunk` points per pass.
+ * Workspace bytes okmi_rho_fused needs to process `chunk` points per pass
+ * (more than 64 MOs: sized for 7 AO sets; requests with mixed second
+ * derivatives then run in shorter passes).
  */
 size_t okmi_rho_fused_workspace_bytes(const okmi_basis* b, int nmo, int precision, int64_t chunk);
 
@@ -134,8 +136,11 @@ size_t okmi_rho_fused_workspace_bytes(const okmi_basis* b, int nmo, int precisio
  * reduction, FP32 values) and contracted with C on the matrix cores
  * (precision OKMI_PREC_F32: FP32 MFMA; OKMI_PREC_BF16X3 / _BF16X6: C and AO
  * split into two / three bf16 parts, 3 / 6 bf16 MFMA products, FP32
- * accumulation); the density terms are assembled in FP64.  MOs are never
- * written to memory.  Up to 64 MOs.  Outputs as okmi_rho.
+ * accumulation); the density terms are assembled in FP64.  Up to 64 MOs
+ * are handled in one fused launch per pass (MOs never written to memory);
+ * with more MOs the AO sets of each chunk of points are staged in HBM
+ * (FP32) and contracted by an MFMA GEMM whose epilogue forms the density
+ * terms of 128 MOs at a time (same precisions).  Outputs as okmi_rho.
  * Replaces slice_rho's density branch (orbkit/core.py:179-308) with the
  * reduced-precision contraction; okmi_rho is the FP64 form.
  */
@@ -146,10 +151,65 @@ int okmi_rho_fused(okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np,
                    void* d_ws, size_t ws_bytes, void* stream);
 
 /*
+ * Workspace bytes okmi_rho_backward needs to process `chunk` points per pass.
+ */
+size_t okmi_rho_backward_workspace_bytes(const okmi_basis* b, int nmo, int64_t chunk);
+
+/*
+ * Gradient of a scalar loss L(rho) through the FP64 density of okmi_rho
+ * (no derivative outputs) on points [p0, p0+np), given d_grad[p] = dL/drho
+ * at point p0+p.  ACCUMULATES (callers zero the outputs first, so that the
+ * slices of one grid can be processed by successive calls):
+ *   d_dc[i][mu]   += dL/dC[i][mu]            (nmo x nao, the layout of d_c)
+ *   d_dalpha[k]   += dL/dalpha_k             (one per primitive, in the order
+ *                                             of prim_coeffs at basis_create)
+ *   d_dgeo[a][d]  += dL/dR[a][d]             (natoms x 3, nuclear positions)
+ * Row factors (cart_scale, the transform, contraction coefficients) are
+ * held fixed.  FP64 throughout; AO values are recomputed per chunk.
+ * orbkit has no derivative code: this differentiates the forward path of
+ * rho_compute (orbkit/core.py:529-607) as evaluated by okmi_rho.
+ */
+int okmi_rho_backward(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np,
+                      const double* d_c, const double* d_occ, int nmo, const double* d_grad,
+                      double* d_dc, double* d_dalpha, double* d_dgeo,
+                      void* d_ws, size_t ws_bytes, void* stream);
+
+/*
+ * Many molecules in one launch: density (and gradient) of every molecule on
+ * its own grid with the fused reduced-precision kernel (okmi_rho_fused's
+ * arithmetic; up to 64 MOs per molecule).  Inputs are bound at creation:
+ * per molecule a basis, a whole grid (regular or vector), device pointers
+ * to C (nmo x nao, FP64) and occupations, and the outputs d_rho[m] (npts)
+ * and, if d_grad is non-NULL and d_grad[m] non-NULL, d_grad[m] (3 x npts,
+ * rows x, y, z).  okmi_batch_launch re-reads C and the occupations from
+ * those pointers on every launch.  order_by_cost: process molecules
+ * costliest first (1) or in the given order (0).
+ * orbkit evaluates one molecule per rho_compute call (orbkit/core.py:314);
+ * a batch is that call for many molecules at once.
+ */
+typedef struct OkmiBatch okmi_batch;
+int okmi_batch_create(int nmol, okmi_basis* const* bases, const okmi_grid* grids,
+                      const double* const* d_c, const double* const* d_occ, const int* nmo,
+                      double* const* d_rho, double* const* d_grad, int precision, int order_by_cost,
+                      okmi_batch** out);
+/* tiles of 64 points over all molecules */
+int64_t okmi_batch_tiles(const okmi_batch* b);
+/* persistent workgroup count (0: one workgroup per tile; default 0) */
+int okmi_batch_set_workgroups(okmi_batch* b, int workgroups);
+/* brick of 2^log2_x x 2^log2_y x 2^log2_z regular-grid points per tile
+ * (sum 6; default 2, 2, 2; vector grids always take 64 consecutive points) */
+int okmi_batch_set_tile_shape(okmi_batch* b, int log2_x, int log2_y, int log2_z);
+/* enqueue the whole batch on `stream`; use_graph = 1 captures the launch
+ * sequence into a HIP graph on first use (per stream; `stream` must not be
+ * the NULL stream) and replays it afterwards */
+int okmi_batch_launch(okmi_batch* b, int use_graph, void* stream);
+int okmi_batch_destroy(okmi_batch* b);
+
+/*
  * Kernel timing for measurement tools: while enabled, every kernel launch
  * records a pair of HIP events on its stream.  okmi_profile_query sums the
  * elapsed times of the launches of one kernel family ("ao_eval", "gemm_f64",
- * "rho_f64", "rho_fused") since the last okmi_profile_enable call.
+ * "rho_f64", "rho_fused", "rho_big", "rho_backward", "rho_batch") since the last okmi_profile_enable call.
  * No counterpart in the reference (orbkit has no kernel timers).
  */
 int okmi_profile_enable(int on);

@@ -41,6 +41,17 @@ SIGNATURES = {
                          c_vp, c_vp, c_i64, c_vp, c_vp, c_sz, c_vp]),
     'okmi_profile_enable': (c_int, [c_int]),
     'okmi_profile_query': (c_int, [ctypes.c_char_p, ctypes.POINTER(c_int), ctypes.POINTER(c_dbl)]),
+    'okmi_rho_backward_workspace_bytes': (c_sz, [c_vp, c_int, c_i64]),
+    'okmi_rho_backward': (c_int, [c_vp, ctypes.POINTER(OkmiGrid), c_i64, c_i64, c_vp, c_vp, c_int, c_vp, c_vp, c_vp,
+                                  c_vp, c_vp, c_sz, c_vp]),
+    'okmi_batch_create': (c_int, [c_int, ctypes.POINTER(c_vp), ctypes.POINTER(OkmiGrid), ctypes.POINTER(c_vp),
+                                  ctypes.POINTER(c_vp), P_int, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), c_int, c_int,
+                                  ctypes.POINTER(c_vp)]),
+    'okmi_batch_tiles': (c_i64, [c_vp]),
+    'okmi_batch_set_workgroups': (c_int, [c_vp, c_int]),
+    'okmi_batch_set_tile_shape': (c_int, [c_vp, c_int, c_int, c_int]),
+    'okmi_batch_launch': (c_int, [c_vp, c_int, c_vp]),
+    'okmi_batch_destroy': (c_int, [c_vp]),
     'okmi_rho_fused_workspace_bytes': (c_sz, [c_vp, c_int, c_int, c_i64]),
     'okmi_rho_fused': (c_int, [c_vp, ctypes.POINTER(OkmiGrid), c_i64, c_i64, c_vp, c_vp, c_int, P_int, c_int,
                                c_int, c_vp, c_vp, c_i64, c_vp, c_sz, c_vp]),

@@ -514,3 +514,90 @@ extern "C" int okmi_rho_fused(okmi_basis* b, const okmi_grid* g, int64_t p0, int
     return okmi_launch_rho_fused(b, to_desc(g), p0, np, d_c, d_occ, nmo, precision, drv_out, nout, d_rho, d_delta,
                                  ld_delta, d_ws, ws_bytes, (hipStream_t)stream);
 }
+
+extern "C" size_t okmi_rho_backward_workspace_bytes(const okmi_basis* b, int nmo, int64_t chunk)
+{
+    if (!b || nmo <= 0 || chunk <= 0) return 0;
+    return okmi_backward_ws_bytes(b, nmo, chunk);
+}
+
+extern "C" int okmi_rho_backward(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np, const double* d_c,
+                                 const double* d_occ, int nmo, const double* d_grad, double* d_dc, double* d_dalpha,
+                                 double* d_dgeo, void* d_ws, size_t ws_bytes, void* stream)
+{
+    g_err.clear();
+    if (!b || !d_c || !d_occ || !d_grad || !d_dc || !d_dalpha || !d_dgeo || nmo <= 0) {
+        okmi_set_error("rho_backward: invalid arguments");
+        return 1;
+    }
+    if (check_grid(g, p0, np)) return 1;
+    return okmi_launch_rho_backward(b, to_desc(g), p0, np, d_c, d_occ, nmo, d_grad, d_dc, d_dalpha, d_dgeo, d_ws,
+                                    ws_bytes, (hipStream_t)stream);
+}
+
+extern "C" int okmi_batch_create(int nmol, okmi_basis* const* bases, const okmi_grid* grids, const double* const* d_c,
+                                 const double* const* d_occ, const int* nmo, double* const* d_rho,
+                                 double* const* d_grad, int precision, int order_by_cost, okmi_batch** out)
+{
+    g_err.clear();
+    if (!out || nmol <= 0 || !bases || !grids || !d_c || !d_occ || !nmo || !d_rho) {
+        okmi_set_error("batch_create: invalid arguments");
+        return 1;
+    }
+    *out = nullptr;
+    std::vector<GridDesc> gd(nmol);
+    std::vector<int64_t> np(nmol);
+    std::vector<double*> grad(nmol, nullptr);
+    for (int m = 0; m < nmol; ++m) {
+        const okmi_grid* g = grids + m;
+        if (!bases[m] || !d_c[m] || !d_occ[m] || !d_rho[m]) {
+            okmi_set_error("batch_create: molecule %d has a null input or output", m);
+            return 1;
+        }
+        const int64_t total = g->regular ? g->nx * g->ny * g->nz : g->nx;
+        if (check_grid(g, 0, total)) return 1;
+        gd[m] = to_desc(g);
+        np[m] = total;
+        if (d_grad) grad[m] = d_grad[m];
+    }
+    return okmi_batch_build(nmol, bases, gd.data(), np.data(), d_c, d_occ, nmo, d_rho, grad.data(), precision,
+                            order_by_cost, out);
+}
+
+extern "C" int64_t okmi_batch_tiles(const okmi_batch* b) { return b ? okmi_batch_ntile(b) : 0; }
+
+extern "C" int okmi_batch_set_workgroups(okmi_batch* b, int workgroups)
+{
+    g_err.clear();
+    if (!b) {
+        okmi_set_error("batch_set_workgroups: null batch");
+        return 1;
+    }
+    return okmi_batch_set_grid_impl(b, workgroups);
+}
+
+extern "C" int okmi_batch_set_tile_shape(okmi_batch* b, int log2_x, int log2_y, int log2_z)
+{
+    g_err.clear();
+    if (!b) {
+        okmi_set_error("batch_set_tile_shape: null batch");
+        return 1;
+    }
+    return okmi_batch_set_shape_impl(b, log2_x, log2_y, log2_z);
+}
+
+extern "C" int okmi_batch_launch(okmi_batch* b, int use_graph, void* stream)
+{
+    g_err.clear();
+    if (!b) {
+        okmi_set_error("batch_launch: null batch");
+        return 1;
+    }
+    return okmi_batch_run(b, use_graph, (hipStream_t)stream);
+}
+
+extern "C" int okmi_batch_destroy(okmi_batch* b)
+{
+    okmi_batch_free(b);
+    return 0;
+}

@@ -60,6 +60,30 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
                           const double* d_occ, int nmo, int prec, const int* drv_out, int nout, double* rho,
                           double* delta, int64_t ld_delta, void* ws, size_t ws_bytes, hipStream_t st);
 
+// many-MO density: FP32 AO sets per chunk + MFMA GEMM with density epilogue
+// (S AO sets: 1, 4, 7 or 10, the smallest holding the requested codes)
+size_t okmi_big_ws_bytes(const OkmiBasis* b, int nmo, int prec, int S, int64_t np);
+int okmi_launch_rho_big(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const double* d_c,
+                        const double* d_occ, int nmo, int prec, const int* drv_out, int nout, double* rho,
+                        double* delta, int64_t ld_delta, void* ws, size_t ws_bytes, hipStream_t st);
+
+// backward of the FP64 density (accumulating dL/dC, dL/dalpha, dL/dR)
+size_t okmi_backward_ws_bytes(const OkmiBasis* b, int nmo, int64_t chunk);
+int okmi_launch_rho_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const double* d_c,
+                             const double* d_occ, int nmo, const double* d_grad, double* d_dc, double* d_dalpha,
+                             double* d_dr, void* ws, size_t ws_bytes, hipStream_t st);
+
+// many molecules in one persistent launch (rho_fused.hip)
+struct OkmiBatch;
+int okmi_batch_build(int nmol, OkmiBasis* const* bases, const GridDesc* grids, const int64_t* npts,
+                     const double* const* d_c, const double* const* d_occ, const int* nmo, double* const* d_rho,
+                     double* const* d_grad, int prec, int order_by_cost, OkmiBatch** out);
+int okmi_batch_run(OkmiBatch* b, int use_graph, hipStream_t st);
+int okmi_batch_set_grid_impl(OkmiBatch* b, int blocks);
+int okmi_batch_set_shape_impl(OkmiBatch* b, int lx, int ly, int lz);
+int64_t okmi_batch_ntile(const OkmiBatch* b);
+void okmi_batch_free(OkmiBatch* b);
+
 // Optional per-kernel timing with HIP events recorded on the launch stream
 // (okmi_profile_enable / okmi_profile_query).  Scope object around a launch.
 struct OkmiProfScope {
