@@ -1,0 +1,412 @@
+// Density for many MOs (more than the fused kernel keeps in registers).
+// Replaces slice_rho's density branch (orbkit/core.py:179-308) for large
+// NMO: per chunk of points
+//   1. k_ao_eval writes the AO sets of codes 0..S-1 in FP32 (FP64
+//      evaluation rounded once), rows coalesced: ao[s][k][p];
+//   2. k_rho_big contracts them with C on the matrix cores.  A workgroup owns
+//      128 MOs x NP points x all S sets; the K loop (AO rows, 32 per step)
+//      stages the C tile (pre-split bf16 planes or FP32) and the AO tile
+//      (FP32) in LDS, double buffered with one barrier per step.  Each wave
+//      holds 64 MOs x NP/2 points x S sets of FP32 accumulators, so the
+//      density epilogue (all sets of one MO and point in one lane) runs in
+//      FP64 without leaving registers; the 128-MO partial sums go to
+//      partial[mo tile][code][p];
+//   3. k_reduce_big sums the MO tiles in a fixed order and writes rho and
+//      orbkit's derivative rows.
+// S is 1 (rho), 4 (+ first derivatives), 7 (+ diagonal second derivatives)
+// or 10 (+ mixed second derivatives); a request is served by the smallest S
+// holding its codes and their product terms.
+#include "okmi_internal.h"
+#include "split_mfma.h"
+
+namespace {
+
+using namespace okmi_split;
+
+constexpr int MB = 128;   // MOs per workgroup
+constexpr int KS = 32;    // AO rows per K step
+
+template <int S> struct BigCfg { static constexpr int NPW = S == 1 ? 4 : S == 4 ? 2 : 1; };
+
+struct BigArgs {
+    const float* ao;       // [S][nao][ld]
+    int64_t ld, set_stride;
+    int nao;
+    int64_t np;            // points in this chunk
+    const void* C;         // [planes][mpad][kpad] bf16 or [mpad][kpad] FP32
+    int64_t mpad, kpad;
+    const double* occ;     // [mpad], zero padded
+    double* partial;       // [nmt][S][ld]
+    int nmt;
+    int64_t nblocks;
+};
+
+template <int PREC> struct Prec {
+    static constexpr int planes = PREC == OKMI_PREC_F32 ? 1 : PREC == OKMI_PREC_BF16X3 ? 2 : 3;
+};
+
+// LDS row strides chosen for conflict-free operand reads:
+//   bf16 B reads: rows 8g + j of lane group g -> stride = 2 (mod 8)
+//   F32  B reads: rows 4ks + g              -> stride = 16 (mod 64)
+template <int PREC, int NP> struct AoLd { static constexpr int v = PREC == OKMI_PREC_F32 ? NP + 16 : NP + 2; };
+constexpr int CLD_BF = KS + 8;    // bf16 elements per C row in LDS (80 bytes)
+constexpr int CLD_F = KS + 4;     // floats per C row in LDS (144 bytes)
+
+template <int PREC, int S>
+struct BigLds {
+    static constexpr int NP = 32 * BigCfg<S>::NPW;
+    static constexpr int AOLD = AoLd<PREC, NP>::v;
+    static constexpr int AO_FLOATS = S * KS * AOLD;
+    static constexpr int C_BYTES = PREC == OKMI_PREC_F32 ? MB * CLD_F * 4 : Prec<PREC>::planes * MB * CLD_BF * 2;
+    static constexpr int STAGE = AO_FLOATS * 4 + C_BYTES;          // bytes per buffer
+    static constexpr int TOTAL = 2 * STAGE;
+};
+
+// XCD-aware block order: hardware places block b on XCD b % 8; blocks that
+// share a point tile (its MO tiles) are given consecutive logical indices on
+// one XCD so its L2 serves the AO tile to all of them.
+__device__ __forceinline__ int64_t logical_block(int64_t b, int64_t nb)
+{
+    const int64_t xcd = b & 7, j = b >> 3, per = nb >> 3, rem = nb & 7;
+    return (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + j;
+}
+
+template <int PREC, int S>
+__global__ __launch_bounds__(256, 1) void k_rho_big(BigArgs a)
+{
+    using LD = BigLds<PREC, S>;
+    constexpr int NPW = BigCfg<S>::NPW, NP = LD::NP, AOLD = LD::AOLD;
+    constexpr int NPL = Prec<PREC>::planes;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+
+    const int64_t L = logical_block(blockIdx.x, a.nblocks);
+    const int mt = (int)(L % a.nmt);
+    const int64_t tn = L / a.nmt;
+    const int64_t p0 = tn * NP;
+    const int m0 = mt * MB;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int g = lane >> 4, n16 = lane & 15;
+    const int nk = (a.nao + KS - 1) / KS;
+
+    // ---- global -> register staging
+    constexpr int AO_V4 = S * KS * NP / 4 / 256;          // float4 per thread per step
+    static_assert(S * KS * NP / 4 % 256 == 0, "AO tile must split evenly over 256 threads");
+    constexpr int C_V4 = PREC == OKMI_PREC_F32 ? MB * KS / 4 / 256 : NPL * MB * KS / 8 / 256;   // 16-byte pieces
+    typedef float fl4 __attribute__((ext_vector_type(4)));
+    fl4 rao[AO_V4];
+    fl4 rc[C_V4];
+    auto gload = [&](int ks) {
+        const int k0 = ks * KS;
+#pragma unroll
+        for (int v = 0; v < AO_V4; ++v) {
+            const int e = v * 256 + threadIdx.x;          // float4 index in [S][KS][NP/4]
+            const int pc = e % (NP / 4), r = (e / (NP / 4)) % KS, s = e / (NP / 4 * KS);
+            const int k = k0 + r;
+            if (k < a.nao)
+                rao[v] = *(const fl4*)(a.ao + s * a.set_stride + (int64_t)k * a.ld + p0 + 4 * pc);
+            else
+                rao[v] = fl4{0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+        for (int v = 0; v < C_V4; ++v) {
+            const int e = v * 256 + threadIdx.x;
+            if constexpr (PREC == OKMI_PREC_F32) {
+                const int kc = e % (KS / 4), r = e / (KS / 4);
+                rc[v] = *(const fl4*)((const float*)a.C + (int64_t)(m0 + r) * a.kpad + k0 + 4 * kc);
+            } else {
+                const int kc = e % (KS / 8), r = (e / (KS / 8)) % MB, pl = e / (KS / 8 * MB);
+                rc[v] = *(const fl4*)((const __bf16*)a.C + (int64_t)pl * a.mpad * a.kpad + (int64_t)(m0 + r) * a.kpad +
+                                      k0 + 8 * kc);
+            }
+        }
+    };
+    auto sstore = [&](int buf) {
+        float* ao = (float*)(smem + buf * LD::STAGE);
+        unsigned char* cl = smem + buf * LD::STAGE + LD::AO_FLOATS * 4;
+#pragma unroll
+        for (int v = 0; v < AO_V4; ++v) {
+            const int e = v * 256 + threadIdx.x;
+            const int pc = e % (NP / 4), r = (e / (NP / 4)) % KS, s = e / (NP / 4 * KS);
+            float* d = ao + (s * KS + r) * AOLD + 4 * pc;   // 8-byte aligned (AOLD even)
+            *(float2*)d = float2{rao[v][0], rao[v][1]};
+            *(float2*)(d + 2) = float2{rao[v][2], rao[v][3]};
+        }
+#pragma unroll
+        for (int v = 0; v < C_V4; ++v) {
+            const int e = v * 256 + threadIdx.x;
+            if constexpr (PREC == OKMI_PREC_F32) {
+                const int kc = e % (KS / 4), r = e / (KS / 4);
+                *(fl4*)((float*)cl + r * CLD_F + 4 * kc) = rc[v];
+            } else {
+                const int kc = e % (KS / 8), r = (e / (KS / 8)) % MB, pl = e / (KS / 8 * MB);
+                *(fl4*)((__bf16*)cl + (pl * MB + r) * CLD_BF + 8 * kc) = rc[v];
+            }
+        }
+    };
+
+    f4 acc[4][NPW][S];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int c = 0; c < NPW; ++c)
+#pragma unroll
+            for (int s = 0; s < S; ++s) acc[rb][c][s] = f4{0.f, 0.f, 0.f, 0.f};
+
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int ks = 0; ks < nk; ++ks) {
+        if (ks + 1 < nk) gload(ks + 1);
+        const float* ao = (const float*)(smem + buf * LD::STAGE);
+        const unsigned char* cl = smem + buf * LD::STAGE + LD::AO_FLOATS * 4;
+        if constexpr (PREC == OKMI_PREC_F32) {
+#pragma unroll
+            for (int kk = 0; kk < KS / 4; ++kk) {
+                float av[4];
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb)
+                    av[rb] = ((const float*)cl)[(64 * wm + 16 * rb + n16) * CLD_F + 4 * kk + g];
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+#pragma unroll
+                    for (int c = 0; c < NPW; ++c) {
+                        const float bv = ao[(s * KS + 4 * kk + g) * AOLD + (wn * NPW + c) * 16 + n16];
+#pragma unroll
+                        for (int rb = 0; rb < 4; ++rb)
+                            acc[rb][c][s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rb], bv, acc[rb][c][s], 0, 0, 0);
+                    }
+            }
+        } else {
+            Split<PREC> A[4];
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int pl = 0; pl < NPL; ++pl)
+                    A[rb].p[pl] = *(const bf8*)((const __bf16*)cl + (pl * MB + 64 * wm + 16 * rb + n16) * CLD_BF + 8 * g);
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+                for (int c = 0; c < NPW; ++c) {
+                    float bx[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) bx[j] = ao[(s * KS + 8 * g + j) * AOLD + (wn * NPW + c) * 16 + n16];
+                    Split<PREC> B;
+                    split8<PREC>(bx, B);
+#pragma unroll
+                    for (int rb = 0; rb < 4; ++rb) acc[rb][c][s] = mfma_split<PREC>(A[rb], B, acc[rb][c][s]);
+                }
+        }
+        if (ks + 1 < nk) sstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+
+    // ---- epilogue: lane holds Phi_s[m = m0 + 64 wm + 16 rb + 4 g + i][p0 + (wn NPW + c) 16 + n16]
+    double* red = (double*)smem;   // [2][S][NP], reused after the last barrier
+#pragma unroll
+    for (int c = 0; c < NPW; ++c) {
+        double v[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) v[s] = 0.0;
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double w = a.occ[m0 + 64 * wm + 16 * rb + 4 * g + i];
+                const double f0 = acc[rb][c][0][i];
+                v[0] += w * (f0 * f0);
+#pragma unroll
+                for (int s = 1; s < S; ++s) {
+                    double t = f0 * (double)acc[rb][c][s][i];
+                    if (s >= 4 && s <= 6) {
+                        const double fa = acc[rb][c][s - 3][i];
+                        t += fa * fa;
+                    } else if (s >= 7) {
+                        const double fa = acc[rb][c][s == 9 ? 2 : 1][i], fb = acc[rb][c][s == 7 ? 2 : 3][i];
+                        t += fa * fb;
+                    }
+                    v[s] += 2.0 * w * t;
+                }
+            }
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            v[s] += __shfl_xor(v[s], 16);
+            v[s] += __shfl_xor(v[s], 32);
+        }
+        if (g == 0) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) red[(wm * S + s) * NP + (wn * NPW + c) * 16 + n16] = v[s];
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < S * NP; e += 256) {
+        const int s = e / NP, pt = e % NP;
+        const int64_t p = p0 + pt;
+        if (p < a.np)
+            a.partial[((int64_t)mt * S + s) * a.ld + p] = red[s * NP + pt] + red[(S + s) * NP + pt];
+    }
+}
+
+__global__ void k_pad_occ_big(const double* occ, int nmo, int mpad, double* out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < mpad) out[i] = i < nmo ? occ[i] : 0.0;
+}
+
+struct OutCodes {
+    int n;
+    int c[16];
+};
+
+// rho[p] = sum_mt partial[mt][0][p]; delta[r][p] = sum_mt partial[mt][c_r][p]
+// (code 0: 2 rho, orbkit/core.py:299-300 with drv None)
+__global__ void k_reduce_big(const double* partial, int nmt, int S, int64_t ld, int64_t np, OutCodes oc,
+                             double* rho, double* delta, int64_t ld_delta)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    double r = 0.0;
+    for (int m = 0; m < nmt; ++m) r += partial[(int64_t)m * S * ld + p];
+    rho[p] = r;
+    for (int j = 0; j < oc.n; ++j) {
+        const int c = oc.c[j];
+        double v = 0.0;
+        if (c == 0) {
+            v = 2.0 * r;
+        } else {
+            for (int m = 0; m < nmt; ++m) v += partial[((int64_t)m * S + c) * ld + p];
+        }
+        delta[j * ld_delta + p] = v;
+    }
+}
+
+template <int PREC, int S>
+int launch_big_s(const BigArgs& a, hipStream_t st)
+{
+    using LD = BigLds<PREC, S>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_rho_big<PREC, S>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           LD::TOTAL);
+        if (e != hipSuccess) return okmi_hip_fail(e, "rho_big: LDS size");
+        attr_set = true;
+    }
+    OkmiProfScope prof("rho_big", st);
+    hipLaunchKernelGGL((k_rho_big<PREC, S>), dim3((unsigned)a.nblocks), dim3(256), LD::TOTAL, st, a);
+    return 0;
+}
+
+template <int PREC>
+int launch_big(const BigArgs& a, int S, hipStream_t st)
+{
+    switch (S) {
+    case 1: return launch_big_s<PREC, 1>(a, st);
+    case 4: return launch_big_s<PREC, 4>(a, st);
+    case 7: return launch_big_s<PREC, 7>(a, st);
+    default: return launch_big_s<PREC, 10>(a, st);
+    }
+}
+
+inline size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int big_sets(const int* drv_out, int nout)
+{
+    int mx = 0;
+    for (int r = 0; r < nout; ++r) mx = std::max(mx, drv_out[r]);
+    return mx == 0 ? 1 : mx <= 3 ? 4 : mx <= 6 ? 7 : 10;
+}
+
+struct BigLayout {
+    int64_t mpad, kpad, chunk;
+    size_t c_bytes, occ_bytes, per_point, fixed;
+};
+
+BigLayout big_layout(const OkmiBasis* b, int nmo, int prec, int S)
+{
+    BigLayout L{};
+    L.mpad = (int64_t)((nmo + MB - 1) / MB) * MB;
+    L.kpad = (int64_t)((b->nao + KS - 1) / KS) * KS;
+    const size_t elem = prec == OKMI_PREC_F32 ? 4 : prec == OKMI_PREC_BF16X3 ? 4 : 6;
+    L.c_bytes = a256(elem * L.mpad * L.kpad);
+    L.occ_bytes = a256(8 * L.mpad);
+    // AO sets (FP32) + partials (FP64) per point
+    L.per_point = (size_t)S * b->nao * 4 + (size_t)(L.mpad / MB) * S * 8;
+    L.fixed = L.c_bytes + L.occ_bytes + 2 * 256 + 256;
+    return L;
+}
+
+}  // namespace
+
+size_t okmi_big_ws_bytes(const OkmiBasis* b, int nmo, int prec, int S, int64_t np)
+{
+    const BigLayout L = big_layout(b, nmo, prec, S);
+    const int64_t n = (np + 127) & ~(int64_t)127;
+    return L.fixed + L.per_point * (size_t)n;
+}
+
+int okmi_launch_rho_big(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const double* d_c,
+                        const double* d_occ, int nmo, int prec, const int* drv_out, int nout, double* rho,
+                        double* delta, int64_t ld_delta, void* ws, size_t ws_bytes, hipStream_t st)
+{
+    if (np <= 0) return 0;
+    if (prec != OKMI_PREC_F32 && prec != OKMI_PREC_BF16X3 && prec != OKMI_PREC_BF16X6) {
+        okmi_set_error("rho_big: unknown precision %d", prec);
+        return 1;
+    }
+    if (nout > 16) {
+        okmi_set_error("rho_big: %d derivative outputs (up to 16)", nout);
+        return 1;
+    }
+    for (int r = 0; r < nout; ++r)
+        if (drv_out[r] < 0 || drv_out[r] > 9) {
+            okmi_set_error("The selection `drv=%d` is not valid!", drv_out[r]);
+            return 1;
+        }
+    const int S = big_sets(drv_out, nout);
+    const BigLayout L = big_layout(b, nmo, prec, S);
+    if (!ws || ws_bytes < L.fixed + L.per_point * 128) {
+        okmi_set_error("rho_big: workspace of %zu bytes is too small (need at least %zu)", ws_bytes,
+                       L.fixed + L.per_point * 128);
+        return 1;
+    }
+    int64_t chunk = (int64_t)((ws_bytes - L.fixed) / L.per_point) & ~(int64_t)127;
+    const int64_t npad = (np + 127) & ~(int64_t)127;
+    if (chunk > npad) chunk = npad;
+    char* w = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+    void* cpl = w;
+    double* occp = (double*)(w + L.c_bytes);
+    float* ao = (float*)(w + L.c_bytes + L.occ_bytes);
+    double* part = (double*)((char*)ao + a256((size_t)S * b->nao * 4 * chunk));
+    const int nmt = (int)(L.mpad / MB);
+
+    hipLaunchKernelGGL(k_prep_c, dim3((unsigned)((L.mpad * L.kpad + 255) / 256)), dim3(256), 0, st, d_c, nmo, b->nao,
+                       prec, L.mpad, L.kpad, cpl);
+    hipLaunchKernelGGL(k_pad_occ_big, dim3((unsigned)((L.mpad + 255) / 256)), dim3(256), 0, st, d_occ, nmo,
+                       (int)L.mpad, occp);
+    int codes[10];
+    for (int s = 0; s < S; ++s) codes[s] = s;
+    OutCodes oc{};
+    oc.n = nout;
+    for (int r = 0; r < nout; ++r) oc.c[r] = drv_out[r];
+    const int NP = 32 * (S == 1 ? 4 : S == 4 ? 2 : 1);
+    for (int64_t c0 = 0; c0 < np; c0 += chunk) {
+        const int64_t n = np - c0 < chunk ? np - c0 : chunk;
+        int rc = okmi_launch_ao_eval(b, g, p0 + c0, n, codes, S, OKMI_F32, ao, chunk, (int64_t)b->nao * chunk, st);
+        if (rc) return rc;
+        BigArgs a{};
+        a.ao = ao; a.ld = chunk; a.set_stride = (int64_t)b->nao * chunk; a.nao = b->nao; a.np = n;
+        a.C = cpl; a.mpad = L.mpad; a.kpad = L.kpad; a.occ = occp; a.partial = part; a.nmt = nmt;
+        a.nblocks = (int64_t)nmt * ((n + NP - 1) / NP);
+        switch (prec) {
+        case OKMI_PREC_F32: rc = launch_big<OKMI_PREC_F32>(a, S, st); break;
+        case OKMI_PREC_BF16X3: rc = launch_big<OKMI_PREC_BF16X3>(a, S, st); break;
+        default: rc = launch_big<OKMI_PREC_BF16X6>(a, S, st); break;
+        }
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_reduce_big, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part, nmt, S, chunk, n, oc,
+                           rho + c0, delta ? delta + c0 : nullptr, ld_delta);
+    }
+    return okmi_check_launch("rho_big");
+}

@@ -30,11 +30,11 @@
 // gather kernel then forms orbkit's output rows.
 #include "okmi_internal.h"
 #include "ao_device.h"
+#include "split_mfma.h"
 
 namespace {
 
-typedef float f4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+using namespace okmi_split;
 
 constexpr int KC = 32;        // AO rows per chunk
 constexpr int LDS_LD = 65;    // padded row stride (floats) of the LDS AO tile
@@ -57,8 +57,10 @@ struct FusedArgs {
     double* term2;            // [10][ld_term]: f_a f_b halves (codes 4..9)
     int64_t ld_term;
     int need_mask;            // codes whose terms are wanted
-    int brick;                // regular grid tiled in 4x4x4 bricks (p0 = 0, all points)
-    int64_t nbx, nby, nbz;
+    int brick;                // regular grid tiled in bricks of 64 points (p0 = 0, all points)
+    int tlx, tly, tlz;        // brick extents 2^tlx x 2^tly x 2^tlz (tlx + tly + tlz = 6)
+    int tl_pad;
+    int64_t nbx, nby, nbz;    // bricks per axis
 };
 
 template <int PASS> struct PassCodes;
@@ -69,15 +71,17 @@ template <> struct PassCodes<3> { static constexpr int n = 4; static constexpr i
 
 // point of lane `ln` of this block's tile: index relative to p0, validity,
 // and the (clamped) coordinates
-__device__ __forceinline__ void tile_point(const FusedArgs& a, int ln, int64_t& p, bool& act, double& px, double& py,
-                                           double& pz)
+__device__ __forceinline__ void tile_point(const FusedArgs& a, int64_t tile, int ln, int64_t& p, bool& act, double& px,
+                                           double& py, double& pz)
 {
     if (a.brick) {
-        int64_t t = blockIdx.x;
+        int64_t t = tile;
         const int64_t bz = t % a.nbz;
         t /= a.nbz;
         const int64_t by = t % a.nby, bx = t / a.nby;
-        int64_t ix = 4 * bx + (ln >> 4), iy = 4 * by + ((ln >> 2) & 3), iz = 4 * bz + (ln & 3);
+        int64_t ix = (bx << a.tlx) + (ln >> (a.tly + a.tlz));
+        int64_t iy = (by << a.tly) + ((ln >> a.tlz) & ((1 << a.tly) - 1));
+        int64_t iz = (bz << a.tlz) + (ln & ((1 << a.tlz) - 1));
         act = ix < a.grid.nx && iy < a.grid.ny && iz < a.grid.nz;
         ix = ix < a.grid.nx ? ix : a.grid.nx - 1;
         iy = iy < a.grid.ny ? iy : a.grid.ny - 1;
@@ -85,7 +89,7 @@ __device__ __forceinline__ void tile_point(const FusedArgs& a, int ln, int64_t& 
         p = (ix * a.grid.ny + iy) * a.grid.nz + iz;
         px = a.grid.x[ix]; py = a.grid.y[iy]; pz = a.grid.z[iz];
     } else {
-        const int64_t pl = (int64_t)blockIdx.x * PTS + ln;
+        const int64_t pl = tile * PTS + ln;
         act = pl < a.np;
         p = act ? pl : a.np - 1;
         grid_point(a.grid, a.p0 + p, px, py, pz);
@@ -212,80 +216,26 @@ __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const Shell
     }
 }
 
-template <int PREC>
-struct Split {
-    bf8 p[PREC == OKMI_PREC_BF16X6 ? 3 : 2];
+template <int PASS>
+struct FusedLds {
+    static constexpr int BUF = PassCodes<PASS>::n * KC * LDS_LD;   // floats per buffer
 };
 
-// x = hi + mid (+ lo) exactly up to the last part: hi and mid keep the top 16
-// bits of x and of the residual x - hi (a truncating split, so every
-// residual is exact in FP32); the last part is the residual rounded to
-// nearest.  Two bf16 of consecutive k share a dword (low half: even k).
-template <int PREC>
-__device__ __forceinline__ void split8(const float (&x)[8], Split<PREC>& o)
-{
-    typedef unsigned u4 __attribute__((ext_vector_type(4)));
-    u4 h, m, lo;
-    float r1[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const unsigned xb = __float_as_uint(x[j]);
-        r1[j] = x[j] - __uint_as_float(xb & 0xffff0000u);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-        h[i] = __builtin_amdgcn_perm(__float_as_uint(x[2 * i + 1]), __float_as_uint(x[2 * i]), 0x07060302u);
-    if constexpr (PREC == OKMI_PREC_BF16X6) {
-        float r2[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) r2[j] = r1[j] - __uint_as_float(__float_as_uint(r1[j]) & 0xffff0000u);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            m[i] = __builtin_amdgcn_perm(__float_as_uint(r1[2 * i + 1]), __float_as_uint(r1[2 * i]), 0x07060302u);
-            const __bf16 a = (__bf16)r2[2 * i], b = (__bf16)r2[2 * i + 1];
-            lo[i] = (unsigned)__builtin_bit_cast(unsigned short, a) | ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
-        }
-        o.p[2] = __builtin_bit_cast(bf8, lo);
-    } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const __bf16 a = (__bf16)r1[2 * i], b = (__bf16)r1[2 * i + 1];
-            m[i] = (unsigned)__builtin_bit_cast(unsigned short, a) | ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
-        }
-    }
-    o.p[0] = __builtin_bit_cast(bf8, h);
-    o.p[1] = __builtin_bit_cast(bf8, m);
-}
-
-template <int PREC>
-__device__ __forceinline__ f4 mfma_split(const Split<PREC>& A, const Split<PREC>& B, f4 acc)
-{
-    // smallest terms first
-    if constexpr (PREC == OKMI_PREC_BF16X6) {
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.p[2], B.p[0], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.p[1], B.p[1], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.p[0], B.p[2], acc, 0, 0, 0);
-    }
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.p[1], B.p[0], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.p[0], B.p[1], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.p[0], B.p[0], acc, 0, 0, 0);
-    return acc;
-}
-
+// One tile of 64 points: every thread of the workgroup calls this with the
+// same tile.  `lds` holds two AO buffers, `next` two work counters.  The
+// first barrier orders it after any previous tile's LDS reads.
 template <int PREC, int PASS, int MT>
-__global__ __launch_bounds__(256, 2) void k_rho_fused(FusedArgs a)
+__device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float* lds, int* next)
 {
     constexpr int NS = PassCodes<PASS>::n;
-    constexpr int BUF = NS * KC * LDS_LD;
-    __shared__ float lds[2 * BUF];
-    __shared__ int next[2];   // per buffer: next work entry to evaluate
+    constexpr int BUF = FusedLds<PASS>::BUF;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (threadIdx.x < 2) next[threadIdx.x] = 0;
     __syncthreads();
     int64_t pidx;
     bool pact;
     double px, py, pz;
-    tile_point(a, lane, pidx, pact, px, py, pz);   // lane l of every wave: point l of the tile
+    tile_point(a, tile, lane, pidx, pact, px, py, pz);   // lane l of every wave: point l of the tile
     Box box;
     box.lo[0] = wave_min(px); box.hi[0] = wave_max(px);
     box.lo[1] = wave_min(py); box.hi[1] = wave_max(py);
@@ -429,7 +379,7 @@ __global__ __launch_bounds__(256, 2) void k_rho_fused(FusedArgs a)
     int64_t p;
     bool act;
     double qx, qy, qz;
-    tile_point(a, wave * 16 + n16, p, act, qx, qy, qz);
+    tile_point(a, tile, wave * 16 + n16, p, act, qx, qy, qz);
     if (g == 0 && act) {
         if constexpr (PASS <= 1) a.rho[p] = r0;
 #pragma unroll
@@ -443,6 +393,14 @@ __global__ __launch_bounds__(256, 2) void k_rho_fused(FusedArgs a)
                 if (a.need_mask >> (4 + q) & 1) a.term2[(4 + q) * a.ld_term + p] = u[q];
         }
     }
+}
+
+template <int PREC, int PASS, int MT>
+__global__ __launch_bounds__(256, 2) void k_rho_fused(FusedArgs a)
+{
+    __shared__ float lds[2 * FusedLds<PASS>::BUF];
+    __shared__ int next[2];   // per buffer: next work entry to evaluate
+    rho_tile<PREC, PASS, MT>(a, blockIdx.x, lds, next);
 }
 
 struct GatherCodes {
@@ -495,27 +453,6 @@ int launch_pass(const FusedArgs& a, int pass, int mt, hipStream_t st)
     case 1: return launch_mt<PREC, 1>(a, mt, st);
     case 2: return launch_mt<PREC, 2>(a, mt, st);
     default: return launch_mt<PREC, 3>(a, mt, st);
-    }
-}
-
-// C (FP64 [nmo][nao]) -> padded operand planes
-__global__ void k_prep_c(const double* C, int nmo, int nao, int prec, int64_t mpad, int64_t kpad, void* out)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= mpad * kpad) return;
-    const int64_t m = i / kpad, k = i - m * kpad;
-    const double v = (m < nmo && k < nao) ? C[m * nao + k] : 0.0;
-    if (prec == OKMI_PREC_F32) {
-        ((float*)out)[i] = (float)v;
-    } else {
-        __bf16* o = (__bf16*)out;
-        const float x = (float)v;
-        const __bf16 h = (__bf16)x;
-        const float r1 = x - (float)h;
-        const __bf16 md = (__bf16)r1;
-        o[i] = h;
-        o[mpad * kpad + i] = md;
-        if (prec == OKMI_PREC_BF16X6) o[2 * mpad * kpad + i] = (__bf16)(r1 - (float)md);
     }
 }
 
@@ -575,6 +512,7 @@ int prepare_fused_tables(OkmiBasis* b)
 
 size_t okmi_fused_ws_bytes(const OkmiBasis* b, int nmo, int prec, int64_t np)
 {
+    if (nmo > 64) return okmi_big_ws_bytes(b, nmo, prec, 7, np);
     return fused_fixed_bytes(b, nmo, prec) + align256((size_t)20 * np * 8) + 256;
 }
 
@@ -584,10 +522,9 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
 {
     if (np <= 0) return 0;
     const int mt = (nmo + 15) / 16;
-    if (mt > 4) {
-        okmi_set_error("rho_fused: %d MOs (the fused kernel holds up to 64)", nmo);
-        return 1;
-    }
+    if (mt > 4)   // more MOs than the fused kernel keeps in registers
+        return okmi_launch_rho_big(b, g, p0, np, d_c, d_occ, nmo, prec, drv_out, nout, rho, delta, ld_delta, ws,
+                                   ws_bytes, st);
     if (prec != OKMI_PREC_F32 && prec != OKMI_PREC_BF16X3 && prec != OKMI_PREC_BF16X6) {
         okmi_set_error("rho_fused: unknown precision %d", prec);
         return 1;
@@ -640,6 +577,7 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
     a.C = cpl; a.kpad = kpad; a.mpad = mpad; a.occ = occp;
     a.term = term; a.term2 = term2; a.ld_term = chunk; a.need_mask = need;
     a.brick = brick;
+    a.tlx = a.tly = a.tlz = 2;
     a.nbx = (g.nx + 3) / 4; a.nby = (g.ny + 3) / 4; a.nbz = (g.nz + 3) / 4;
     for (int64_t c0 = 0; c0 < np; c0 += chunk) {
         const int64_t n = np - c0 < chunk ? np - c0 : chunk;
@@ -667,3 +605,384 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
     }
     return okmi_check_launch("rho_fused");
 }
+
+// ---------------------------------------------------------------------------
+// Many molecules in one persistent launch.  Every molecule keeps its own
+// tables (shells, primitives, transform, chunk map), MO coefficient planes,
+// grid and outputs, described by one FusedArgs record; the tiles of all
+// molecules form one index space (prefix sums tile_start, molecules in work
+// order, costliest first by default).  A fixed grid of workgroups strides
+// through it; each tile finds its molecule by a uniform binary search and
+// runs the single-molecule tile body.  Launch sequence (graph-capturable:
+// kernels only, no memsets, no allocation): C planes + padded occupations for
+// every molecule, then the tile kernel, whose first-derivative terms are
+// written straight into the caller's gradient rows.
+
+namespace {
+
+struct PrepItem {
+    const double* C;      // [nmo][nao]
+    const double* occ;    // [nmo]
+    void* planes;         // [planes][mpad][kpad]
+    double* occ_pad;      // [mpad]
+    int nmo, nao;
+    int64_t kpad;
+};
+
+struct BatchArgs {
+    const FusedArgs* mols;       // per molecule, in work order
+    const int64_t* tile_start;   // [nmol + 1]
+    int nmol;
+    int64_t ntile;
+};
+
+static_assert(sizeof(FusedArgs) % 8 == 0, "FusedArgs is loaded as 64-bit words");
+
+// FusedArgs of molecule m through scalar (constant address space) loads
+__device__ __forceinline__ FusedArgs load_fused_args(const FusedArgs* base, int m)
+{
+    constexpr int NW = sizeof(FusedArgs) / 8;
+    struct Words { uint64_t w[NW]; } v;
+    const OKMI_CONST uint64_t* src = (const OKMI_CONST uint64_t*)cptr(base + m);
+#pragma unroll
+    for (int i = 0; i < NW; ++i) v.w[i] = src[i];
+    return __builtin_bit_cast(FusedArgs, v);
+}
+
+__device__ __forceinline__ int find_item(const int64_t* start, int n, int64_t t)
+{
+    const OKMI_CONST int64_t* s = cptr(start);
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int PREC, int PASS, int MT>
+__global__ __launch_bounds__(256, 2) void k_rho_batch(BatchArgs b)
+{
+    __shared__ float lds[2 * FusedLds<PASS>::BUF];
+    __shared__ int next[2];
+    for (int64_t t = blockIdx.x; t < b.ntile; t += gridDim.x) {
+        const int m = find_item(b.tile_start, b.nmol, t);
+        const FusedArgs a = load_fused_args(b.mols, m);
+        rho_tile<PREC, PASS, MT>(a, t - cptr(b.tile_start)[m], lds, next);
+    }
+}
+
+// element e of item m: C plane entries first (mpad * kpad), then occupations
+__global__ void k_prep_batch(const PrepItem* items, const int64_t* start, int nitem, int64_t total, int prec,
+                             int64_t mpad)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nitem - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (start[mid] <= e) lo = mid;
+            else hi = mid - 1;
+        }
+        const PrepItem it = items[lo];
+        const int64_t i = e - start[lo];
+        const int64_t nc = mpad * it.kpad;
+        if (i >= nc) {
+            const int64_t j = i - nc;
+            it.occ_pad[j] = j < it.nmo ? it.occ[j] : 0.0;
+            continue;
+        }
+        const int64_t m = i / it.kpad, k = i - m * it.kpad;
+        const double v = (m < it.nmo && k < it.nao) ? it.C[m * it.nao + k] : 0.0;
+        if (prec == OKMI_PREC_F32) {
+            ((float*)it.planes)[i] = (float)v;
+        } else {
+            __bf16* o = (__bf16*)it.planes;
+            const float x = (float)v;
+            const __bf16 h = (__bf16)x;
+            const float r1 = x - (float)h;
+            const __bf16 md = (__bf16)r1;
+            o[i] = h;
+            o[nc + i] = md;
+            if (prec == OKMI_PREC_BF16X6) o[2 * nc + i] = (__bf16)(r1 - (float)md);
+        }
+    }
+}
+
+template <int PREC, int PASS>
+int launch_batch_mt(const BatchArgs& a, int mt, unsigned grid, hipStream_t st)
+{
+    switch (mt) {
+    case 1: hipLaunchKernelGGL((k_rho_batch<PREC, PASS, 1>), dim3(grid), dim3(256), 0, st, a); return 0;
+    case 2: hipLaunchKernelGGL((k_rho_batch<PREC, PASS, 2>), dim3(grid), dim3(256), 0, st, a); return 0;
+    case 3: hipLaunchKernelGGL((k_rho_batch<PREC, PASS, 3>), dim3(grid), dim3(256), 0, st, a); return 0;
+    case 4: hipLaunchKernelGGL((k_rho_batch<PREC, PASS, 4>), dim3(grid), dim3(256), 0, st, a); return 0;
+    default: break;
+    }
+    okmi_set_error("batch: %d MO tiles (up to 4)", mt);
+    return 1;
+}
+
+template <int PREC>
+int launch_batch(const BatchArgs& a, int pass, int mt, unsigned grid, hipStream_t st)
+{
+    return pass == 0 ? launch_batch_mt<PREC, 0>(a, mt, grid, st) : launch_batch_mt<PREC, 1>(a, mt, grid, st);
+}
+
+template <typename T>
+int upload(T** d, const std::vector<T>& h, const char* what)
+{
+    hipError_t e = hipMalloc((void**)d, sizeof(T) * std::max<size_t>(1, h.size()));
+    if (e != hipSuccess) return okmi_hip_fail(e, what);
+    e = hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return okmi_hip_fail(e, what);
+    return 0;
+}
+
+}  // namespace
+
+struct OkmiBatch {
+    int nmol = 0, prec = 0, pass = 0, mt = 0;
+    int64_t mpad = 0, ntile = 0, prep_total = 0;
+    unsigned grid = 0;                    // persistent workgroups (0: one per tile)
+    FusedArgs* d_mols = nullptr;
+    int64_t* d_tile_start = nullptr;
+    PrepItem* d_items = nullptr;
+    int64_t* d_item_start = nullptr;
+    void* d_planes = nullptr;
+    double* d_occ = nullptr;
+    std::vector<int> order;               // work order -> caller's molecule index
+    std::vector<FusedArgs> h_mols;        // host copy of the descriptors (tile shape changes)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+};
+
+void okmi_batch_free(OkmiBatch* b)
+{
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (void* p : {(void*)b->d_mols, (void*)b->d_tile_start, (void*)b->d_items, (void*)b->d_item_start,
+                    b->d_planes, (void*)b->d_occ})
+        if (p) (void)hipFree(p);
+    delete b;
+}
+
+// brick extents 2^lx x 2^ly x 2^lz for the regular-grid molecules: tile
+// counts, tile offsets and descriptors (re)uploaded; any captured graph is
+// dropped (its kernels hold the old tile count)
+static int batch_set_shape(OkmiBatch* B, int lx, int ly, int lz)
+{
+    if (lx < 0 || ly < 0 || lz < 0 || lx + ly + lz != 6) {
+        okmi_set_error("batch: tile shape 2^%d x 2^%d x 2^%d is not 64 points", lx, ly, lz);
+        return 1;
+    }
+    std::vector<int64_t> tstart(B->nmol + 1, 0);
+    for (int j = 0; j < B->nmol; ++j) {
+        FusedArgs& a = B->h_mols[j];
+        int64_t tiles;
+        if (a.brick) {
+            a.tlx = lx; a.tly = ly; a.tlz = lz;
+            a.nbx = (a.grid.nx + (1 << lx) - 1) >> lx;
+            a.nby = (a.grid.ny + (1 << ly) - 1) >> ly;
+            a.nbz = (a.grid.nz + (1 << lz) - 1) >> lz;
+            tiles = a.nbx * a.nby * a.nbz;
+        } else {
+            tiles = (a.np + PTS - 1) / PTS;
+        }
+        tstart[j + 1] = tstart[j] + tiles;
+    }
+    if (B->exec) {
+        (void)hipGraphExecDestroy(B->exec);
+        (void)hipGraphDestroy(B->graph);
+        B->exec = nullptr;
+        B->graph = nullptr;
+    }
+    if (B->d_mols) (void)hipFree(B->d_mols);
+    if (B->d_tile_start) (void)hipFree(B->d_tile_start);
+    B->d_mols = nullptr;
+    B->d_tile_start = nullptr;
+    int rc = upload(&B->d_mols, B->h_mols, "batch: descriptors");
+    if (!rc) rc = upload(&B->d_tile_start, tstart, "batch: tile offsets");
+    B->ntile = tstart[B->nmol];
+    return rc;
+}
+
+int okmi_batch_set_shape_impl(OkmiBatch* B, int lx, int ly, int lz) { return batch_set_shape(B, lx, ly, lz); }
+
+int okmi_batch_build(int nmol, OkmiBasis* const* bases, const GridDesc* grids, const int64_t* npts,
+                     const double* const* d_c, const double* const* d_occ, const int* nmo, double* const* d_rho,
+                     double* const* d_grad, int prec, int order_by_cost, OkmiBatch** out)
+{
+    if (prec != OKMI_PREC_F32 && prec != OKMI_PREC_BF16X3 && prec != OKMI_PREC_BF16X6) {
+        okmi_set_error("batch: unknown precision %d", prec);
+        return 1;
+    }
+    OkmiBatch* B = new OkmiBatch();
+    B->nmol = nmol;
+    B->prec = prec;
+    int mt = 1;
+    bool grad = false;
+    std::vector<double> cost(nmol);
+    for (int m = 0; m < nmol; ++m) {
+        if (nmo[m] <= 0 || nmo[m] > 64) {
+            okmi_set_error("batch: molecule %d has %d MOs (1..64 in a batch)", m, nmo[m]);
+            okmi_batch_free(B);
+            return 1;
+        }
+        if (npts[m] <= 0) {
+            okmi_set_error("batch: molecule %d has an empty grid", m);
+            okmi_batch_free(B);
+            return 1;
+        }
+        int rc = prepare_fused_tables(bases[m]);
+        if (rc) {
+            okmi_batch_free(B);
+            return rc;
+        }
+        mt = std::max(mt, (nmo[m] + 15) / 16);
+        grad = grad || d_grad[m] != nullptr;
+        double prims = 0;
+        for (int s = 0; s < bases[m]->nshell; ++s) prims += bases[m]->h_shells[s].nprim + bases[m]->h_shells[s].nrow;
+        cost[m] = prims * (double)npts[m];
+    }
+    B->mt = mt;
+    B->mpad = 16 * mt;
+    B->pass = grad ? 1 : 0;
+    B->order.resize(nmol);
+    for (int m = 0; m < nmol; ++m) B->order[m] = m;
+    if (order_by_cost)
+        std::stable_sort(B->order.begin(), B->order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    const size_t elem = prec == OKMI_PREC_F32 ? 4 : prec == OKMI_PREC_BF16X3 ? 4 : 6;
+    std::vector<size_t> poff(nmol);
+    size_t pbytes = 0;
+    for (int j = 0; j < nmol; ++j) {
+        const int m = B->order[j];
+        poff[j] = pbytes;
+        pbytes += (elem * B->mpad * ((int64_t)bases[m]->nchunk * KC) + 255) & ~(size_t)255;
+    }
+    hipError_t e = hipMalloc(&B->d_planes, std::max<size_t>(pbytes, 256));
+    if (e == hipSuccess) e = hipMalloc((void**)&B->d_occ, sizeof(double) * B->mpad * nmol);
+    if (e != hipSuccess) {
+        okmi_batch_free(B);
+        return okmi_hip_fail(e, "batch: coefficient planes");
+    }
+    std::vector<FusedArgs> args(nmol);
+    std::vector<int64_t> istart(nmol + 1, 0);
+    std::vector<PrepItem> items(nmol);
+    for (int j = 0; j < nmol; ++j) {
+        const int m = B->order[j];
+        const OkmiBasis* bs = bases[m];
+        const GridDesc& g = grids[m];
+        FusedArgs a{};
+        a.shells = bs->d_shells; a.prims = bs->d_prims; a.T = bs->d_T;
+        a.chunks = bs->d_chunks; a.work = bs->d_work;
+        a.grid = g; a.p0 = 0; a.np = npts[m];
+        a.nao = bs->nao; a.nchunk = bs->nchunk;
+        a.kpad = (int64_t)bs->nchunk * KC; a.mpad = B->mpad;
+        a.C = (char*)B->d_planes + poff[j];
+        a.occ = B->d_occ + (int64_t)j * B->mpad;
+        a.rho = d_rho[m];
+        // first-derivative terms go straight to the gradient rows: term[c][p]
+        // for c = 1..3 is row c-1 of d_grad[m]
+        a.term = d_grad[m] ? (double*)((uintptr_t)d_grad[m] - sizeof(double) * (uintptr_t)npts[m]) : nullptr;
+        a.term2 = nullptr;
+        a.ld_term = npts[m];
+        a.need_mask = d_grad[m] ? 0xE : 0;
+        a.brick = g.regular;
+        args[j] = a;
+        PrepItem it{};
+        it.C = d_c[m]; it.occ = d_occ[m]; it.planes = (void*)a.C; it.occ_pad = (double*)a.occ;
+        it.nmo = nmo[m]; it.nao = bs->nao; it.kpad = a.kpad;
+        items[j] = it;
+        istart[j + 1] = istart[j] + B->mpad * a.kpad + B->mpad;
+    }
+    B->prep_total = istart[nmol];
+    B->h_mols = args;
+    int rc = batch_set_shape(B, 2, 2, 2);
+    if (!rc) rc = upload(&B->d_items, items, "batch: prep items");
+    if (!rc) rc = upload(&B->d_item_start, istart, "batch: prep offsets");
+    if (rc) {
+        okmi_batch_free(B);
+        return rc;
+    }
+    *out = B;
+    return 0;
+}
+
+int okmi_batch_enqueue(OkmiBatch* B, hipStream_t st)
+{
+    const int64_t pblocks = std::min<int64_t>((B->prep_total + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_prep_batch, dim3((unsigned)pblocks), dim3(256), 0, st, B->d_items, B->d_item_start, B->nmol,
+                       B->prep_total, B->prec, B->mpad);
+    BatchArgs a{};
+    a.mols = B->d_mols; a.tile_start = B->d_tile_start; a.nmol = B->nmol; a.ntile = B->ntile;
+    const unsigned grid = B->grid ? (unsigned)std::min<int64_t>(B->grid, B->ntile) : (unsigned)B->ntile;
+    switch (B->prec) {
+    case OKMI_PREC_F32: return launch_batch<OKMI_PREC_F32>(a, B->pass, B->mt, grid, st);
+    case OKMI_PREC_BF16X3: return launch_batch<OKMI_PREC_BF16X3>(a, B->pass, B->mt, grid, st);
+    default: return launch_batch<OKMI_PREC_BF16X6>(a, B->pass, B->mt, grid, st);
+    }
+}
+
+int okmi_batch_run(OkmiBatch* B, int use_graph, hipStream_t st)
+{
+    if (!use_graph) {
+        OkmiProfScope prof("rho_batch", st);
+        int rc = okmi_batch_enqueue(B, st);
+        if (rc) return rc;
+        return okmi_check_launch("batch");
+    }
+    if (!st) {
+        okmi_set_error("batch: graph launches need a non-default stream");
+        return 1;
+    }
+    if (B->exec && B->graph_stream != st) {
+        (void)hipGraphExecDestroy(B->exec);
+        (void)hipGraphDestroy(B->graph);
+        B->exec = nullptr;
+        B->graph = nullptr;
+    }
+    if (!B->exec) {
+        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return okmi_hip_fail(e, "batch: begin capture");
+        const int rc = okmi_batch_enqueue(B, st);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return okmi_hip_fail(e, "batch: end capture");
+        e = hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return okmi_hip_fail(e, "batch: instantiate graph");
+        }
+        B->graph = g;
+        B->graph_stream = st;
+    }
+    OkmiProfScope prof("rho_batch", st);
+    const hipError_t e = hipGraphLaunch(B->exec, st);
+    if (e != hipSuccess) return okmi_hip_fail(e, "batch: graph launch");
+    return 0;
+}
+
+int okmi_batch_set_grid_impl(OkmiBatch* B, int blocks)
+{
+    if (blocks < 0) {
+        okmi_set_error("batch: negative workgroup count");
+        return 1;
+    }
+    if ((unsigned)blocks != B->grid && B->exec) {
+        (void)hipGraphExecDestroy(B->exec);
+        (void)hipGraphDestroy(B->graph);
+        B->exec = nullptr;
+        B->graph = nullptr;
+    }
+    B->grid = (unsigned)blocks;
+    return 0;
+}
+
+int64_t okmi_batch_ntile(const OkmiBatch* B) { return B->ntile; }

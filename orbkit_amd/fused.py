@@ -12,6 +12,9 @@ from . import _native
 
 #: device bytes for the fused path's per-point term buffers
 WORKSPACE_BYTES = 256 << 20
+#: points per pass of the many-MO path (more than 64 MOs): AO sets of a pass
+#: are staged in HBM (FP32), e.g. 2,950 AOs x 7 sets x 131,072 points = 10.8 GB
+MANY_MO_CHUNK = 131072
 
 
 class FusedPlan:
@@ -39,7 +42,7 @@ class FusedPlan:
     n = max(self.np, 1)
     self.rho = torch.empty(n, dtype=torch.float64, device=dev)
     self.delta = torch.empty((max(self.nd, 1), n), dtype=torch.float64, device=dev)
-    chunk = min(n, max(64, WORKSPACE_BYTES // 160))
+    chunk = min(n, MANY_MO_CHUNK if self.nmo > 64 else max(64, WORKSPACE_BYTES // 160))
     nbytes = self.L.okmi_rho_fused_workspace_bytes(basis.handle, self.nmo, self.prec, chunk)
     self.ws = torch.empty(nbytes // 8 + 1, dtype=torch.float64, device=dev)
 

@@ -149,8 +149,10 @@ def _grid(geo, n, pad=5.0):
   return [numpy.linspace(lo[i], hi[i], n) for i in range(3)]
 
 
-def make(name, seed=None, npts_side=None):
-  '''Returns (qc, (x, y, z) regular axes, info dict).'''
+def make(name, seed=None, npts_side=None, nmo=None):
+  '''Returns (qc, (x, y, z) regular axes, info dict).  ``nmo`` overrides the
+  workload's number of (orthonormal, doubly occupied) MOs.'''
+  nmo_override = nmo
   seeds = {'small': 1, 'medium': 2, 'gradient': 3, 'large': 5}
   rng = numpy.random.default_rng(seeds[name] if seed is None else seed)
   if name == 'small':
@@ -172,6 +174,8 @@ def make(name, seed=None, npts_side=None):
     raise ValueError('unknown workload %r' % name)
   if npts_side is not None:
     side = npts_side
+  if nmo_override is not None:
+    nmo = int(nmo_override)
   S = ao_overlap(geo, ao)
   mo = _mos(rng, S, nmo)
   qc = QCinfo()

@@ -262,3 +262,66 @@ def test_fused_precision(ok, prec, tol):
   print(prec, errs)
   for k in ('rho', 'dd', 'lap', 'grad'):
     assert errs[k] <= tol, (k, errs[k])
+
+
+@pytest.mark.parametrize('prec,tol', [('f32', 1e-5), ('bf16x6', 1e-5), ('bf16x3', 1e-3)])
+def test_many_mo_precision(ok, prec, tol):
+  '''More MOs than the fused kernel holds (130: two 128-MO tiles, the second
+  ragged): FP32 AO chunks + MFMA GEMM path vs the FP64 path, all four AO-set
+  widths (rho; rho+grad; +Laplacian; +mixed second derivatives), regular grid
+  and a vector grid whose length is no multiple of any tile.'''
+  core, grid = ok['core'], ok['grid']
+  qc, axes, _ = ok['W'].make('medium', npts_side=20, nmo=130)
+  print('case many-MO', prec, flush=True)
+  grid.set_grid(*axes, is_vector=False)
+  errs = {}
+  r64 = core.rho_compute(qc)
+  rlo = core.rho_compute(qc, precision=prec)
+  errs['rho'] = numpy.linalg.norm(rlo - r64) / numpy.linalg.norm(r64)
+  rho, d = core.rho_compute(qc, drv=['x', 'y', 'z'])
+  frho, fd = core.rho_compute(qc, drv=['x', 'y', 'z'], precision=prec)
+  errs['rho4'] = numpy.linalg.norm(frho - rho) / numpy.linalg.norm(rho)
+  errs['grad'] = numpy.linalg.norm(fd - d) / numpy.linalg.norm(d)
+  rho, d, lap = core.rho_compute(qc, laplacian=True)
+  frho, fd, flap = core.rho_compute(qc, laplacian=True, precision=prec)
+  errs['lap'] = numpy.linalg.norm(flap - lap) / numpy.linalg.norm(lap)
+  rho, d = core.rho_compute(qc, drv=['xy', 'z', None])
+  frho, fd = core.rho_compute(qc, drv=['xy', 'z', None], precision=prec)
+  errs['mixed'] = numpy.linalg.norm(fd - d) / numpy.linalg.norm(d)
+  x, y, z = _pts(axes, 3001, seed=3)
+  grid.set_grid(x, y, z, is_vector=True)
+  rho, d = core.rho_compute(qc, drv=['x', 'zz'])
+  frho, fd = core.rho_compute(qc, drv=['x', 'zz'], precision=prec)
+  errs['vec_rho'] = numpy.linalg.norm(frho - rho) / numpy.linalg.norm(rho)
+  errs['vec_d'] = numpy.linalg.norm(fd - d) / numpy.linalg.norm(d)
+  print(prec, errs)
+  for k, v in errs.items():
+    assert v <= tol, (k, v)
+
+
+def test_large_basis_slices(ok):
+  '''2,950 spherical AOs (s, p, d, f on 200 atoms), 400 MOs: the many-MO
+  bf16x6 path and the FP64 path on a 20,000-point vector sample of the
+  128^3 grid (FP64 vs oracle on 2,000 of them), and all 400 MOs on an 8^3
+  subgrid vs the oracle.'''
+  core, grid, O = ok['core'], ok['grid'], ok['O']
+  qc, axes, info = ok['W'].make('large')
+  assert info['nao'] == 2950 and info['nmo'] == 400
+  print('case large', flush=True)
+  x, y, z = _pts(axes, 20000, seed=11)
+  grid.set_grid(x, y, z, is_vector=True)
+  rho, d = core.rho_compute(qc, drv=['x', 'y', 'z'])
+  frho, fd = core.rho_compute(qc, drv=['x', 'y', 'z'], precision='bf16x6')
+  e_rho = numpy.linalg.norm(frho - rho) / numpy.linalg.norm(rho)
+  e_d = numpy.linalg.norm(fd - d) / numpy.linalg.norm(d)
+  print('large bf16x6 vs f64', e_rho, e_d)
+  assert e_rho <= 1e-5 and e_d <= 1e-5
+  r_o = O.rho_compute(qc, x[:2000], y[:2000], z[:2000], True, drv=['x', 'y', 'z'])
+  assert rel_err(rho[:2000], r_o[0]) <= TOL and rel_err(d[:, :2000], r_o[1]) <= TOL
+  sub = [a[60:68] for a in axes]
+  grid.set_grid(*sub, is_vector=False)
+  mo = core.rho_compute(qc, calc_mo=True)
+  X, Y, Z = numpy.meshgrid(*sub, indexing='ij')
+  mo_o = O.rho_compute(qc, X.ravel(), Y.ravel(), Z.ravel(), True, calc_mo=True)
+  assert mo.shape == (400, 8, 8, 8)
+  assert rel_err(mo.reshape(400, -1), mo_o) <= TOL
