@@ -1,11 +1,20 @@
 '''Throughput of the grid-density hot path on one MI355X.
 
-Step = one evaluation of rho and grad(rho) over the whole 100^3 regular grid
-of the medium workload (20 atoms, spherical [s,s,p,p,d,f] shells, 400 AOs,
-3 primitives per shell, exponents in [0.05, 1e5], 40 doubly occupied MOs),
-inputs resident in HBM, outputs written to HBM (FP64).  Prints one JSON line.
+Default (the headline line): step = one evaluation of rho and grad(rho) over
+the whole 100^3 regular grid of the medium workload (20 atoms, spherical
+[s,s,p,p,d,f] shells, 400 AOs, 3 primitives per shell, exponents in
+[0.05, 1e5], 40 doubly occupied MOs), inputs resident in HBM, outputs written
+to HBM (FP64).  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--precision bf16x6|bf16x3|f32|f64]
+                    [--workload medium|small|large|gradient|batch] [--laplacian]
+
+Other workloads (parity cases of the same path, measured the same way):
+  large     200 atoms, 2,950 spherical AOs, 400 MOs, 128^3 grid (many-MO path)
+  gradient  forward + backward of L = sum d3r (rho - rho_t)^2 on the medium
+            basis, 80^3 grid (FP64; dL/dC, dL/dalpha, dL/dR)
+  batch     256 molecules of 5-60 atoms, uneven grids 32^3..80^3, one tuned
+            persistent launch replayed from a HIP graph
 '''
 import argparse
 import ctypes
@@ -22,6 +31,8 @@ sys.path.insert(0, ROOT)
 # MI355X published dense peaks
 PEAK_HBM_GBS = 8000.0
 PEAK_TFLOPS = {'bf16': 2500.0, 'f32': 157.3, 'f64': 78.6}
+PRODUCTS = {'bf16x6': 6, 'bf16x3': 3, 'f32': 1}
+METRIC = 'grid points/s for rho_compute (+grad rho)'
 
 
 def parse():
@@ -30,8 +41,9 @@ def parse():
   ap.add_argument('--steps', type=int, default=20)
   ap.add_argument('--warmup', type=int, default=3)
   ap.add_argument('--precision', default='bf16x6', choices=['bf16x6', 'bf16x3', 'f32', 'f64'])
-  ap.add_argument('--workload', default='medium')
+  ap.add_argument('--workload', default='medium', choices=['medium', 'small', 'large', 'gradient', 'batch'])
   ap.add_argument('--side', type=int, default=None, help='grid points per axis (default: workload)')
+  ap.add_argument('--laplacian', action='store_true', help='also the Laplacian (7 AO sets)')
   ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
   ap.add_argument('--cpu-threads', type=int, default=None)
   ap.add_argument('--errors', action='store_true', help='also report FP64-relative errors')
@@ -53,23 +65,211 @@ def measured_traffic(kernel, workload, precision):
   return None if v is None else v['bytes_per_launch']
 
 
-def cpu_baseline(qc, axes, threads, slice_length, slices_per_thread=4):
+def cpu_baseline(qc, axes, threads, slice_length, codes, slices_per_thread=4):
   '''orbkit's algorithm (C restatement in oracle/, OpenMP over slices) on a
   bounded sample: slices_per_thread slices of slice_length points per
   thread, a contiguous run of the flattened grid from its middle.'''
   from oracle import core_oracle
   X, Y, Z = numpy.meshgrid(*axes, indexing='ij')
   P = numpy.stack([X.ravel(), Y.ravel(), Z.ravel()])
-  n = int(threads * slice_length * slices_per_thread)
+  n = int(min(P.shape[1], threads * slice_length * slices_per_thread))
   a = max(0, P.shape[1] // 2 - n // 2)
   x, y, z = P[0, a:a + n].copy(), P[1, a:a + n].copy(), P[2, a:a + n].copy()
-  core_oracle.rho_slices_c(qc, x[:64], y[:64], z[:64], [1, 2, 3], slice_length=64, nthreads=1)
+  core_oracle.rho_slices_c(qc, x[:64], y[:64], z[:64], codes, slice_length=64, nthreads=1)
   t = time.perf_counter()
-  core_oracle.rho_slices_c(qc, x, y, z, [1, 2, 3], slice_length=slice_length, nthreads=threads)
+  core_oracle.rho_slices_c(qc, x, y, z, codes, slice_length=slice_length, nthreads=threads)
   dt = time.perf_counter() - t
   return {'value': n / dt, 'unit': 'grid points/s', 'cores': threads, 'kind': 'port',
-          'sample': '%d of the %d grid points (%d slices of %d, rho + grad rho, FP64) in %.1f s'
-                    % (n, P.shape[1], n // slice_length, slice_length, dt)}
+          'sample': '%d of the %d grid points (%d slices of %d, rho + derivative codes %s, FP64) in %.1f s'
+                    % (n, P.shape[1], -(-n // slice_length), slice_length, list(codes), dt)}
+
+
+def cpu_baseline_grad(qc, axes, n=3000):
+  """FP64 numpy/C gradient oracle (forward + dL/dC, dL/dalpha, dL/dR) on a
+  bounded sample of n grid points from the middle of the grid, one thread."""
+  from threadpoolctl import threadpool_limits
+  from oracle import grad_oracle
+  X, Y, Z = numpy.meshgrid(*axes, indexing='ij')
+  a = X.size // 2 - n // 2
+  x, y, z = (V.ravel()[a:a + n].copy() for V in (X, Y, Z))
+  g = numpy.ones(n)
+  with threadpool_limits(1):
+    t = time.perf_counter()
+    grad_oracle.vjp(qc, x, y, z, g)
+    dt = time.perf_counter() - t
+  return {'value': n / dt, 'unit': 'grid points/s', 'cores': 1, 'kind': 'port',
+          'sample': '%d of the %d grid points, forward + backward, FP64, in %.1f s' % (n, X.size, dt)}
+
+
+def prof_times(L, names):
+  from orbkit_amd import _native
+  out = {}
+  for k in names:
+    cnt, ms = ctypes.c_int(), ctypes.c_double()
+    _native.check(L.okmi_profile_query(k.encode(), ctypes.byref(cnt), ctypes.byref(ms)), 'profile_query')
+    out[k] = (cnt.value, ms.value)
+  return out
+
+
+def timed(step, args, L, names, sync):
+  for _ in range(args.warmup):
+    step()
+  sync()
+  L.okmi_profile_enable(1)
+  sync()
+  t0 = time.perf_counter()
+  for _ in range(args.steps):
+    step()
+  sync()
+  dt = time.perf_counter() - t0
+  times = prof_times(L, names)
+  L.okmi_profile_enable(0)
+  return dt, times
+
+
+# ---------------------------------------------------------------- workloads
+
+def density_step(args, torch, info, qc, axes):
+  '''rho (+ grad, + Laplacian) over one regular grid; fused / many-MO or FP64.'''
+  from orbkit_amd import _native
+  from orbkit_amd.basis import device_basis
+  from orbkit_amd.core import DeviceGrid
+  from orbkit_amd.fused import FusedPlan
+  basis = device_basis(qc.geo_spec, qc.ao_spec)
+  dg = DeviceGrid(*axes, regular=True)
+  C, occ = qc.mo_spec.get_coeffs(), qc.mo_spec.get_occ()
+  codes = numpy.array([1, 2, 3] + ([4, 5, 6] if args.laplacian else []), dtype=numpy.intc)
+  nsets = 7 if args.laplacian else 4
+  L = _native.lib()
+  npts, nao, nmo = dg.npts, info['nao'], info['nmo']
+  if args.precision == 'f64':
+    dev = dg.x.device
+    dC, docc = torch.as_tensor(C, device=dev), torch.as_tensor(occ, device=dev)
+    rho = torch.empty(npts, dtype=torch.float64, device=dev)
+    delta = torch.empty((len(codes), npts), dtype=torch.float64, device=dev)
+    ws = torch.empty(L.okmi_workspace_bytes(basis.handle, nsets, 1 << 17) // 8 + 1, dtype=torch.float64, device=dev)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def step():
+      _native.check(L.okmi_rho(basis.handle, ctypes.byref(dg.desc), 0, npts, ctypes.c_void_p(dC.data_ptr()),
+                               ctypes.c_void_p(docc.data_ptr()), nmo, codes.ctypes.data_as(_native.P_int), len(codes),
+                               ctypes.c_void_p(rho.data_ptr()), ctypes.c_void_p(delta.data_ptr()), npts, None,
+                               ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, st), 'okmi_rho')
+    names, result = ['ao_eval', 'rho_f64'], (lambda: (rho.cpu().numpy(), delta.cpu().numpy()))
+    dom = ('rho_f64', 'k_rho_f64', 'f64', 1)
+  else:
+    plan = FusedPlan(basis, dg, C, occ, codes, precision=args.precision)
+    step, result = plan.launch, plan.result
+    if nmo > 64:
+      names = ['ao_eval', 'rho_big']
+      dom = ('rho_big', 'k_rho_big', 'f32' if args.precision == 'f32' else 'bf16', PRODUCTS[args.precision])
+    else:
+      names = ['rho_fused']
+      dom = ('rho_fused', 'k_rho_fused', 'f32' if args.precision == 'f32' else 'bf16', PRODUCTS[args.precision])
+  dt, times = timed(step, args, L, names, torch.cuda.synchronize)
+  flops = 2.0 * nmo * nao * nsets * npts   # algorithmic, one pass over the grid
+  cnt, ms = times[dom[0]]
+  t_launch = ms / 1e3 / args.steps
+  achieved = flops / t_launch / 1e12
+  peak = PEAK_TFLOPS[dom[2]] / dom[3]
+  roof = {'bound': 'mfma', 'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s', 'frac': achieved / peak,
+          'traffic': measured_traffic(dom[1], args.workload, args.precision), 'kernel': dom[1],
+          'kernel_ms_per_step': t_launch * 1e3, 'launches_per_step': cnt / args.steps,
+          'algorithmic_flop_per_step': flops}
+  if dom[3] > 1:
+    roof.update(executed_tflops=achieved * dom[3], mfma_dtype=dom[2], products=dom[3])
+  roof['other_kernels_ms_per_step'] = {k: v[1] / args.steps for k, v in times.items() if k != dom[0]}
+  out = {'value': npts * args.steps / dt, 'ms_per_step': dt / args.steps * 1e3,
+         'dtype': {'bf16x6': 'bf16x6 (fp32 acc, fp64 epilogue)', 'bf16x3': 'bf16x3 (fp32 acc, fp64 epilogue)',
+                   'f32': 'f32 (fp64 epilogue)', 'f64': 'f64'}[args.precision],
+         'config': {'workload': '%s: %d atoms, %d spherical AOs (l<=3, 3 prims/shell), %d MOs, %d^3 regular grid, '
+                                'rho + grad rho%s' % (args.workload, info['natoms'], nao, nmo, info['side'],
+                                                      ' + laplacian' if args.laplacian else ''),
+                    'npts': npts, 'nao': nao, 'nmo': nmo, 'ao_sets': nsets, 'precision': args.precision},
+         'roofline': roof}
+  if args.errors or args.precision != 'f64':
+    r, d = result()
+    from orbkit_amd.core import eval_rho
+    rr, dd, _ = eval_rho(basis, dg, C, occ, codes)
+    mask = rr > 1e-10 * rr.max()
+    d3r = numpy.prod([a[1] - a[0] for a in axes])
+    err = {'rho_max_rel': float(numpy.max(numpy.abs(r - rr)[mask] / rr[mask])),
+           'rho_l2_rel': float(numpy.linalg.norm(r - rr) / numpy.linalg.norm(rr)),
+           'grad_l2_rel': float(numpy.linalg.norm(d[:3] - dd[:3]) / numpy.linalg.norm(dd[:3])),
+           'grad_max_rel': float(numpy.max(numpy.abs(d[:3] - dd[:3])) / numpy.max(numpy.abs(dd[:3]))),
+           'electrons': float(numpy.sum(r) * d3r), 'electrons_f64': float(numpy.sum(rr) * d3r)}
+    if args.laplacian:
+      lap, lap0 = d[3:].sum(axis=0), dd[3:].sum(axis=0)
+      err['lap_l2_rel'] = float(numpy.linalg.norm(lap - lap0) / numpy.linalg.norm(lap0))
+    out['error_vs_f64'] = err
+  return out, codes
+
+
+def gradient_step(args, torch, info, qc, axes):
+  '''Forward rho and backward of L = sum d3r (rho - rho_t)^2, FP64.'''
+  from orbkit_amd import _native, workloads
+  from orbkit_amd.autograd import DensityModel
+  L = _native.lib()
+  model = DensityModel(qc, axes)
+  C, alpha, geo = model.parameters()
+  # target from perturbed coefficients C + 0.01 N(0,1) (seed 33)
+  Ct = C.detach() + 0.01 * torch.as_tensor(numpy.random.default_rng(33).normal(size=tuple(C.shape)), device=C.device)
+  rho_t = model.rho(Ct, alpha, geo)
+  d3r = float(numpy.prod([a[1] - a[0] for a in axes]))
+
+  def step():
+    for p in (C, alpha, geo):
+      p.grad = None
+    rho = model(C, alpha, geo)
+    loss = (d3r * (rho - rho_t) ** 2).sum()
+    loss.backward()
+  names = ['rho_f64', 'ao_eval', 'gemm_f64', 'gemm_tn', 'shell_grad', 'rho_backward']
+  dt, times = timed(step, args, L, names, torch.cuda.synchronize)
+  npts, nao, nmo = model.npts, info['nao'], info['nmo']
+  # dominant backward kernel: the dL/dC contraction over points (FP64 MFMA)
+  cnt, ms = times['gemm_tn']
+  t = ms / 1e3 / args.steps
+  flops = 2.0 * nmo * nao * npts
+  roof = {'bound': 'mfma', 'achieved': flops / t / 1e12, 'peak': PEAK_TFLOPS['f64'], 'unit': 'TFLOP/s',
+          'frac': flops / t / 1e12 / PEAK_TFLOPS['f64'], 'traffic': measured_traffic('k_gemm_tn', 'gradient', 'f64'),
+          'kernel': 'k_gemm_tn', 'kernel_ms_per_step': t * 1e3,
+          'other_kernels_ms_per_step': {k: v[1] / args.steps for k, v in times.items() if k != 'gemm_tn'}}
+  return {'value': npts * args.steps / dt, 'ms_per_step': dt / args.steps * 1e3, 'dtype': 'f64',
+          'metric_note': 'forward rho + backward (dL/dC, dL/dalpha, dL/dR) per grid point',
+          'config': {'workload': 'gradient: %d atoms, %d spherical AOs, %d MOs, %d^3 regular grid, '
+                                 'L = sum d3r (rho - rho_t)^2' % (info['natoms'], nao, nmo, info['side']),
+                     'npts': npts, 'nao': nao, 'nmo': nmo, 'nprim': info['nprim']},
+          'roofline': roof}, numpy.array([], dtype=numpy.intc)
+
+
+def batch_step(args, torch):
+  from orbkit_amd import _native, autotune, workloads
+  from orbkit_amd.batch import BatchPlan
+  L = _native.lib()
+  prec = 'bf16x6' if args.precision == 'f64' else args.precision
+  mols = workloads.make_batch(n_mol=256)
+  plan = BatchPlan(mols, precision=prec)
+  tune = autotune.tune_batch(plan, reps=2, use_cache=False)
+  stream = torch.cuda.Stream()
+
+  def step():
+    plan.launch(use_graph=True, stream=stream.cuda_stream)
+  dt, times = timed(step, args, L, ['rho_batch'], stream.synchronize)
+  flops = sum(2.0 * c.shape[0] * b.nao * 4 * g.npts for c, b, g in zip(plan.C, plan.bases, plan.grids))
+  t = times['rho_batch'][1] / 1e3 / args.steps
+  peak = PEAK_TFLOPS['f32' if prec == 'f32' else 'bf16'] / PRODUCTS[prec]
+  roof = {'bound': 'mfma', 'achieved': flops / t / 1e12, 'peak': peak, 'unit': 'TFLOP/s',
+          'frac': flops / t / 1e12 / peak, 'traffic': measured_traffic('k_rho_batch', 'batch', prec),
+          'kernel': 'k_rho_batch', 'kernel_ms_per_step': t * 1e3, 'algorithmic_flop_per_step': flops,
+          'tuned': {'tile_shape': tune['shape'], 'workgroups': tune['workgroups']}}
+  nao = [b.nao for b in plan.bases]
+  return {'value': plan.npts * args.steps / dt, 'ms_per_step': dt / args.steps * 1e3,
+          'dtype': '%s (fp32 acc, fp64 epilogue)' % prec,
+          'config': {'workload': 'batch: 256 molecules, 5-60 atoms x [s,s,p,d] spherical (NAO %d-%d), 8-30 MOs, '
+                                 'regular grids 32^3..80^3, rho + grad rho, one persistent launch in a HIP graph'
+                                 % (min(nao), max(nao)),
+                     'npts': plan.npts, 'nmol': plan.nmol, 'tiles': plan.ntile, 'precision': prec},
+          'roofline': roof}, None
 
 
 def main():
@@ -79,116 +279,38 @@ def main():
           file=sys.stderr)
     sys.exit(2)
   import torch
-  from orbkit_amd import _native, workloads
-  from orbkit_amd.basis import device_basis
-  from orbkit_amd.core import DeviceGrid
-  from orbkit_amd.fused import FusedPlan
-
+  from orbkit_amd import workloads
   torch.cuda.set_device(0)
-  qc, axes, info = workloads.make(args.workload, npts_side=args.side)
-  basis = device_basis(qc.geo_spec, qc.ao_spec)
-  dg = DeviceGrid(*axes, regular=True)
-  C, occ = qc.mo_spec.get_coeffs(), qc.mo_spec.get_occ()
-  codes = numpy.array([1, 2, 3], dtype=numpy.intc)
-  L = _native.lib()
-  npts = dg.npts
-  nao, nmo, nsets = info['nao'], info['nmo'], 4
-
-  if args.precision == 'f64':
-    dev = dg.x.device
-    dC = torch.as_tensor(C, device=dev)
-    docc = torch.as_tensor(occ, device=dev)
-    rho = torch.empty(npts, dtype=torch.float64, device=dev)
-    delta = torch.empty((3, npts), dtype=torch.float64, device=dev)
-    ws = torch.empty(L.okmi_workspace_bytes(basis.handle, nsets, 1 << 17) // 8 + 1, dtype=torch.float64,
-                     device=dev)
-    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-
-    def step():
-      _native.check(L.okmi_rho(basis.handle, ctypes.byref(dg.desc), 0, npts, ctypes.c_void_p(dC.data_ptr()),
-                               ctypes.c_void_p(docc.data_ptr()), nmo, codes.ctypes.data_as(_native.P_int), 3,
-                               ctypes.c_void_p(rho.data_ptr()), ctypes.c_void_p(delta.data_ptr()), npts, None,
-                               ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, st), 'okmi_rho')
-    kernels = ['ao_eval', 'rho_f64']
-    result = lambda: (rho.cpu().numpy(), delta.cpu().numpy())
-  else:
-    plan = FusedPlan(basis, dg, C, occ, codes, precision=args.precision)
-    step = plan.launch
-    kernels = ['rho_fused']
-    result = plan.result
-
   if args.calibrate_copy:
-    src = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device=dg.x.device)
+    src = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device='cuda')
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
     del src, dst
-  for _ in range(args.warmup):
-    step()
-  torch.cuda.synchronize()
-  L.okmi_profile_enable(1)
-  torch.cuda.synchronize()
-  t0 = time.perf_counter()
-  for _ in range(args.steps):
-    step()
-  torch.cuda.synchronize()
-  dt = time.perf_counter() - t0
-  times = {}
-  for k in kernels:
-    cnt, ms = ctypes.c_int(), ctypes.c_double()
-    _native.check(L.okmi_profile_query(k.encode(), ctypes.byref(cnt), ctypes.byref(ms)), 'profile_query')
-    times[k] = (cnt.value, ms.value)
-  L.okmi_profile_enable(0)
-
-  value = npts * args.steps / dt
-  gemm_flops = 2.0 * nmo * nao * nsets * npts   # algorithmic, per launch (one pass over the grid)
-  if args.precision == 'f64':
-    # dominant kernel of the FP64 path: the MFMA contraction with density epilogue
-    cnt, ms = times['rho_f64']
-    t_launch = ms / 1e3 / args.steps
-    achieved = gemm_flops / t_launch / 1e12
-    roof = {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_TFLOPS['f64'], 'unit': 'TFLOP/s',
-            'frac': achieved / PEAK_TFLOPS['f64'],
-            'traffic': measured_traffic('k_rho_f64', args.workload, args.precision), 'kernel': 'k_rho_f64',
-            'launch_ms': t_launch * 1e3, 'other_kernels_ms': {'k_ao_eval': times['ao_eval'][1] / args.steps}}
-    dtype = 'f64'
+  qc = axes = info = None
+  if args.workload == 'batch':
+    res, codes = batch_step(args, torch)
   else:
-    cnt, ms = times['rho_fused']
-    t_launch = ms / 1e3 / args.steps
-    achieved = gemm_flops / t_launch / 1e12
-    nprod = {'bf16x6': 6, 'bf16x3': 3, 'f32': 1}[args.precision]
-    unit_dtype = 'f32' if args.precision == 'f32' else 'bf16'
-    peak = PEAK_TFLOPS[unit_dtype] / nprod
-    roof = {'bound': 'mfma', 'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s', 'frac': achieved / peak,
-            'traffic': measured_traffic('k_rho_fused', args.workload, args.precision), 'kernel': 'k_rho_fused',
-            'launch_ms': t_launch * 1e3,
-            'executed_tflops': achieved * nprod, 'mfma_dtype': unit_dtype, 'products': nprod}
-    dtype = {'bf16x6': 'bf16x6 (fp32 acc, fp64 epilogue)', 'bf16x3': 'bf16x3 (fp32 acc, fp64 epilogue)',
-             'f32': 'f32'}[args.precision]
-
-  out = {'metric': 'grid points/s for rho_compute (+grad rho)', 'value': value, 'unit': 'grid points/s',
-         'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': dt / args.steps * 1e3,
-         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': dtype,
-         'data': 'synthetic (seeded random molecule, basis and orthonormal MOs)',
-         'config': {'workload': '%s: %d atoms, %d spherical AOs (l<=3, 3 prims/shell), %d MOs, %d^3 regular '
-                                'grid, rho + grad rho' % (args.workload, info['natoms'], nao, nmo, info['side']),
-                    'npts': npts, 'nao': nao, 'nmo': nmo, 'ao_sets': nsets, 'precision': args.precision},
-         'roofline': roof}
-  if args.errors or args.precision != 'f64':
-    r, d = result()
-    from orbkit_amd.core import eval_rho
-    rr, dd, _ = eval_rho(basis, dg, C, occ, codes)
-    mask = rr > 1e-10 * rr.max()
-    out['error_vs_f64'] = {
-        'rho_max_rel': float(numpy.max(numpy.abs(r - rr)[mask] / rr[mask])),
-        'rho_l2_rel': float(numpy.linalg.norm(r - rr) / numpy.linalg.norm(rr)),
-        'grad_l2_rel': float(numpy.linalg.norm(d - dd) / numpy.linalg.norm(dd)),
-        'grad_max_rel': float(numpy.max(numpy.abs(d - dd)) / numpy.max(numpy.abs(dd))),
-        'electrons': float(numpy.sum(r) * numpy.prod([a[1] - a[0] for a in axes])),
-        'electrons_f64': float(numpy.sum(rr) * numpy.prod([a[1] - a[0] for a in axes]))}
-  if not args.no_cpu:
+    qc, axes, info = workloads.make(args.workload, npts_side=args.side)
+    if args.workload == 'gradient':
+      res, codes = gradient_step(args, torch, info, qc, axes)
+    else:
+      res, codes = density_step(args, torch, info, qc, axes)
+  out = {'metric': METRIC, 'value': res.pop('value'), 'unit': 'grid points/s', 'n_gpus': 1, 'steps': args.steps,
+         'warmup': args.warmup, 'ms_per_step': res.pop('ms_per_step'), 'higher_is_better': True, 'scaling': 'weak',
+         'vs_baseline': None, 'dtype': res.pop('dtype'),
+         'data': 'synthetic (seeded random molecules, bases and orthonormal MOs)'}
+  out.update(res)
+  if not args.no_cpu and args.workload == 'gradient':
+    out['cpu_baseline'] = cpu_baseline_grad(qc, axes)
+  elif not args.no_cpu and qc is not None:
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    out['cpu_baseline'] = cpu_baseline(qc, axes, threads, int(info['slice_length']))
+    sl = int(info['slice_length'])
+    per = 4 if info['nao'] <= 1000 else 1
+    if args.workload == 'large':
+      sl = 512
+    out['cpu_baseline'] = cpu_baseline(qc, axes, threads, sl, list(codes) if len(codes) else [1, 2, 3],
+                                       slices_per_thread=per)
   print(json.dumps(out))
 
 

@@ -175,6 +175,16 @@ int okmi_rho_backward(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64
                       void* d_ws, size_t ws_bytes, void* stream);
 
 /*
+ * Integrated electron count d_out[0] = d3r * sum_p d_rho[p] in FP64 with
+ * compensated summation (Neumaier per thread, TwoSum merges in a fixed
+ * order; deterministic).  orbkit prints numpy.sum(rho) * d3r
+ * (orbkit/core.py:590-592).
+ */
+size_t okmi_electron_count_workspace_bytes(void);
+int okmi_electron_count(const double* d_rho, int64_t n, double d3r, void* d_ws, size_t ws_bytes,
+                        double* d_out, void* stream);
+
+/*
  * Many molecules in one launch: density (and gradient) of every molecule on
  * its own grid with the fused reduced-precision kernel (okmi_rho_fused's
  * arithmetic; up to 64 MOs per molecule).  Inputs are bound at creation:
@@ -209,7 +219,7 @@ int okmi_batch_destroy(okmi_batch* b);
  * Kernel timing for measurement tools: while enabled, every kernel launch
  * records a pair of HIP events on its stream.  okmi_profile_query sums the
  * elapsed times of the launches of one kernel family ("ao_eval", "gemm_f64",
- * "rho_f64", "rho_fused", "rho_big", "rho_backward", "rho_batch") since the last okmi_profile_enable call.
+ * "rho_f64", "rho_fused", "rho_big", "rho_backward", "gemm_tn", "shell_grad", "rho_batch") since the last okmi_profile_enable call.
  * No counterpart in the reference (orbkit has no kernel timers).
  */
 int okmi_profile_enable(int on);

@@ -44,6 +44,8 @@ SIGNATURES = {
     'okmi_rho_backward_workspace_bytes': (c_sz, [c_vp, c_int, c_i64]),
     'okmi_rho_backward': (c_int, [c_vp, ctypes.POINTER(OkmiGrid), c_i64, c_i64, c_vp, c_vp, c_int, c_vp, c_vp, c_vp,
                                   c_vp, c_vp, c_sz, c_vp]),
+    'okmi_electron_count_workspace_bytes': (c_sz, []),
+    'okmi_electron_count': (c_int, [c_vp, c_i64, c_dbl, c_vp, c_sz, c_vp, c_vp]),
     'okmi_batch_create': (c_int, [c_int, ctypes.POINTER(c_vp), ctypes.POINTER(OkmiGrid), ctypes.POINTER(c_vp),
                                   ctypes.POINTER(c_vp), P_int, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), c_int, c_int,
                                   ctypes.POINTER(c_vp)]),

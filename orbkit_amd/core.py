@@ -165,6 +165,23 @@ def eval_rho(basis, dgrid, coeffs, occ, codes, p0=0, np_=None, mo_norm=False):
   return r, d, (nrm.cpu().numpy() if mo_norm else None)
 
 
+def electron_count(rho, d3r):
+  '''d3r * sum(rho) in FP64 with compensated summation on the device
+  (okmi_electron_count); ``rho`` is a numpy array or a device tensor.'''
+  torch = _torch()
+  L = _native.lib()
+  if not isinstance(rho, torch.Tensor):
+    rho = torch.as_tensor(numpy.ascontiguousarray(rho, dtype=numpy.float64).reshape(-1),
+                          device=torch.device('cuda', torch.cuda.current_device()))
+  rho = rho.reshape(-1).contiguous()
+  ws = torch.empty(L.okmi_electron_count_workspace_bytes() // 8 + 1, dtype=torch.float64, device=rho.device)
+  out = torch.empty(1, dtype=torch.float64, device=rho.device)
+  _native.check(L.okmi_electron_count(ctypes.c_void_p(rho.data_ptr()), rho.numel(), float(d3r),
+                                      ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8,
+                                      ctypes.c_void_p(out.data_ptr()), _stream()), 'okmi_electron_count')
+  return float(out.item())
+
+
 def gemm(A, B):
   '''A (m x k) @ B (k x n) in FP64 on the device (the MO kernel).'''
   torch = _torch()
@@ -368,7 +385,7 @@ def rho_compute(qc, calc_ao=False, calc_mo=False, drv=None, laplacian=False,
     for i, v in enumerate(mo_norm):
       display('\t%.6f\tMO %d' % (v * grid.d3r, i))
   if not was_vector:
-    display('We have ' + str(numpy.sum(rho) * grid.d3r) + ' electrons.')
+    display('We have ' + str(electron_count(rho, grid.d3r)) + ' electrons.')
   rho = rho.reshape(N)
   if not is_drv:
     return rho
@@ -441,7 +458,7 @@ def rho_compute_no_slice(qc, calc_ao=False, calc_mo=False, drv=None, laplacian=F
   rho, delta, _ = eval_rho(basis, dg, C, occ, codes)
   rho = rho.reshape(N)
   if not was_vector:
-    display('We have ' + str(numpy.sum(rho) * d3r) + ' electrons.')
+    display('We have ' + str(electron_count(rho, d3r)) + ' electrons.')
   if drv is None:
     return (ao_list, mo_list, rho) if return_components else rho
   delta_rho = delta.reshape((len(drv),) + N)

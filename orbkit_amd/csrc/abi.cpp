@@ -601,3 +601,16 @@ extern "C" int okmi_batch_destroy(okmi_batch* b)
     okmi_batch_free(b);
     return 0;
 }
+
+extern "C" size_t okmi_electron_count_workspace_bytes(void) { return okmi_electrons_ws_bytes(); }
+
+extern "C" int okmi_electron_count(const double* d_rho, int64_t n, double d3r, void* d_ws, size_t ws_bytes,
+                                   double* d_out, void* stream)
+{
+    g_err.clear();
+    if ((!d_rho && n > 0) || !d_out || n < 0) {
+        okmi_set_error("electron_count: invalid arguments");
+        return 1;
+    }
+    return okmi_launch_electrons(d_rho, n, d3r, d_ws, ws_bytes, d_out, (hipStream_t)stream);
+}

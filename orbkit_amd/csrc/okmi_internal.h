@@ -73,6 +73,11 @@ int okmi_launch_rho_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, 
                              const double* d_occ, int nmo, const double* d_grad, double* d_dc, double* d_dalpha,
                              double* d_dr, void* ws, size_t ws_bytes, hipStream_t st);
 
+// compensated FP64 electron count (electrons.hip)
+size_t okmi_electrons_ws_bytes();
+int okmi_launch_electrons(const double* d_rho, int64_t n, double d3r, void* ws, size_t ws_bytes, double* d_out,
+                          hipStream_t st);
+
 // many molecules in one persistent launch (rho_fused.hip)
 struct OkmiBatch;
 int okmi_batch_build(int nmol, OkmiBasis* const* bases, const GridDesc* grids, const int64_t* npts,

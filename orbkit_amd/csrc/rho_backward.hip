@@ -376,7 +376,10 @@ int okmi_launch_rho_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, 
         int64_t krange = ((n16 + nsplit - 1) / nsplit + 15) & ~(int64_t)15;
         nsplit = (n16 + krange - 1) / krange;
         dim3 grd((unsigned)((b->nao + 16 * NT - 1) / (16 * NT)), (unsigned)((nmo + 15) / 16), (unsigned)nsplit);
-        hipLaunchKernelGGL(k_gemm_tn, grd, dim3(64), 0, st, B, nmo, A, b->nao, n16, chunk, krange, d_dc);
+        {
+            OkmiProfScope p2("gemm_tn", st);
+            hipLaunchKernelGGL(k_gemm_tn, grd, dim3(64), 0, st, B, nmo, A, b->nao, n16, chunk, krange, d_dc);
+        }
         rc = okmi_launch_gemm_f64(B, chunk, 0, 1, nmo, CT, b->nao, n, G, chunk, st);
         if (rc) return rc;
         ShellGradArgs sa{};
@@ -384,8 +387,12 @@ int okmi_launch_rho_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, 
         sa.p0 = p0 + c0; sa.np = n; sa.ld = chunk; sa.nprim_total = b->nprim; sa.is_normalized = b->is_normalized;
         sa.G = G; sa.partial = part; sa.ncol = Lo.ncol;
         const int nrange = (int)((n + RANGE - 1) / RANGE);
-        hipLaunchKernelGGL(k_shell_grad, dim3((unsigned)nrange, (unsigned)b->nshell, (unsigned)((maxp + MAXP - 1) / MAXP)),
-                           dim3(256), 0, st, sa);
+        {
+            OkmiProfScope p3("shell_grad", st);
+            hipLaunchKernelGGL(k_shell_grad,
+                               dim3((unsigned)nrange, (unsigned)b->nshell, (unsigned)((maxp + MAXP - 1) / MAXP)),
+                               dim3(256), 0, st, sa);
+        }
         hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)((Lo.ncol + 255) / 256)), dim3(256), 0, st, part, nrange,
                            Lo.ncol, cols);
         hipLaunchKernelGGL(k_scatter_grad, dim3((unsigned)((b->nprim + 3 * b->natoms + 255) / 256)), dim3(256), 0, st,

@@ -1,0 +1,162 @@
+'''Many molecules in one launch (okmi_batch_*) and the launch-shape tuner.
+
+The batch runs the single-molecule fused kernel's tile body over the tiles of
+every molecule, so each molecule's rho and gradient must equal the
+single-molecule fused path bit for bit (same arithmetic, same order), and
+the FP64 path within the precision's bar.  Graph replays are checked against
+the eager launch, with eager work and synchronisation between replays, and
+after the MO coefficients change in place.
+'''
+import numpy
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def mols():
+  import torch
+  if not torch.cuda.is_available():
+    pytest.skip('no HIP device')
+  from orbkit_amd import workloads
+  ms = workloads.make_batch(n_mol=10, seed=7, side_range=(12, 24))
+  # one molecule on a vector grid whose length is no multiple of 64
+  qc, axes = ms[3]
+  X, Y, Z = numpy.meshgrid(*axes, indexing='ij')
+  idx = numpy.random.default_rng(0).choice(X.size, size=1001, replace=False)
+  ms[3] = (qc, (X.ravel()[idx].copy(), Y.ravel()[idx].copy(), Z.ravel()[idx].copy()), True)
+  return ms
+
+
+def _single(item, precision):
+  from orbkit_amd.basis import device_basis
+  from orbkit_amd.core import DeviceGrid
+  from orbkit_amd.fused import FusedPlan
+  qc, axes = item[0], item[1]
+  vec = len(item) > 2 and item[2]
+  b = device_basis(qc.geo_spec, qc.ao_spec)
+  g = DeviceGrid(*axes, regular=not vec)
+  p = FusedPlan(b, g, qc.mo_spec.get_coeffs(), qc.mo_spec.get_occ(), [1, 2, 3], precision=precision)
+  p.launch()
+  r, d = p.result()
+  return r, d, g.shape
+
+
+@pytest.mark.parametrize('precision', ['bf16x6', 'f32'])
+def test_batch_matches_single(mols, precision):
+  from orbkit_amd.batch import BatchPlan
+  print('case batch', precision, len(mols), flush=True)
+  plan = BatchPlan(mols, precision=precision)
+  plan.launch()
+  for m, item in enumerate(mols):
+    r, d, shape = _single(item, precision)
+    rb, db = plan.result(m)
+    assert numpy.array_equal(rb.reshape(-1), r), m
+    assert numpy.array_equal(db.reshape(3, -1), d), m
+
+
+def test_batch_vs_f64_and_order(mols):
+  from orbkit_amd.basis import device_basis
+  from orbkit_amd.batch import BatchPlan
+  from orbkit_amd.core import DeviceGrid, eval_rho
+  print('case batch f64', flush=True)
+  plan = BatchPlan(mols, precision='bf16x6', order_by_cost=False)
+  plan.launch()
+  for m, item in enumerate(mols):
+    qc, axes = item[0], item[1]
+    vec = len(item) > 2 and item[2]
+    r, d, _ = eval_rho(device_basis(qc.geo_spec, qc.ao_spec), DeviceGrid(*axes, regular=not vec),
+                       qc.mo_spec.get_coeffs(), qc.mo_spec.get_occ(), [1, 2, 3])
+    rb, db = plan.result(m)
+    assert numpy.linalg.norm(rb.reshape(-1) - r) <= 1e-6 * numpy.linalg.norm(r), m
+    assert numpy.linalg.norm(db.reshape(3, -1) - d) <= 1e-6 * numpy.linalg.norm(d), m
+
+
+def test_batch_density_only_and_workgroups(mols):
+  from orbkit_amd.batch import BatchPlan
+  print('case batch rho-only, persistent grids', flush=True)
+  ref = BatchPlan(mols, precision='bf16x6')
+  ref.launch()
+  plan = BatchPlan(mols, precision='bf16x6', grad=False)
+  for w in (0, 7, 256, 1024):
+    plan.set_workgroups(w)
+    plan.launch()
+    for m in range(len(mols)):
+      assert numpy.array_equal(plan.result(m)[0], ref.result(m)[0]), (w, m)
+
+
+def test_batch_graph_replays(mols):
+  import torch
+  from orbkit_amd.batch import BatchPlan
+  print('case batch graph', flush=True)
+  plan = BatchPlan(mols, precision='bf16x6')
+  plan.set_workgroups(512)
+  plan.launch()
+  torch.cuda.synchronize()
+  eager = [plan.result(m) for m in range(len(mols))]
+  s = torch.cuda.Stream()
+  for rep in range(3):
+    for t in plan.rho:
+      t.fill_(-1.0)
+    for t in plan.dgrad:
+      t.fill_(-1.0)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+      plan.launch(use_graph=True, stream=s.cuda_stream)
+    s.synchronize()
+    junk = torch.randn(1 << 20, device='cuda').sum().item()   # eager work between replays
+    assert junk == junk
+    for m in range(len(mols)):
+      r, d = plan.result(m)
+      assert numpy.array_equal(r, eager[m][0]), (rep, m)
+      assert numpy.array_equal(d, eager[m][1]), (rep, m)
+  # C and occupations are re-read on every replay: scaling C by 1.5 scales rho
+  # by 2.25 (up to the bf16 split of the scaled values)
+  for c in plan.C:
+    c.mul_(1.5)
+  with torch.cuda.stream(s):
+    plan.launch(use_graph=True, stream=s.cuda_stream)
+  s.synchronize()
+  for m in range(len(mols)):
+    r, _ = plan.result(m)
+    assert numpy.linalg.norm(r - 2.25 * eager[m][0]) <= 1e-6 * numpy.linalg.norm(eager[m][0]), m
+
+
+def test_tuner(mols, tmp_path, monkeypatch):
+  from orbkit_amd import autotune
+  from orbkit_amd.batch import BatchPlan
+  print('case tuner', flush=True)
+  monkeypatch.setenv('ORBKIT_AMD_TUNE_CACHE', str(tmp_path / 'tune.json'))
+  plan = BatchPlan(mols, precision='bf16x6')
+  r1 = autotune.tune_batch(plan, reps=2)
+  assert not r1['cached'] and tuple(r1['shape']) in autotune.SHAPES
+  assert plan.workgroups == r1['workgroups'] and plan.tile_shape == tuple(r1['shape'])
+  plan2 = BatchPlan(mols, precision='bf16x6')
+  r2 = autotune.tune_batch(plan2, reps=2)
+  assert r2['cached'] and r2['workgroups'] == r1['workgroups'] and plan2.tile_shape == plan.tile_shape
+  # results do not depend on the launch shape beyond screened (< 1e-40) terms
+  ref = BatchPlan(mols, precision='bf16x6')
+  ref.launch()
+  plan2.launch()
+  for m in range(len(mols)):
+    _close(ref.result(m), plan2.result(m), m)
+
+
+def _close(a, b, what):
+  for x, y in zip(a, b):
+    assert numpy.linalg.norm(x - y) <= 1e-12 * numpy.linalg.norm(x), what
+
+
+@pytest.mark.parametrize('shape', [(1, 2, 3), (0, 3, 3), (0, 0, 6), (3, 3, 0)])
+def test_batch_tile_shapes(mols, shape):
+  '''Other bricks screen shells per tile differently; only terms below the
+  screening bound (1e-40) may change.'''
+  from orbkit_amd.batch import BatchPlan
+  print('case batch tile shape', shape, flush=True)
+  ref = BatchPlan(mols, precision='bf16x6')
+  ref.launch()
+  plan = BatchPlan(mols, precision='bf16x6')
+  plan.set_tile_shape(*shape)
+  plan.launch()
+  for m in range(len(mols)):
+    _close(ref.result(m), plan.result(m), (shape, m))

@@ -325,3 +325,18 @@ def test_large_basis_slices(ok):
   mo_o = O.rho_compute(qc, X.ravel(), Y.ravel(), Z.ravel(), True, calc_mo=True)
   assert mo.shape == (400, 8, 8, 8)
   assert rel_err(mo.reshape(400, -1), mo_o) <= TOL
+
+
+def test_electron_count(ok):
+  '''Compensated FP64 device sum vs the exactly rounded math.fsum, on data
+  where plain summation loses digits (a few huge values among many small).'''
+  import math
+  rng = numpy.random.default_rng(5)
+  v = rng.uniform(0.0, 1.0, (1 << 22) + 17) * 10.0 ** rng.integers(-12, 1, (1 << 22) + 17)
+  v[rng.choice(v.size, 5, replace=False)] = 1e15
+  d3r = 0.037
+  got = ok['core'].electron_count(v, d3r)
+  exact = math.fsum(v) * d3r
+  print('electron count', got, exact, abs(got - exact) / exact, flush=True)
+  assert abs(got - exact) <= 4e-16 * exact
+  assert ok['core'].electron_count(numpy.zeros(0), 1.0) == 0.0
