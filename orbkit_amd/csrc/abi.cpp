@@ -20,11 +20,11 @@ int dfact(int n)
 }
 
 // Smallest R (conservative) beyond which every primitive term of a shell,
-// with the polynomial and up to second-derivative factors, is below 1e-40:
-// max_p |d_p| (2l+2) (1+r)^(l+2) (1+2 a_p r)^2 exp(-a_p r^2) < 1e-40.
-double shell_rcut(int l, const double* alpha, const double* d, int n)
+// with the polynomial and up to second-derivative factors, is below `bound`:
+// max_p |d_p| (2l+2) (1+r)^(l+2) (1+2 a_p r)^2 exp(-a_p r^2) < bound.
+double shell_rcut(int l, const double* alpha, const double* d, int n, double bound)
 {
-    const double cut = std::log(1e-40);
+    const double cut = std::log(bound);
     auto logf = [&](double r) {
         double m = -1e300;
         for (int p = 0; p < n; ++p) {
@@ -226,7 +226,7 @@ extern "C" int okmi_basis_create(int ncont, const int* ncart_per_cont, const int
     *out = nullptr;
     std::vector<ShellRec> shells(ncont);
     std::vector<PrimRec> prims;
-    std::vector<double> T, craw;
+    std::vector<double> T, craw, frc2, fprc2;
     std::vector<int> pl;
     const double pref = std::pow(2.0 / M_PI, 0.75);
     int crow = 0, prow = 0, orow = 0, tmat = 0, max_l = 0;
@@ -296,11 +296,15 @@ extern "C" int okmi_basis_create(int ncont, const int* ncart_per_cont, const int
             for (double t : blk) tmax = std::max(tmax, std::fabs(t));
             dd[p] = cc * nh * std::max(1.0, tmax) * nc;
         }
-        const double rc = shell_rcut(l, al.data(), dd.data(), np);
+        const double rc = shell_rcut(l, al.data(), dd.data(), np, OKMI_SCREEN);
         s.rcut2 = rc >= 1e150 ? 1e300 : rc * rc;
+        const double rf = shell_rcut(l, al.data(), dd.data(), np, OKMI_SCREEN_FUSED);
+        frc2.push_back(rf >= 1e150 ? 1e300 : rf * rf);
         for (int p = 0; p < np; ++p) {
-            const double rp = shell_rcut(l, &al[p], &dd[p], 1);
+            const double rp = shell_rcut(l, &al[p], &dd[p], 1, OKMI_SCREEN);
             prims[prims.size() - np + p].rcut2 = rp >= 1e150 ? 1e300 : rp * rp;
+            const double rq = shell_rcut(l, &al[p], &dd[p], 1, OKMI_SCREEN_FUSED);
+            fprc2.push_back(rq >= 1e150 ? 1e300 : rq * rq);
         }
         if (l > max_l) max_l = l;
         crow += nc; prow += np; orow += nrow;
@@ -319,6 +323,8 @@ extern "C" int okmi_basis_create(int ncont, const int* ncart_per_cont, const int
     std::memcpy(b->h_c, craw.data(), sizeof(double) * prow);
     b->h_prim_l = new int[prow];
     std::memcpy(b->h_prim_l, pl.data(), sizeof(int) * prow);
+    b->h_frcut2.assign(frc2.begin(), frc2.end());
+    b->h_fprcut2.assign(fprc2.begin(), fprc2.end());
     int rc = dev_upload(&b->d_shells, shells.data(), shells.size(), "basis_create: shells");
     if (!rc) rc = dev_upload(&b->d_prims, prims.data(), prims.size(), "basis_create: primitives");
     if (!rc) rc = dev_upload(&b->d_T, T.data(), T.size(), "basis_create: transform");
@@ -338,6 +344,9 @@ extern "C" int okmi_basis_destroy(okmi_basis* b)
     if (b->d_T) (void)hipFree(b->d_T);
     if (b->d_chunks) (void)hipFree(b->d_chunks);
     if (b->d_work) (void)hipFree(b->d_work);
+    if (b->d_fshells) (void)hipFree(b->d_fshells);
+    if (b->d_fprims) (void)hipFree(b->d_fprims);
+    if (b->d_colmap) (void)hipFree(b->d_colmap);
     delete[] b->h_shells;
     delete[] b->h_prims;
     delete[] b->h_T;

@@ -24,16 +24,40 @@ struct AoArgs {
     int drv[OKMI_MAX_SETS];
 };
 
+// squared distance from a shell centre to the wave's points' bounding box
+// (wave-uniform); lo/hi are wave minima/maxima of the point coordinates
+__device__ __forceinline__ double box_dist2(const double (&lo)[3], const double (&hi)[3], const ShellRec& sh)
+{
+    const double c[3] = {sh.cx, sh.cy, sh.cz};
+    double d2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double d = fmax(fmax(lo[k] - c[k], c[k] - hi[k]), 0.0);
+        d2 = fma(d, d, d2);
+    }
+    return d2;
+}
+
+// FP64 outputs use the libm-accurate exponential (parity path); FP32
+// outputs the Cody-Waite one (relative error < 7e-12, far below FP32)
+template <typename OutT>
+__device__ __forceinline__ double gauss_out(double t)
+{
+    if constexpr (sizeof(OutT) == 8) return gauss_f64(t);
+    else return gauss_f64_fast(t);
+}
+
 template <int L, typename OutT>
-__device__ __forceinline__ void shell_out(const AoArgs& a, const ShellRec& sh, double px, double py, double pz,
-                                          int64_t i, bool act, OutT* out)
+__device__ __forceinline__ void shell_out(const AoArgs& a, const ShellRec& sh, double d2, double px, double py,
+                                          double pz, int64_t i, bool act, OutT* out)
 {
     const double X = px - sh.cx, Y = py - sh.cy, Z = pz - sh.cz;
     const double rr = X * X + Y * Y + Z * Z;
     double S0 = 0.0, S1 = 0.0, S2 = 0.0;
     for (int p = 0; p < sh.nprim; ++p) {
         const PrimRec pr = load_prim(a.prims, sh.prim0 + p);
-        const double de = pr.d * gauss_f64(pr.alpha * rr);
+        if (d2 >= pr.rcut2) continue;   // below 1e-40 for every point of the wave
+        const double de = pr.d * gauss_out<OutT>(pr.alpha * rr);
         S0 += de;
         if (a.need1) S1 += (-2.0 * pr.alpha) * de;
         if (a.need2) S2 += (4.0 * pr.alpha * pr.alpha) * de;
@@ -62,14 +86,34 @@ __global__ __launch_bounds__(256) void k_ao_eval(AoArgs a, OutT* __restrict__ ou
     const bool act = i < a.np;
     double px, py, pz;
     grid_point(a.grid, a.p0 + (act ? i : 0), px, py, pz);
+    // bounding box of the wave's points (inactive lanes repeat point 0,
+    // which only enlarges the box)
+    double lo[3] = {px, py, pz}, hi[3] = {px, py, pz};
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fmin(lo[k], __shfl_xor(lo[k], o));
+            hi[k] = fmax(hi[k], __shfl_xor(hi[k], o));
+        }
     for (int s = 0; s < a.nshell; ++s) {
         const ShellRec sh = load_shell(a.shells, s);
+        const double d2 = __longlong_as_double(__builtin_amdgcn_readfirstlane(__double_as_longlong(box_dist2(lo, hi, sh))));
+        if (d2 >= sh.rcut2) {
+            // every value and derivative of the shell is below 1e-40 here
+            if (act)
+                for (int j = 0; j < a.ndrv; ++j) {
+                    OutT* o = out + j * a.set_stride + (int64_t)sh.row0 * a.ld + i;
+                    for (int r = 0; r < sh.nrow; ++r) o[(int64_t)r * a.ld] = (OutT)0;
+                }
+            continue;
+        }
         switch (sh.l) {
-        case 0: shell_out<0, OutT>(a, sh, px, py, pz, i, act, out); break;
-        case 1: shell_out<1, OutT>(a, sh, px, py, pz, i, act, out); break;
-        case 2: shell_out<2, OutT>(a, sh, px, py, pz, i, act, out); break;
-        case 3: shell_out<3, OutT>(a, sh, px, py, pz, i, act, out); break;
-        default: shell_out<4, OutT>(a, sh, px, py, pz, i, act, out); break;
+        case 0: shell_out<0, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
+        case 1: shell_out<1, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
+        case 2: shell_out<2, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
+        case 3: shell_out<3, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
+        default: shell_out<4, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
         }
     }
 }

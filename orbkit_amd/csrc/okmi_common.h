@@ -67,6 +67,14 @@ __device__ __forceinline__ void grid_point(const GridDesc& g, int64_t p, double&
 // Read-only tables are read through the constant address space: loads with
 // wave-uniform addresses then become scalar loads (the compiler cannot prove
 // on its own that the kernel's stores leave the tables untouched).
+// Screening bounds: every AO value and derivative term of a shell or
+// primitive is below the bound beyond its radius.  The FP64 paths skip terms
+// below 1e-40; the reduced-precision fused kernels, whose AO values are
+// rounded to FP32, below 1e-24 (far below the FP32 rounding of any density
+// or derivative value they report).
+#define OKMI_SCREEN 1e-40
+#define OKMI_SCREEN_FUSED 1e-24
+
 #define OKMI_CONST __attribute__((address_space(4)))
 template <typename T>
 __device__ __forceinline__ const OKMI_CONST T* cptr(const T* p)

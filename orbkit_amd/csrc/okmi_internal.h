@@ -25,10 +25,18 @@ struct OkmiBasis {
     double* h_T = nullptr;
     double* h_c = nullptr;      // raw contraction coefficients per primitive
     int* h_prim_l = nullptr;    // l per primitive
-    // fused-kernel tables (built on first use)
-    int2* d_chunks = nullptr;
+    // fused-kernel tables (built on first use).  The fused kernels use their
+    // own row order: each shell's rows lie inside one 32-row chunk (zero rows
+    // pad the chunks), colmap maps internal rows to output rows (-1: pad),
+    // and screening radii for the bound OKMI_SCREEN_FUSED.
+    int2* d_chunks = nullptr;   // per chunk: first work entry, shells | used rows << 16
     int* d_work = nullptr;
+    ShellRec* d_fshells = nullptr;   // row0 = internal row, rcut2 = fused radius
+    PrimRec* d_fprims = nullptr;     // rcut2 = fused radius
+    int* d_colmap = nullptr;         // [nkpad]
     int nchunk = 0;
+    int nkpad = 0;
+    std::vector<double> h_frcut2, h_fprcut2;
     int is_normalized = 0;
 };
 

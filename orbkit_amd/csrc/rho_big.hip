@@ -382,7 +382,7 @@ int okmi_launch_rho_big(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64
     const int nmt = (int)(L.mpad / MB);
 
     hipLaunchKernelGGL(k_prep_c, dim3((unsigned)((L.mpad * L.kpad + 255) / 256)), dim3(256), 0, st, d_c, nmo, b->nao,
-                       prec, L.mpad, L.kpad, cpl);
+                       prec, L.mpad, L.kpad, (const int*)nullptr, cpl);
     hipLaunchKernelGGL(k_pad_occ_big, dim3((unsigned)((L.mpad + 255) / 256)), dim3(256), 0, st, d_occ, nmo,
                        (int)L.mpad, occp);
     int codes[10];

@@ -139,26 +139,24 @@ struct Box {
 };
 
 template <int L, int R>
-__device__ __forceinline__ void sph_rows(const double* c, float* dst, int rlo, int rhi)
+__device__ __forceinline__ void sph_rows(const double* c, float* dst)
 {
     if constexpr (L <= 3 && R < 2 * L + 1) {
-        if (R >= rlo && R < rhi) {
-            constexpr SphRow row = sph_row(L, R);
-            double v = row.t[0].v * c[row.t[0].c];
-            if constexpr (row.n > 1) v = fma(row.t[1].v, c[row.t[1].c], v);
-            if constexpr (row.n > 2) v = fma(row.t[2].v, c[row.t[2].c], v);
-            dst[R * LDS_LD] = (float)v;
-        }
-        sph_rows<L, R + 1>(c, dst, rlo, rhi);
+        constexpr SphRow row = sph_row(L, R);
+        double v = row.t[0].v * c[row.t[0].c];
+        if constexpr (row.n > 1) v = fma(row.t[1].v, c[row.t[1].c], v);
+        if constexpr (row.n > 2) v = fma(row.t[2].v, c[row.t[2].c], v);
+        dst[R * LDS_LD] = (float)v;
+        sph_rows<L, R + 1>(c, dst);
     }
 }
 
 template <int L, int R>
-__device__ __forceinline__ void cart_rows(const double* c, float* dst, int rlo, int rhi)
+__device__ __forceinline__ void cart_rows(const double* c, float* dst)
 {
     if constexpr (R < ncart(L)) {
-        if (R >= rlo && R < rhi) dst[R * LDS_LD] = (float)(c[R] * cart_norm(L, R));
-        cart_rows<L, R + 1>(c, dst, rlo, rhi);
+        dst[R * LDS_LD] = (float)(c[R] * cart_norm(L, R));
+        cart_rows<L, R + 1>(c, dst);
     }
 }
 
@@ -183,7 +181,6 @@ __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const Shell
     ShellPows<L, double> P;
     P.init(X, Y, Z);
     constexpr int NC = ncart(L);
-    const int rlo = max(0, k0 - sh.row0), rhi = min(sh.nrow, k0 + KC - sh.row0);
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         double c[NC];
@@ -201,12 +198,12 @@ __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const Shell
         }
         float* dst = lds + s * KC * LDS_LD + (sh.row0 - k0) * LDS_LD + lane;
         if (sh.kind == SHELL_T_SPHERICAL) {
-            sph_rows<L, 0>(c, dst, rlo, rhi);
+            sph_rows<L, 0>(c, dst);
         } else if (sh.kind == SHELL_T_CARTESIAN) {
-            cart_rows<L, 0>(c, dst, rlo, rhi);
+            cart_rows<L, 0>(c, dst);
         } else {
             const OKMI_CONST double* T = cptr(a.T) + sh.t0;
-            for (int r = rlo; r < rhi; ++r) {
+            for (int r = 0; r < sh.nrow; ++r) {
                 double v = 0.0;
 #pragma unroll
                 for (int k = 0; k < NC; ++k) v = fma(T[r * NC + k], c[k], v);
@@ -252,8 +249,9 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float
     for (int ch = 0; ch < a.nchunk; ++ch) {
         const int k0 = ch * KC;
         const int2 cw = load_int2(a.chunks, ch);
+        const int nsh = cw.y & 0xffff, used = cw.y >> 16;
         bool any = false;
-        for (int e = cw.x; e < cw.x + cw.y; ++e) any |= box.reaches(load_shell(a.shells, cptr(a.work)[e] & 0xffffff));
+        for (int e = cw.x; e < cw.x + nsh; ++e) any |= box.reaches(load_shell(a.shells, cptr(a.work)[e] & 0xffffff));
         if (!any) continue;   // every AO row of this chunk is zero on the tile (same decision in all waves)
         float* L = lds + buf * BUF;
         // operand A (MO coefficients) for this chunk
@@ -278,9 +276,9 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float
         // AO rows [k0, k0+KC): the waves take the chunk's shells one at a
         // time from a shared counter (costliest first), so that screening
         // cannot leave one wave with most of the work
-        if (wave == 0 && k0 + KC > a.nao) {
+        if (wave == 0 && used < KC) {   // pad rows after the chunk's last shell
             for (int s = 0; s < NS; ++s)
-                for (int r = a.nao - k0; r < KC; ++r) L[s * KC * LDS_LD + r * LDS_LD + lane] = 0.f;
+                for (int r = used; r < KC; ++r) L[s * KC * LDS_LD + r * LDS_LD + lane] = 0.f;
         }
         // the other buffer's counter: every wave finished taking from it
         // before the previous barrier, the next chunk reads it after the next
@@ -289,7 +287,7 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float
             int e = 0;
             if (lane == 0) e = atomicAdd(&next[buf], 1);
             e = __builtin_amdgcn_readfirstlane(__shfl(e, 0));
-            if (e >= cw.y) break;
+            if (e >= nsh) break;
             const ShellRec sh = load_shell(a.shells, cptr(a.work)[cw.x + e] & 0xffffff);
             const double d2 = box.dist2(sh);
             if (d2 < sh.rcut2) {
@@ -301,9 +299,8 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float
                 default: eval_shell_chunk<4, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
                 }
             } else {
-                const int rlo = max(0, k0 - sh.row0), rhi = min(sh.nrow, k0 + KC - sh.row0);
                 for (int s = 0; s < NS; ++s)
-                    for (int r = rlo; r < rhi; ++r) L[s * KC * LDS_LD + (sh.row0 - k0 + r) * LDS_LD + lane] = 0.f;
+                    for (int r = 0; r < sh.nrow; ++r) L[s * KC * LDS_LD + (sh.row0 - k0 + r) * LDS_LD + lane] = 0.f;
             }
         }
         // All rows of buffer `buf` written; the other buffer is free because
@@ -461,51 +458,89 @@ int launch_pass(const FusedArgs& a, int pass, int mt, hipStream_t st)
 namespace {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+template <typename T>
+int upload(T** d, const std::vector<T>& h, const char* what)
+{
+    hipError_t e = hipMalloc((void**)d, sizeof(T) * std::max<size_t>(1, h.size()));
+    if (e != hipSuccess) return okmi_hip_fail(e, what);
+    e = hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return okmi_hip_fail(e, what);
+    return 0;
+}
+
+// Internal row layout of the fused kernels: shells in order, and a shell
+// that would cross a 32-row chunk boundary starts the next chunk, so every
+// chunk holds whole shells.  Returns the padded row count (-1 if a shell
+// has more than 32 rows) and each shell's first internal row.
+int fused_layout(const OkmiBasis* b, std::vector<int>* krow)
+{
+    int k = 0;
+    if (krow) krow->assign(b->nshell, 0);
+    for (int s = 0; s < b->nshell; ++s) {
+        const int n = b->h_shells[s].nrow;
+        if (n > KC) return -1;
+        if (k % KC + n > KC) k = (k + KC - 1) / KC * KC;
+        if (krow) (*krow)[s] = k;
+        k += n;
+    }
+    return (k + KC - 1) / KC * KC;
+}
+
 size_t fused_fixed_bytes(const OkmiBasis* b, int nmo, int prec)
 {
-    const size_t mpad = ((nmo + 15) / 16) * 16, kpad = ((b->nao + KC - 1) / KC) * KC;
+    const size_t mpad = ((nmo + 15) / 16) * 16, kpad = (size_t)std::max(KC, fused_layout(b, nullptr));
     const size_t elem = prec == OKMI_PREC_F32 ? 4 : prec == OKMI_PREC_BF16X3 ? 4 : 6;   // bytes per C entry
     return align256(elem * mpad * kpad) + align256(mpad * 8);
 }
 
-// Per chunk of KC rows: the shells overlapping it, each assigned to one of
-// the four waves of a workgroup so that the estimated evaluation cost
-// (primitives, Cartesian components, output rows) is balanced.
+// Fused tables: per chunk of KC internal rows the shells inside it, each
+// chunk's work list sorted by estimated evaluation cost (primitives,
+// Cartesian components, output rows; costliest first, taken dynamically by
+// the waves), copies of the shell and primitive tables with internal rows and
+// the fused screening radii, and the internal-row -> output-row map.
 int prepare_fused_tables(OkmiBasis* b)
 {
     if (b->d_chunks) return 0;
-    const int nchunk = (b->nao + KC - 1) / KC;
+    std::vector<int> krow;
+    const int nkpad = fused_layout(b, &krow);
+    if (nkpad < 0) {
+        okmi_set_error("rho_fused: a contraction has more than %d output rows", KC);
+        return 1;
+    }
+    const int nchunk = nkpad / KC;
     std::vector<int2> ch(nchunk);
     std::vector<int> work;
     int s = 0;
     for (int c = 0; c < nchunk; ++c) {
-        const int r0 = c * KC, r1 = std::min(b->nao, r0 + KC) - 1;
-        while (s < b->nshell && b->h_shells[s].row0 + b->h_shells[s].nrow <= r0) ++s;
-        int e = s;
-        while (e + 1 < b->nshell && b->h_shells[e + 1].row0 <= r1) ++e;
         std::vector<std::pair<int, int>> cost;
-        for (int i = s; i <= e; ++i) {
-            const ShellRec& sh = b->h_shells[i];
-            cost.push_back({16 * sh.nprim + 8 * (ncart(sh.l) + sh.nrow), i});
+        int used = 0;
+        while (s < b->nshell && krow[s] < (c + 1) * KC) {
+            const ShellRec& sh = b->h_shells[s];
+            cost.push_back({16 * sh.nprim + 8 * (ncart(sh.l) + sh.nrow), s});
+            used = krow[s] + sh.nrow - c * KC;
+            ++s;
         }
         std::sort(cost.begin(), cost.end(), [](auto& x, auto& y) { return x.first > y.first; });
-        int load[4] = {0, 0, 0, 0};
-        ch[c] = make_int2((int)work.size(), (int)cost.size());
-        for (auto& cs : cost) {
-            const int w = (int)(std::min_element(load, load + 4) - load);
-            load[w] += cs.first;
-            work.push_back(cs.second | (w << 24));
-        }
+        ch[c] = make_int2((int)work.size(), (int)cost.size() | (used << 16));
+        for (auto& cs : cost) work.push_back(cs.second);
     }
-    hipError_t e = hipMalloc((void**)&b->d_chunks, sizeof(int2) * nchunk);
-    if (e != hipSuccess) return okmi_hip_fail(e, "rho_fused: chunk map");
-    e = hipMemcpy(b->d_chunks, ch.data(), sizeof(int2) * nchunk, hipMemcpyHostToDevice);
-    if (e != hipSuccess) return okmi_hip_fail(e, "rho_fused: chunk map");
-    e = hipMalloc((void**)&b->d_work, sizeof(int) * work.size());
-    if (e != hipSuccess) return okmi_hip_fail(e, "rho_fused: work list");
-    e = hipMemcpy(b->d_work, work.data(), sizeof(int) * work.size(), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return okmi_hip_fail(e, "rho_fused: work list");
+    std::vector<ShellRec> fs(b->h_shells, b->h_shells + b->nshell);
+    std::vector<int> colmap(nkpad, -1);
+    for (int i = 0; i < b->nshell; ++i) {
+        for (int r = 0; r < fs[i].nrow; ++r) colmap[krow[i] + r] = fs[i].row0 + r;
+        fs[i].row0 = krow[i];
+        fs[i].rcut2 = b->h_frcut2[i];
+    }
+    std::vector<PrimRec> fp(b->h_prims, b->h_prims + b->nprim);
+    for (int i = 0; i < b->nprim; ++i) fp[i].rcut2 = b->h_fprcut2[i];
+    int rc = upload(&b->d_chunks, ch, "rho_fused: chunk map");
+    if (!rc) rc = upload(&b->d_work, work, "rho_fused: work list");
+    if (!rc) rc = upload(&b->d_fshells, fs, "rho_fused: shells");
+    if (!rc) rc = upload(&b->d_fprims, fp, "rho_fused: primitives");
+    if (!rc) rc = upload(&b->d_colmap, colmap, "rho_fused: row map");
+    if (rc) return rc;
     b->nchunk = nchunk;
+    b->nkpad = nkpad;
     return 0;
 }
 }  // namespace
@@ -546,7 +581,7 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
     const bool brick = g.regular && p0 == 0 && np == g.nx * g.ny * g.nz && chunk >= np;
     if (chunk >= np) chunk = np;
     else chunk &= ~(int64_t)63;
-    const int64_t mpad = mt * 16, kpad = (int64_t)b->nchunk * KC;
+    const int64_t mpad = mt * 16, kpad = b->nkpad;
     char* w = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
     void* cpl = w;
     const size_t elem = prec == OKMI_PREC_F32 ? 4 : prec == OKMI_PREC_BF16X3 ? 4 : 6;
@@ -555,7 +590,7 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
     double* term2 = term + 10 * chunk;
 
     hipLaunchKernelGGL(k_prep_c, dim3((unsigned)((mpad * kpad + 255) / 256)), dim3(256), 0, st, d_c, nmo, b->nao,
-                       prec, mpad, kpad, cpl);
+                       prec, mpad, kpad, (const int*)b->d_colmap, cpl);
     hipLaunchKernelGGL(k_pad_occ, dim3(1), dim3(256), 0, st, d_occ, nmo, (int)mpad, occp);
 
     int need = 0;
@@ -572,7 +607,7 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
     const bool p1 = (need & ~1) != 0, p2 = (need & (7 << 4)) != 0, p3 = (need & (7 << 7)) != 0;
 
     FusedArgs a{};
-    a.shells = b->d_shells; a.prims = b->d_prims; a.T = b->d_T; a.chunks = b->d_chunks; a.work = b->d_work;
+    a.shells = b->d_fshells; a.prims = b->d_fprims; a.T = b->d_T; a.chunks = b->d_chunks; a.work = b->d_work;
     a.grid = g; a.nao = b->nao; a.nchunk = b->nchunk;
     a.C = cpl; a.kpad = kpad; a.mpad = mpad; a.occ = occp;
     a.term = term; a.term2 = term2; a.ld_term = chunk; a.need_mask = need;
@@ -621,6 +656,7 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
 namespace {
 
 struct PrepItem {
+    const int* colmap;    // [kpad] internal row -> C column (-1: zero)
     const double* C;      // [nmo][nao]
     const double* occ;    // [nmo]
     void* planes;         // [planes][mpad][kpad]
@@ -693,19 +729,9 @@ __global__ void k_prep_batch(const PrepItem* items, const int64_t* start, int ni
             continue;
         }
         const int64_t m = i / it.kpad, k = i - m * it.kpad;
-        const double v = (m < it.nmo && k < it.nao) ? it.C[m * it.nao + k] : 0.0;
-        if (prec == OKMI_PREC_F32) {
-            ((float*)it.planes)[i] = (float)v;
-        } else {
-            __bf16* o = (__bf16*)it.planes;
-            const float x = (float)v;
-            const __bf16 h = (__bf16)x;
-            const float r1 = x - (float)h;
-            const __bf16 md = (__bf16)r1;
-            o[i] = h;
-            o[nc + i] = md;
-            if (prec == OKMI_PREC_BF16X6) o[2 * nc + i] = (__bf16)(r1 - (float)md);
-        }
+        const int src = it.colmap[k];
+        const double v = (m < it.nmo && src >= 0) ? it.C[m * it.nao + src] : 0.0;
+        store_c_split(v, prec, i, nc, it.planes);
     }
 }
 
@@ -727,16 +753,6 @@ template <int PREC>
 int launch_batch(const BatchArgs& a, int pass, int mt, unsigned grid, hipStream_t st)
 {
     return pass == 0 ? launch_batch_mt<PREC, 0>(a, mt, grid, st) : launch_batch_mt<PREC, 1>(a, mt, grid, st);
-}
-
-template <typename T>
-int upload(T** d, const std::vector<T>& h, const char* what)
-{
-    hipError_t e = hipMalloc((void**)d, sizeof(T) * std::max<size_t>(1, h.size()));
-    if (e != hipSuccess) return okmi_hip_fail(e, what);
-    e = hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return okmi_hip_fail(e, what);
-    return 0;
 }
 
 }  // namespace
@@ -860,7 +876,7 @@ int okmi_batch_build(int nmol, OkmiBasis* const* bases, const GridDesc* grids, c
     for (int j = 0; j < nmol; ++j) {
         const int m = B->order[j];
         poff[j] = pbytes;
-        pbytes += (elem * B->mpad * ((int64_t)bases[m]->nchunk * KC) + 255) & ~(size_t)255;
+        pbytes += (elem * B->mpad * (int64_t)bases[m]->nkpad + 255) & ~(size_t)255;
     }
     hipError_t e = hipMalloc(&B->d_planes, std::max<size_t>(pbytes, 256));
     if (e == hipSuccess) e = hipMalloc((void**)&B->d_occ, sizeof(double) * B->mpad * nmol);
@@ -876,11 +892,11 @@ int okmi_batch_build(int nmol, OkmiBasis* const* bases, const GridDesc* grids, c
         const OkmiBasis* bs = bases[m];
         const GridDesc& g = grids[m];
         FusedArgs a{};
-        a.shells = bs->d_shells; a.prims = bs->d_prims; a.T = bs->d_T;
+        a.shells = bs->d_fshells; a.prims = bs->d_fprims; a.T = bs->d_T;
         a.chunks = bs->d_chunks; a.work = bs->d_work;
         a.grid = g; a.p0 = 0; a.np = npts[m];
         a.nao = bs->nao; a.nchunk = bs->nchunk;
-        a.kpad = (int64_t)bs->nchunk * KC; a.mpad = B->mpad;
+        a.kpad = bs->nkpad; a.mpad = B->mpad;
         a.C = (char*)B->d_planes + poff[j];
         a.occ = B->d_occ + (int64_t)j * B->mpad;
         a.rho = d_rho[m];
@@ -893,7 +909,7 @@ int okmi_batch_build(int nmol, OkmiBasis* const* bases, const GridDesc* grids, c
         a.brick = g.regular;
         args[j] = a;
         PrepItem it{};
-        it.C = d_c[m]; it.occ = d_occ[m]; it.planes = (void*)a.C; it.occ_pad = (double*)a.occ;
+        it.colmap = bs->d_colmap; it.C = d_c[m]; it.occ = d_occ[m]; it.planes = (void*)a.C; it.occ_pad = (double*)a.occ;
         it.nmo = nmo[m]; it.nao = bs->nao; it.kpad = a.kpad;
         items[j] = it;
         istart[j + 1] = istart[j] + B->mpad * a.kpad + B->mpad;

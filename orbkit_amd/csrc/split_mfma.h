@@ -72,14 +72,10 @@ __device__ __forceinline__ f4 mfma_split(const Split<PREC>& A, const Split<PREC>
 
 namespace {
 
-// C (FP64 [nmo][nao]) -> padded operand planes [plane][mpad][kpad] (bf16,
-// hi and mid rounded to nearest) or [mpad][kpad] FP32
-__global__ void k_prep_c(const double* C, int nmo, int nao, int prec, int64_t mpad, int64_t kpad, void* out)
+// one entry of the padded operand planes: [plane][mpad][kpad] bf16 (hi and
+// mid rounded to nearest; nc = mpad * kpad entries per plane) or FP32
+__device__ inline void store_c_split(double v, int prec, int64_t i, int64_t nc, void* out)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= mpad * kpad) return;
-    const int64_t m = i / kpad, k = i - m * kpad;
-    const double v = (m < nmo && k < nao) ? C[m * nao + k] : 0.0;
     if (prec == OKMI_PREC_F32) {
         ((float*)out)[i] = (float)v;
     } else {
@@ -89,9 +85,22 @@ __global__ void k_prep_c(const double* C, int nmo, int nao, int prec, int64_t mp
         const float r1 = x - (float)h;
         const __bf16 md = (__bf16)r1;
         o[i] = h;
-        o[mpad * kpad + i] = md;
-        if (prec == OKMI_PREC_BF16X6) o[2 * mpad * kpad + i] = (__bf16)(r1 - (float)md);
+        o[nc + i] = md;
+        if (prec == OKMI_PREC_BF16X6) o[2 * nc + i] = (__bf16)(r1 - (float)md);
     }
+}
+
+// C (FP64 [nmo][nao]) -> padded operand planes; with a column map, internal
+// column k takes C's column colmap[k] (-1: zero)
+__global__ void k_prep_c(const double* C, int nmo, int nao, int prec, int64_t mpad, int64_t kpad, const int* colmap,
+                         void* out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= mpad * kpad) return;
+    const int64_t m = i / kpad, k = i - m * kpad;
+    const int64_t src = colmap ? colmap[k] : (k < nao ? k : -1);
+    const double v = (m < nmo && src >= 0) ? C[m * nao + src] : 0.0;
+    store_c_split(v, prec, i, mpad * kpad, out);
 }
 
 }  // namespace
