@@ -185,6 +185,51 @@ int okmi_electron_count(const double* d_rho, int64_t n, double d3r, void* d_ws, 
                         double* d_out, void* stream);
 
 /*
+ * Gross atomic densities (orbkit/extras.py:306-385) on points [p0, p0+np):
+ *   d_out[j][p] = sum_i occ_i phi_i(p) phi_i^(j)(p),
+ *   phi_i^(j) = sum_{mu in rows of selection j} C[i][mu] chi_mu,
+ * with chi the AOs of derivative code drv (orbkit's drv argument); the AO
+ * rows of selection j are sel_rows[sel_start[j] .. sel_start[j+1]) (host
+ * arrays; usually the rows of one atom).  Evaluated as sum_{mu in j}
+ * chi_mu (D chi)_mu with the density matrix D = C^T diag(occ) C (FP64).
+ */
+size_t okmi_gross_atomic_workspace_bytes(const okmi_basis* b, int64_t chunk, int nrows_total, int nsel);
+int okmi_gross_atomic_density(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np, int drv,
+                              const double* d_c, const double* d_occ, int nmo,
+                              const int* sel_start, const int* sel_rows, int nsel,
+                              double* d_out, int64_t ld_out, void* d_ws, size_t ws_bytes, void* stream);
+/*
+ * The same from caller-supplied AO values d_ao [nao][ld] and MO values
+ * d_mo [nmo][ld] (orbkit's ao_list / mo_list arguments) for one selection
+ * of AO rows d_rows (device array): d_out[p] = sum_i occ_i mo_i(p)
+ * sum_{r in rows} C[i][r] ao_r(p).
+ */
+int okmi_masked_mo_product(const double* d_c, const double* d_occ, int nmo, int nao, const double* d_ao,
+                           const double* d_mo, int64_t np, int64_t ld, const int* d_rows, int nrows,
+                           double* d_out, void* stream);
+/*
+ * MO pair products (orbkit/core.py:935-939, calc_mo_matrix):
+ *   d_out[d][n][m][p] = d_bra[n][p] * d_ket[d][m][p]   (row strides ld_in, ld_out)
+ */
+int okmi_mo_pair_products(const double* d_bra, int na, const double* d_ket, int nb, int nd, int64_t np,
+                          int64_t ld_in, double* d_out, int64_t ld_out, void* stream);
+/*
+ * Transition electronic flux densities (orbkit/extras.py:441-492, calc_jmo):
+ *   d_out[d][k][p] = -1/2 (mo[0][i][p] mo[1+d][j][p] - mo[0][j][p] mo[1+d][i][p])
+ * for pairs (i, j) = d_pairs[2k], d_pairs[2k+1] (device array); d_mo is
+ * [1 + nd][nmo][ld_in]: MO values, then nd derivative sets.
+ */
+int okmi_transition_flux(const double* d_mo, int nmo, int nd, int64_t np, int64_t ld_in,
+                         const int* d_pairs, int npair, double* d_out, int64_t ld_out, void* stream);
+/*
+ * Text body of a Gaussian cube file as orbkit writes it
+ * (orbkit/output/cube.py:88-96); host function.  data [nsets][nx][ny][nz].
+ * With out == NULL returns the buffer size needed; else the number of bytes
+ * written (0 if cap is too small); the text is NUL-terminated.
+ */
+size_t okmi_cube_format(const double* data, int nsets, int64_t nx, int64_t ny, int64_t nz, char* out, size_t cap);
+
+/*
  * Many molecules in one launch: density (and gradient) of every molecule on
  * its own grid with the fused reduced-precision kernel (okmi_rho_fused's
  * arithmetic; up to 64 MOs per molecule).  Inputs are bound at creation:

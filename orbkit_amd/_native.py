@@ -46,6 +46,13 @@ SIGNATURES = {
                                   c_vp, c_vp, c_sz, c_vp]),
     'okmi_electron_count_workspace_bytes': (c_sz, []),
     'okmi_electron_count': (c_int, [c_vp, c_i64, c_dbl, c_vp, c_sz, c_vp, c_vp]),
+    'okmi_gross_atomic_workspace_bytes': (c_sz, [c_vp, c_i64, c_int, c_int]),
+    'okmi_gross_atomic_density': (c_int, [c_vp, ctypes.POINTER(OkmiGrid), c_i64, c_i64, c_int, c_vp, c_vp, c_int,
+                                          P_int, P_int, c_int, c_vp, c_i64, c_vp, c_sz, c_vp]),
+    'okmi_masked_mo_product': (c_int, [c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_i64, c_i64, c_vp, c_int, c_vp, c_vp]),
+    'okmi_mo_pair_products': (c_int, [c_vp, c_int, c_vp, c_int, c_int, c_i64, c_i64, c_vp, c_i64, c_vp]),
+    'okmi_transition_flux': (c_int, [c_vp, c_int, c_int, c_i64, c_i64, c_vp, c_int, c_vp, c_i64, c_vp]),
+    'okmi_cube_format': (c_sz, [ctypes.POINTER(c_dbl), c_int, c_i64, c_i64, c_i64, ctypes.c_char_p, c_sz]),
     'okmi_batch_create': (c_int, [c_int, ctypes.POINTER(c_vp), ctypes.POINTER(OkmiGrid), ctypes.POINTER(c_vp),
                                   ctypes.POINTER(c_vp), P_int, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), c_int, c_int,
                                   ctypes.POINTER(c_vp)]),

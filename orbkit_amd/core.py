@@ -470,3 +470,120 @@ def rho_compute_no_slice(qc, calc_ao=False, calc_mo=False, drv=None, laplacian=F
 def gemm_sets(C, sets):
   '''C @ sets[j] for each derivative set j.'''
   return numpy.array([gemm(C, s) for s in sets])
+
+def gross_atomic(basis, dgrid, coeffs, occ, code, rows_per_sel, p0=0, np_=None):
+  '''Gross atomic densities (len(rows_per_sel), np) for AO derivative code
+  `code`; rows_per_sel[j] are the AO rows of selection j (okmi_gross_atomic_density).'''
+  torch = _torch()
+  L = _native.lib()
+  np_ = dgrid.npts - p0 if np_ is None else np_
+  C = numpy.ascontiguousarray(coeffs, dtype=numpy.float64)
+  dev = dgrid.x.device
+  dC = torch.as_tensor(C, device=dev)
+  docc = torch.as_tensor(numpy.ascontiguousarray(occ, dtype=numpy.float64), device=dev)
+  start = numpy.zeros(len(rows_per_sel) + 1, dtype=numpy.intc)
+  start[1:] = numpy.cumsum([len(r) for r in rows_per_sel])
+  flat = numpy.ascontiguousarray(numpy.concatenate([numpy.asarray(r, dtype=numpy.intc) for r in rows_per_sel])
+                                 if len(rows_per_sel) else numpy.zeros(0, numpy.intc), dtype=numpy.intc)
+  nsel = len(rows_per_sel)
+  out = torch.empty((max(nsel, 1), max(np_, 1)), dtype=torch.float64, device=dev)
+  chunk = max(64, min(np_, WORKSPACE_BYTES // max(1, 16 * basis.nao)))
+  ws = torch.empty(L.okmi_gross_atomic_workspace_bytes(basis.handle, chunk, int(start[-1]), nsel) // 8 + 1,
+                   dtype=torch.float64, device=dev)
+  _native.check(L.okmi_gross_atomic_density(
+      basis.handle, ctypes.byref(dgrid.desc), p0, np_, int(code), ctypes.c_void_p(dC.data_ptr()),
+      ctypes.c_void_p(docc.data_ptr()), C.shape[0], start.ctypes.data_as(_native.P_int),
+      flat.ctypes.data_as(_native.P_int) if len(flat) else None, nsel, ctypes.c_void_p(out.data_ptr()),
+      out.shape[1], ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream()), 'okmi_gross_atomic_density')
+  return out[:nsel, :np_].cpu().numpy()
+
+
+def masked_mo_product(coeffs, occ, ao, mo, rows):
+  '''sum_i occ_i mo_i sum_{r in rows} C[i][r] ao_r for host AO (nao, n) and
+  MO (nmo, n) arrays (okmi_masked_mo_product).'''
+  torch = _torch()
+  L = _native.lib()
+  dev = torch.device('cuda', torch.cuda.current_device())
+  t = lambda a: torch.as_tensor(numpy.ascontiguousarray(a, dtype=numpy.float64), device=dev)
+  C, docc, A, M = t(coeffs), t(occ), t(ao), t(mo)
+  n = A.shape[1]
+  r = torch.as_tensor(numpy.ascontiguousarray(rows, dtype=numpy.intc), device=dev)
+  out = torch.empty(max(n, 1), dtype=torch.float64, device=dev)
+  _native.check(L.okmi_masked_mo_product(ctypes.c_void_p(C.data_ptr()), ctypes.c_void_p(docc.data_ptr()),
+                                         C.shape[0], C.shape[1], ctypes.c_void_p(A.data_ptr()),
+                                         ctypes.c_void_p(M.data_ptr()), n, n,
+                                         ctypes.c_void_p(r.data_ptr()) if r.numel() else None, r.numel(),
+                                         ctypes.c_void_p(out.data_ptr()), _stream()), 'okmi_masked_mo_product')
+  return out[:n].cpu().numpy()
+
+
+def _mo_values(qc, codes):
+  '''Device MO values (len(codes), nmo, npts) on the grid module's grid.'''
+  torch = _torch()
+  L = _native.lib()
+  if not grid.is_initialized:
+    grid.grid_init()
+  basis = device_basis(require(qc.geo_spec, dtype='f'), qc.ao_spec)
+  dg = DeviceGrid(grid.x, grid.y, grid.z, regular=not grid.is_vector)
+  codes = numpy.ascontiguousarray(codes, dtype=numpy.intc)
+  C = numpy.ascontiguousarray(qc.mo_spec.get_coeffs(), dtype=numpy.float64)
+  dC = torch.as_tensor(C, device=dg.x.device)
+  mo = torch.empty((len(codes), C.shape[0], max(dg.npts, 1)), dtype=torch.float64, device=dg.x.device)
+  chunk = max(64, min(dg.npts, WORKSPACE_BYTES // max(1, len(codes) * basis.nao * 8)))
+  ws = torch.empty(L.okmi_workspace_bytes(basis.handle, len(codes), chunk) // 8 + 1, dtype=torch.float64,
+                   device=dg.x.device)
+  _native.check(L.okmi_mo_eval(basis.handle, ctypes.byref(dg.desc), 0, dg.npts, codes.ctypes.data_as(_native.P_int),
+                               len(codes), ctypes.c_void_p(dC.data_ptr()), C.shape[0], ctypes.c_void_p(mo.data_ptr()),
+                               mo.shape[2], ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream()), 'okmi_mo_eval')
+  N = (len(grid.x),) if grid.is_vector else (len(grid.x), len(grid.y), len(grid.z))
+  return mo, dg.npts, N
+
+
+def calc_mo_matrix(qc_a, qc_b=None, drv=None, numproc=1, slice_length=1e4, save_hdf5=False, **kwargs):
+  '''Products of the bra MOs of qc_a with the ket MOs (or their derivatives)
+  of qc_b (default qc_a) on the grid (orbkit/core.py:841-941):
+  mo_matrix[d, n, m] = mo_a[n] * d_d mo_b[m], shape (NDRV, NMO_a, NMO_b) + N
+  (NDRV = 1 without drv).'''
+  torch = _torch()
+  L = _native.lib()
+  if save_hdf5:
+    import h5py  # noqa: F401  (as orbkit: the HDF5 sink needs h5py)
+  if isinstance(qc_a, dict):
+    qc_a = QCinfo(qc_a)
+  if drv is None:
+    kets = [None]
+  elif not isinstance(drv, list):
+    kets = [drv]
+  else:
+    kets = list(drv)
+  kcodes = [0 if d is None else _codes([d])[0] for d in kets]
+  if qc_b is None or qc_b is qc_a:
+    mo, npts, N = _mo_values(qc_a, [0] + kcodes)
+    bra, ket = mo[0], mo[1:]
+  else:
+    bra, npts, N = _mo_values(qc_a, [0])
+    bra = bra[0]
+    ket, _, _ = _mo_values(qc_b if not isinstance(qc_b, dict) else QCinfo(qc_b), kcodes)
+  na, nb, nd = bra.shape[0], ket.shape[1], ket.shape[0]
+  out = torch.empty((nd, na, nb, max(npts, 1)), dtype=torch.float64, device=bra.device)
+  _native.check(L.okmi_mo_pair_products(ctypes.c_void_p(bra.data_ptr()), na, ctypes.c_void_p(ket.data_ptr()), nb, nd,
+                                        npts, bra.shape[-1], ctypes.c_void_p(out.data_ptr()), out.shape[-1],
+                                        _stream()), 'okmi_mo_pair_products')
+  return out[..., :npts].cpu().numpy().reshape((nd, na, nb) + N)
+
+
+def transition_flux(qc, indices, drv):
+  '''-1/2 (phi_i d phi_j - phi_j d phi_i) for MO index pairs (NDRV, NPAIR) + N.'''
+  torch = _torch()
+  L = _native.lib()
+  drv = [drv] if not isinstance(drv, (list, tuple)) else list(drv)
+  codes = [0] + [_codes([d])[0] for d in drv]
+  mo, npts, N = _mo_values(qc, codes)
+  pairs = torch.as_tensor(numpy.ascontiguousarray(indices, dtype=numpy.intc).reshape(-1), device=mo.device)
+  npair = pairs.numel() // 2
+  out = torch.empty((len(drv), npair, max(npts, 1)), dtype=torch.float64, device=mo.device)
+  _native.check(L.okmi_transition_flux(ctypes.c_void_p(mo.data_ptr()), mo.shape[1], len(drv), npts, mo.shape[-1],
+                                       ctypes.c_void_p(pairs.data_ptr()) if npair else None, npair,
+                                       ctypes.c_void_p(out.data_ptr()), out.shape[-1], _stream()),
+                'okmi_transition_flux')
+  return out[..., :npts].cpu().numpy().reshape((len(drv), npair) + N)

@@ -623,3 +623,65 @@ extern "C" int okmi_electron_count(const double* d_rho, int64_t n, double d3r, v
     }
     return okmi_launch_electrons(d_rho, n, d3r, d_ws, ws_bytes, d_out, (hipStream_t)stream);
 }
+
+extern "C" size_t okmi_gross_atomic_workspace_bytes(const okmi_basis* b, int64_t chunk, int nrows_total, int nsel)
+{
+    if (!b || chunk <= 0 || nrows_total < 0 || nsel < 0) return 0;
+    return okmi_gross_ws_bytes(b, chunk, nrows_total, nsel);
+}
+
+extern "C" int okmi_gross_atomic_density(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np, int drv,
+                                         const double* d_c, const double* d_occ, int nmo, const int* sel_start,
+                                         const int* sel_rows, int nsel, double* d_out, int64_t ld_out, void* d_ws,
+                                         size_t ws_bytes, void* stream)
+{
+    g_err.clear();
+    if (!b || !d_c || !d_occ || nmo <= 0 || nsel < 0 || (nsel > 0 && (!sel_start || !d_out || ld_out < np))) {
+        okmi_set_error("gross_atomic_density: invalid arguments");
+        return 1;
+    }
+    if (check_grid(g, p0, np) || check_drv(&drv, 1)) return 1;
+    if (nsel > 0 && sel_start[nsel] > 0 && !sel_rows) {
+        okmi_set_error("gross_atomic_density: missing AO row list");
+        return 1;
+    }
+    return okmi_launch_gross_atomic(b, to_desc(g), p0, np, drv, d_c, d_occ, nmo, sel_start, sel_rows, nsel, d_out,
+                                    ld_out, d_ws, ws_bytes, (hipStream_t)stream);
+}
+
+extern "C" int okmi_masked_mo_product(const double* d_c, const double* d_occ, int nmo, int nao, const double* d_ao,
+                                      const double* d_mo, int64_t np, int64_t ld, const int* d_rows, int nrows,
+                                      double* d_out, void* stream)
+{
+    g_err.clear();
+    if (!d_c || !d_occ || !d_ao || !d_mo || !d_out || nmo <= 0 || nao <= 0 || ld < np || nrows < 0 ||
+        (nrows > 0 && !d_rows)) {
+        okmi_set_error("masked_mo_product: invalid arguments");
+        return 1;
+    }
+    return okmi_launch_masked_mo_product(d_c, d_occ, nmo, nao, d_ao, d_mo, np, ld, d_rows, nrows, d_out,
+                                         (hipStream_t)stream);
+}
+
+extern "C" int okmi_mo_pair_products(const double* d_bra, int na, const double* d_ket, int nb, int nd, int64_t np,
+                                     int64_t ld_in, double* d_out, int64_t ld_out, void* stream)
+{
+    g_err.clear();
+    if (!d_bra || !d_ket || !d_out || na < 0 || nb < 0 || nd < 0 || ld_in < np || ld_out < np) {
+        okmi_set_error("mo_pair_products: invalid arguments");
+        return 1;
+    }
+    return okmi_launch_mo_pairs(d_bra, na, d_ket, nb, nd, np, ld_in, d_out, ld_out, (hipStream_t)stream);
+}
+
+extern "C" int okmi_transition_flux(const double* d_mo, int nmo, int nd, int64_t np, int64_t ld_in,
+                                    const int* d_pairs, int npair, double* d_out, int64_t ld_out, void* stream)
+{
+    g_err.clear();
+    if (!d_mo || !d_out || nmo <= 0 || nd < 0 || npair < 0 || (npair > 0 && !d_pairs) || ld_in < np ||
+        ld_out < np) {
+        okmi_set_error("transition_flux: invalid arguments");
+        return 1;
+    }
+    return okmi_launch_jmo(d_mo, nmo, nd, np, ld_in, d_pairs, npair, d_out, ld_out, (hipStream_t)stream);
+}

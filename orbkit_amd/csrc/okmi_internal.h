@@ -81,6 +81,19 @@ int okmi_launch_rho_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, 
                              const double* d_occ, int nmo, const double* d_grad, double* d_dc, double* d_dalpha,
                              double* d_dr, void* ws, size_t ws_bytes, hipStream_t st);
 
+// grid analysis helpers (extras.hip)
+size_t okmi_gross_ws_bytes(const OkmiBasis* b, int64_t chunk, int nrows_total, int nsel);
+int okmi_launch_gross_atomic(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, int drv, const double* d_c,
+                             const double* d_occ, int nmo, const int* sel_start, const int* sel_rows, int nsel,
+                             double* d_out, int64_t ld_out, void* ws, size_t ws_bytes, hipStream_t st);
+int okmi_launch_masked_mo_product(const double* d_c, const double* d_occ, int nmo, int nao, const double* d_ao,
+                                  const double* d_mo, int64_t np, int64_t ld, const int* d_rows, int nrows,
+                                  double* d_out, hipStream_t st);
+int okmi_launch_mo_pairs(const double* d_bra, int na, const double* d_ket, int nb, int nd, int64_t np, int64_t ld_in,
+                         double* d_out, int64_t ld_out, hipStream_t st);
+int okmi_launch_jmo(const double* d_mo, int nmo, int nd, int64_t np, int64_t ld_in, const int* d_pairs, int npair,
+                    double* d_out, int64_t ld_out, hipStream_t st);
+
 // compensated FP64 electron count (electrons.hip)
 size_t okmi_electrons_ws_bytes();
 int okmi_launch_electrons(const double* d_rho, int64_t n, double d3r, void* ws, size_t ws_bytes, double* d_out,

@@ -1,0 +1,223 @@
+'''Grid analysis helpers (orbkit/extras.py, core.calc_mo_matrix) and the cube
+writer (orbkit/output/cube.py).
+
+CPU: the extras oracle against invariants tied to the pinned density oracle,
+and the cube text against Gaussian's cube files and the native formatter.
+GPU: gross atomic densities, MO pair products and transition flux densities
+of the device path against the oracle (1e-12), and the thin wrappers
+(calc_mo, mo_set, calc_ao, numerical Mulliken populations).
+'''
+import os
+
+import numpy
+import pytest
+
+from conftest import GOLDEN, rel_err
+
+
+def _near(qc, n, seed):
+  geo = numpy.asarray(qc.geo_spec)
+  rng = numpy.random.default_rng(seed)
+  P = geo[rng.integers(0, len(geo), n)] + rng.normal(0.0, 1.5, (n, 3))
+  return P[:, 0].copy(), P[:, 1].copy(), P[:, 2].copy()
+
+
+def _aos_mos(qc, x, y, z, drv=None):
+  from oracle import core_oracle
+  from orbkit_amd.tools import get_cart2sph
+  ao = core_oracle.ao_creator(qc.geo_spec, qc.ao_spec, drv, x, y, z, get_cart2sph)
+  return ao, qc.mo_spec.get_coeffs() @ ao
+
+
+# ------------------------------------------------------------------ CPU
+
+@pytest.mark.parametrize('name', ['small', 'medium'])
+def test_extras_oracle_invariants(name):
+  from oracle import core_oracle, extras_oracle as eo
+  from orbkit_amd import workloads
+  qc, _, _ = workloads.make(name)
+  x, y, z = _near(qc, 300, 2)
+  ao, mo = _aos_mos(qc, x, y, z)
+  rho = core_oracle.rho_compute(qc, x, y, z, True)
+  ga = eo.gross_atomic_density(qc, range(len(qc.geo_spec)), ao, mo)
+  assert rel_err(numpy.sum(ga, axis=0), rho) < 1e-13
+  # mo_matrix and jmo from the pinned derivative MOs
+  dmo = numpy.array([qc.mo_spec.get_coeffs() @ _aos_mos(qc, x, y, z, d)[0] for d in ('x', 'y', 'z')])
+  mm = eo.mo_matrix(mo[numpy.newaxis], dmo)
+  mm0 = eo.mo_matrix(mo[numpy.newaxis], mo[numpy.newaxis])
+  occ = qc.mo_spec.get_occ()
+  assert rel_err(numpy.einsum('n,nnp->p', occ, mm0[0]), rho) < 1e-13
+  # sum_n occ_n (phi_n d phi_n) = d rho / 2
+  _, drho = core_oracle.rho_compute(qc, x, y, z, True, drv=['x', 'y', 'z'])
+  assert rel_err(numpy.einsum('n,dnnp->dp', occ, mm), 0.5 * drho) < 1e-13
+  pairs = [(0, 1), (1, 0), (2, 2), (0, 3)]
+  j = eo.jmo(mm, pairs)
+  assert numpy.array_equal(j[:, 0], -j[:, 1]) and not numpy.any(j[:, 2])
+
+
+def test_gross_single_atom():
+  '''For a one-atom molecule the gross atomic density is the density.'''
+  from oracle import core_oracle, extras_oracle as eo
+  from orbkit_amd import workloads
+  qc, _, _ = workloads.make('small')
+  keep = [i for i, c in enumerate(qc.ao_spec.data) if c['atom'] == 0]
+  from orbkit_amd.orbitals import AOClass, MOClass
+  ao = AOClass([qc.ao_spec.data[i] for i in keep])
+  ao.update()
+  nao = ao.get_ao_num()
+  rng = numpy.random.default_rng(1)
+  mo = MOClass([{'coeffs': rng.normal(size=nao), 'occ_num': 2.0, 'energy': -1.0, 'sym': '%d.1' % (i + 1)}
+                for i in range(3)])
+  from orbkit_amd.qcinfo import QCinfo
+  q = QCinfo()
+  q.geo_spec, q.ao_spec, q.mo_spec = numpy.asarray(qc.geo_spec)[:1], ao, mo
+  x, y, z = _near(q, 200, 3)
+  a, m = _aos_mos(q, x, y, z)
+  rho = core_oracle.rho_compute(q, x, y, z, True)
+  assert rel_err(eo.gross_atomic_density(q, [0], a, m)[0], rho) < 1e-13
+
+
+def _read_cube(path):
+  lines = open(path).read().split('\n')
+  count = int(lines[2].split()[0])
+  N = [int(lines[3 + i].split()[0]) for i in range(3)]
+  start = 6 + abs(count) + (1 if count < 0 else 0)   # a negative count announces a label line
+  vals = numpy.array([float(v) for ln in lines[start:] for v in ln.split()])
+  return lines, start, N, vals
+
+
+@pytest.mark.parametrize('fname', ['h2o_rhf_sph.cube', 'h2o_rhf_sph_mo.cube', 'h2o_rhf_sph_laplace.cube'])
+def test_cube_body_matches_gaussian(fname):
+  '''Gaussian cubegen writes values the way orbkit does: re-formatting the
+  values of the golden files reproduces their value text byte for byte, with
+  the oracle restatement and with the native formatter.'''
+  from oracle import extras_oracle as eo
+  from orbkit_amd.output import cube
+  path = os.path.join(GOLDEN, fname)
+  lines, start, N, vals = _read_cube(path)
+  ncol = len(vals) // int(numpy.prod(N))
+  if ncol != 1:
+    pytest.skip('multi-value cube')
+  body_ref = '\n'.join(lines[start:])
+  data = vals.reshape(N)[numpy.newaxis]
+  assert cube.body(data) == body_ref
+  full = eo.cube_text(data, [['X', '1', '1']], [[0.0, 0.0, 0.0]], [0, 0, 0], N, [1, 1, 1])
+  assert full.split('\n', 7)[-1] == body_ref
+
+
+def test_cube_creator_vs_oracle(tmp_path):
+  '''Whole file (header, labels, several data sets, 3-digit exponents) equal
+  to the restatement of orbkit's writer; gzip variant reads back the same.'''
+  import gzip
+  from oracle import extras_oracle as eo
+  from orbkit_amd import grid
+  from orbkit_amd.output import cube
+  grid.set_grid(numpy.linspace(-1.5, 1.5, 4), numpy.linspace(-1.0, 2.0, 5), numpy.linspace(0.0, 1.3, 7), is_vector=False)
+  rng = numpy.random.default_rng(0)
+  data = rng.normal(size=(3, 4, 5, 7)) * 10.0 ** rng.integers(-150, 150, (3, 4, 5, 7))
+  geo_info = [['O', '1', '8'], ['H', '2', '1'], ['H', '3', '1']]
+  geo_spec = [[0.0, 0.1, 0.2], [1.0, -0.5, 0.0], [-1.0, -0.5, 0.25]]
+  for labels in (None, [1, 5, 9]):
+    fn = cube.cube_creator(data, str(tmp_path / 'a'), geo_info, geo_spec, comments='test', labels=labels)
+    ref = eo.cube_text(data, geo_info, geo_spec, grid.min_, grid.N_, grid.delta_, 'test', labels)
+    assert open(fn).read() == ref
+  fn = cube.cube_creator(data[0], str(tmp_path / 'b.cube.gz'), geo_info, geo_spec)
+  with gzip.open(fn, 'rb') as f:
+    assert f.read().decode() == eo.cube_text(data[0], geo_info, geo_spec, grid.min_, grid.N_, grid.delta_)
+
+
+def test_atom2index():
+  from orbkit_amd.extras import atom2index
+  gi = [['O', '1', '8'], ['H', '2', '1'], ['H', '3', '1']]
+  assert atom2index(2, gi)[1].tolist() == [1]
+  assert atom2index([3, 1], gi)[1].tolist() == [2, 0]
+  with pytest.raises(ValueError):
+    atom2index(4, gi)
+
+
+# ------------------------------------------------------------------ GPU
+
+@pytest.fixture(scope='module')
+def gpu():
+  import torch
+  if not torch.cuda.is_available():
+    pytest.skip('no HIP device')
+  return torch
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name', ['small', 'medium'])
+def test_gross_atomic_density_gpu(gpu, name):
+  from oracle import extras_oracle as eo
+  from orbkit_amd import extras, grid, workloads
+  qc, _, _ = workloads.make(name)
+  x, y, z = _near(qc, 2001, 4)
+  print('case gross', name, flush=True)
+  grid.set_grid(x, y, z, is_vector=True)
+  ao, mo = _aos_mos(qc, x, y, z)
+  sel = [1, 3] if name == 'small' else list(range(1, 21, 4))
+  idx = [s - 1 for s in sel]
+  ref = eo.gross_atomic_density(qc, idx, ao, mo)
+  got = extras.gross_atomic_density(sel, qc)
+  for g, r in zip(got, ref):
+    assert rel_err(g, r) < 1e-12
+  got2 = extras.gross_atomic_density(sel, qc, ao_list=ao, mo_list=mo)
+  for g, r in zip(got2, ref):
+    assert rel_err(g, r) < 1e-12
+  rho_all = extras.gross_atomic_density('all', qc)
+  from oracle import core_oracle
+  assert rel_err(numpy.sum(rho_all, axis=0), core_oracle.rho_compute(qc, x, y, z, True)) < 1e-12
+  # derivative AOs (orbkit's drv argument) and the per-atom MOs
+  aod, mod = _aos_mos(qc, x, y, z, 'y')
+  rd = eo.gross_atomic_density(qc, idx, aod, mod)
+  gd, mo_atom = extras.gross_atomic_density(sel, qc, bReturnmo=True, drv='y')
+  for g, r in zip(gd, rd):
+    assert rel_err(g, r) < 1e-12
+  ra = eo.row_atoms(qc)
+  C = qc.mo_spec.get_coeffs()
+  for a, ma in zip(idx, mo_atom):
+    Cm = numpy.where(ra[numpy.newaxis, :] == a, C, 0.0)
+    assert rel_err(ma, Cm @ aod) < 1e-12
+
+
+@pytest.mark.gpu
+def test_mo_matrix_and_jmo_gpu(gpu):
+  from oracle import extras_oracle as eo
+  from orbkit_amd import core, extras, grid, workloads
+  qc, _, _ = workloads.make('medium', nmo=6)
+  x, y, z = _near(qc, 1501, 5)
+  print('case mo_matrix', flush=True)
+  grid.set_grid(x, y, z, is_vector=True)
+  _, mo = _aos_mos(qc, x, y, z)
+  dmo = numpy.array([_aos_mos(qc, x, y, z, d)[1] for d in ('x', 'z')])
+  mm = core.calc_mo_matrix(qc, drv=['x', 'z'])
+  ref = eo.mo_matrix(mo[numpy.newaxis], dmo)
+  assert mm.shape == ref.shape and rel_err(mm, ref) < 1e-12
+  mm0 = core.calc_mo_matrix(qc)
+  assert rel_err(mm0, eo.mo_matrix(mo[numpy.newaxis], mo[numpy.newaxis])) < 1e-12
+  ij = [[0, 1], [3, 2], [4, 5], [1, 0]]
+  j = extras.calc_jmo(qc, ij, drv=['x', 'z'])
+  u, ind = numpy.unique(numpy.asarray(ij), return_inverse=True)
+  sub = eo.mo_matrix(mo[u][numpy.newaxis], dmo[:, u])
+  assert rel_err(j, eo.jmo(sub, ind.reshape(-1, 2))) < 1e-12
+
+
+@pytest.mark.gpu
+def test_wrappers_gpu(gpu, tmp_path):
+  from orbkit_amd import core, extras, grid, workloads
+  qc, axes, _ = workloads.make('small', npts_side=16)
+  print('case wrappers', flush=True)
+  grid.set_grid(*axes, is_vector=False)
+  mos = core.rho_compute(qc, calc_mo=True)
+  sel = extras.calc_mo(qc, [0, 2, 4])
+  assert numpy.array_equal(sel, mos[[0, 2, 4]])
+  aos = extras.calc_ao(qc, drv='x')
+  assert numpy.array_equal(aos, core.rho_compute(qc, calc_ao=True, drv='x'))
+  sets, dsets = extras.mo_set(qc, [[0, 1], [2, 3, 4]], drv=['x'])
+  r01, d01 = core.rho_compute(extras._copy_with_mos(qc, qc.mo_spec[[0, 1]]), drv=['x'])
+  assert numpy.array_equal(sets[0], r01) and numpy.array_equal(dsets[0], d01[0])
+  rho_atom, mull = extras.numerical_mulliken_charges('all', qc)
+  total = core.electron_count(core.rho_compute(qc), grid.d3r)
+  assert abs(mull['population'].sum() - total) <= 1e-10 * total
+  fn = extras.calc_mo(qc, [0], otype='cube', ofid=str(tmp_path / 'mo0'))
+  assert os.path.exists(str(tmp_path / 'mo0.cube'))
