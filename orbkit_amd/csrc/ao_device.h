@@ -133,10 +133,11 @@ __device__ __forceinline__ float gauss_f32(double t)
 // AO values end in FP32 or narrower.  Cody-Waite reduction t = k ln2 + r,
 // |r| <= ln2/2 (ln2 split in a high part exact in 32 bits and a low part),
 // then the degree-9 Taylor polynomial of exp(-r) (truncation error
-// r^10/10! < 7e-12) and 2^-k by exponent adjustment (underflows to +0).
+// r^10/10! < 7e-12) and 2^-k by exponent adjustment, which underflows to +0
+// for large t.  Branch-free: t is clamped to 1e4 (exp(-1e4) is +0 in FP64).
 __device__ __forceinline__ double gauss_f64_fast(double t)
 {
-    if (t > 1.0e4) return 0.0;
+    t = fmin(t, 1.0e4);
     const double k = __builtin_rint(t * 1.4426950408889634);
     double r = fma(-k, 6.93147180369123816490e-01, t);
     r = fma(-k, 1.90821492927058770002e-10, r);

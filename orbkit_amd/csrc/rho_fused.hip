@@ -170,13 +170,27 @@ __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const Shell
     const double X = px - sh.cx, Y = py - sh.cy, Z = pz - sh.cz;
     const double rr = X * X + Y * Y + Z * Z;
     double S0 = 0.0, S1 = 0.0, S2 = 0.0;
-    for (int p = 0; p < sh.nprim; ++p) {
-        const PrimRec pr = load_prim(a.prims, sh.prim0 + p);
-        if (d2 >= pr.rcut2) continue;   // below 1e-40 on the whole tile
-        const double de = pr.d * gauss_f64_fast(pr.alpha * rr);
-        S0 += de;
-        if constexpr (need1) S1 = fma(-2.0 * pr.alpha, de, S1);
-        if constexpr (need2) S2 = fma(4.0 * pr.alpha * pr.alpha, de, S2);
+    // primitives four at a time: independent exponential chains (the loop
+    // count is a runtime value); a primitive whose radius misses the tile, or
+    // past the shell's last one, gets t = 1e4 and contributes exactly 0
+    for (int p0 = 0; p0 < sh.nprim; p0 += 4) {
+        double de[4], al[4];
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const PrimRec pr = load_prim(a.prims, sh.prim0 + min(p0 + j, sh.nprim - 1));
+            const bool live = p0 + j < sh.nprim && d2 < pr.rcut2;   // wave-uniform
+            any |= live;
+            al[j] = pr.alpha;
+            de[j] = pr.d * gauss_f64_fast(live ? pr.alpha * rr : 1.0e4);
+        }
+        if (!any) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            S0 += de[j];
+            if constexpr (need1) S1 = fma(-2.0 * al[j], de[j], S1);
+            if constexpr (need2) S2 = fma(4.0 * al[j] * al[j], de[j], S2);
+        }
     }
     ShellPows<L, double> P;
     P.init(X, Y, Z);
