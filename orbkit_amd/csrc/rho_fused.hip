@@ -135,7 +135,6 @@ struct Box {
         }
         return d2;
     }
-    __device__ __forceinline__ bool reaches(const ShellRec sh) const { return dist2(sh) < sh.rcut2; }
 };
 
 template <int L, int R>
@@ -233,10 +232,11 @@ struct FusedLds {
 };
 
 // One tile of 64 points: every thread of the workgroup calls this with the
-// same tile.  `lds` holds two AO buffers, `next` two work counters.  The
-// first barrier orders it after any previous tile's LDS reads.
+// same tile.  `lds` holds two AO buffers, `next` two work counters, `live`
+// two chunk tags.  The first barrier orders it after any previous tile's LDS
+// reads.
 template <int PREC, int PASS, int MT>
-__device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float* lds, int* next)
+__device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float* lds, int* next, int* live)
 {
     constexpr int NS = PassCodes<PASS>::n;
     constexpr int BUF = FusedLds<PASS>::BUF;
@@ -264,9 +264,6 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float
         const int k0 = ch * KC;
         const int2 cw = load_int2(a.chunks, ch);
         const int nsh = cw.y & 0xffff, used = cw.y >> 16;
-        bool any = false;
-        for (int e = cw.x; e < cw.x + nsh; ++e) any |= box.reaches(load_shell(a.shells, cptr(a.work)[e] & 0xffffff));
-        if (!any) continue;   // every AO row of this chunk is zero on the tile (same decision in all waves)
         float* L = lds + buf * BUF;
         // operand A (MO coefficients) for this chunk
         Split<PREC> ca[MT];
@@ -305,6 +302,7 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float
             const ShellRec sh = load_shell(a.shells, cptr(a.work)[cw.x + e] & 0xffffff);
             const double d2 = box.dist2(sh);
             if (d2 < sh.rcut2) {
+                if (lane == 0) live[buf] = ch;
                 switch (sh.l) {
                 case 0: eval_shell_chunk<0, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
                 case 1: eval_shell_chunk<1, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
@@ -320,6 +318,15 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float
         // All rows of buffer `buf` written; the other buffer is free because
         // every wave has passed the previous barrier after reading it.
         __syncthreads();
+        // No shell of the chunk reaches the tile: every row is zero, skip
+        // the contraction (same decision in all waves).  live[buf] is tagged
+        // with the chunk index instead of being reset: it is written before
+        // this barrier and next written (chunk ch + 2) after the following
+        // one, and a stale value can only let a zero chunk through.
+        if (live[buf] != ch) {
+            buf ^= 1;
+            continue;
+        }
         // contraction of this wave's 16-point column
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
@@ -411,7 +418,8 @@ __global__ __launch_bounds__(256, 2) void k_rho_fused(FusedArgs a)
 {
     __shared__ float lds[2 * FusedLds<PASS>::BUF];
     __shared__ int next[2];   // per buffer: next work entry to evaluate
-    rho_tile<PREC, PASS, MT>(a, blockIdx.x, lds, next);
+    __shared__ int live[2];   // per buffer: index of the last chunk with a live shell
+    rho_tile<PREC, PASS, MT>(a, blockIdx.x, lds, next, live);
 }
 
 struct GatherCodes {
@@ -716,10 +724,11 @@ __global__ __launch_bounds__(256, 2) void k_rho_batch(BatchArgs b)
 {
     __shared__ float lds[2 * FusedLds<PASS>::BUF];
     __shared__ int next[2];
+    __shared__ int live[2];
     for (int64_t t = blockIdx.x; t < b.ntile; t += gridDim.x) {
         const int m = find_item(b.tile_start, b.nmol, t);
         const FusedArgs a = load_fused_args(b.mols, m);
-        rho_tile<PREC, PASS, MT>(a, t - cptr(b.tile_start)[m], lds, next);
+        rho_tile<PREC, PASS, MT>(a, t - cptr(b.tile_start)[m], lds, next, live);
     }
 }
 
