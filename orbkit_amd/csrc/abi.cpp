@@ -346,6 +346,7 @@ extern "C" int okmi_basis_destroy(okmi_basis* b)
     if (b->d_work) (void)hipFree(b->d_work);
     if (b->d_fshells) (void)hipFree(b->d_fshells);
     if (b->d_fprims) (void)hipFree(b->d_fprims);
+    if (b->d_oshells) (void)hipFree(b->d_oshells);
     if (b->d_colmap) (void)hipFree(b->d_colmap);
     delete[] b->h_shells;
     delete[] b->h_prims;

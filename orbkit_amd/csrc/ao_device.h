@@ -184,6 +184,22 @@ __host__ __device__ constexpr SphRow sph_row(int l, int r)
     return l == 0 ? L0[r] : l == 1 ? L1[r] : l == 2 ? L2[r] : L3[r];
 }
 
+// Output rows of a standard spherical shell (sph_row table) or a standard
+// Cartesian shell (component norm factors) from the canonical Cartesian
+// values c, unrolled at compile time: store(r, value) for each row r.
+template <int L, int R, typename Store>
+__device__ __forceinline__ void sph_rows_to(const double* c, Store&& store)
+{
+    if constexpr (L <= 3 && R < 2 * L + 1) {
+        constexpr SphRow row = sph_row(L, R);
+        double v = row.t[0].v * c[row.t[0].c];
+        if constexpr (row.n > 1) v = fma(row.t[1].v, c[row.t[1].c], v);
+        if constexpr (row.n > 2) v = fma(row.t[2].v, c[row.t[2].c], v);
+        store(R, v);
+        sph_rows_to<L, R + 1>(c, store);
+    }
+}
+
 // 1/sqrt((2lx-1)!!(2ly-1)!!(2lz-1)!!) of canonical component c
 __host__ __device__ constexpr double cart_norm(int l, int c)
 {
@@ -194,4 +210,13 @@ __host__ __device__ constexpr double cart_norm(int l, int c)
     }
     return d == 1 ? 1.0 : d == 3 ? 0.57735026918962584 : d == 9 ? 0.33333333333333333
          : d == 15 ? 0.2581988897471611 : d == 105 ? 0.09759000729485333 : 0.0;
+}
+
+template <int L, int R, typename Store>
+__device__ __forceinline__ void cart_rows_to(const double* c, Store&& store)
+{
+    if constexpr (R < ncart(L)) {
+        store(R, c[R] * cart_norm(L, R));
+        cart_rows_to<L, R + 1>(c, store);
+    }
 }

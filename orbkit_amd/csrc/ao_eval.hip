@@ -22,6 +22,12 @@ struct AoArgs {
     int ndrv;
     int need1, need2;
     int drv[OKMI_MAX_SETS];
+    // brick mode (regular grids): wave w of the launch takes brick bk0 + w
+    // (4x4x4 points, z fastest) of the nbx x nby x nbz bricks and writes
+    // columns 64 w + lane; lanes of a brick outside the grid repeat its
+    // nearest grid point, so every column of the nbk bricks is written
+    int brick;
+    int64_t bk0, nbk, nbx, nby, nbz;
 };
 
 // squared distance from a shell centre to the wave's points' bounding box
@@ -70,11 +76,22 @@ __device__ __forceinline__ void shell_out(const AoArgs& a, const ShellRec& sh, d
         double c[NC];
         cart_drv<L, double>(P, a.drv[j], S0, S1, S2, c);
         OutT* o = out + j * a.set_stride + (int64_t)sh.row0 * a.ld + i;
-        for (int r = 0; r < sh.nrow; ++r) {
-            double v = 0.0;
-#pragma unroll
-            for (int k = 0; k < NC; ++k) v = fma(T[r * NC + k], c[k], v);
+        auto store = [&](int r, double v) {
             if (act) o[(int64_t)r * a.ld] = (OutT)v;
+        };
+        // standard spherical / Cartesian shells: sparse compile-time rows;
+        // other transforms (ORCA row factors, custom maps): the dense block
+        if (sh.kind == SHELL_T_SPHERICAL) {
+            sph_rows_to<L, 0>(c, store);
+        } else if (sh.kind == SHELL_T_CARTESIAN) {
+            cart_rows_to<L, 0>(c, store);
+        } else {
+            for (int r = 0; r < sh.nrow; ++r) {
+                double v = 0.0;
+#pragma unroll
+                for (int k = 0; k < NC; ++k) v = fma(T[r * NC + k], c[k], v);
+                store(r, v);
+            }
         }
     }
 }
@@ -82,10 +99,29 @@ __device__ __forceinline__ void shell_out(const AoArgs& a, const ShellRec& sh, d
 template <typename OutT>
 __global__ __launch_bounds__(256) void k_ao_eval(AoArgs a, OutT* __restrict__ out)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = i < a.np;
+    int64_t i;
+    bool act;
     double px, py, pz;
-    grid_point(a.grid, a.p0 + (act ? i : 0), px, py, pz);
+    if (a.brick) {
+        const GridDesc& g = a.grid;
+        const int lane = threadIdx.x & 63;
+        const int64_t wb = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+        int64_t t = a.bk0 + wb;
+        const int64_t bz = t % a.nbz;
+        t /= a.nbz;
+        const int64_t by = t % a.nby, bx = t / a.nby;
+        int64_t ix = 4 * bx + (lane >> 4), iy = 4 * by + ((lane >> 2) & 3), iz = 4 * bz + (lane & 3);
+        ix = ix < g.nx ? ix : g.nx - 1;
+        iy = iy < g.ny ? iy : g.ny - 1;
+        iz = iz < g.nz ? iz : g.nz - 1;
+        act = wb < a.nbk;
+        i = wb * 64 + lane;
+        px = g.x[ix]; py = g.y[iy]; pz = g.z[iz];
+    } else {
+        i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        act = i < a.np;
+        grid_point(a.grid, a.p0 + (act ? i : 0), px, py, pz);
+    }
     // bounding box of the wave's points (inactive lanes repeat point 0,
     // which only enlarges the box)
     double lo[3] = {px, py, pz}, hi[3] = {px, py, pz};
@@ -123,10 +159,18 @@ __global__ __launch_bounds__(256) void k_ao_eval(AoArgs a, OutT* __restrict__ ou
 int okmi_launch_ao_eval(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const int* drv, int ndrv,
                         int out_dtype, void* out, int64_t ld, int64_t set_stride, hipStream_t st)
 {
-    if (np <= 0) return 0;
+    return okmi_launch_ao_eval_ex(b, b->d_shells, b->d_prims, g, p0, np, nullptr, drv, ndrv, out_dtype, out, ld,
+                                  set_stride, st);
+}
+
+int okmi_launch_ao_eval_ex(const OkmiBasis* b, const ShellRec* shells, const PrimRec* prims, const GridDesc& g,
+                           int64_t p0, int64_t np, const AoBricks* bricks, const int* drv, int ndrv, int out_dtype,
+                           void* out, int64_t ld, int64_t set_stride, hipStream_t st)
+{
+    if (np <= 0 && !bricks) return 0;
     AoArgs a{};
-    a.shells = b->d_shells;
-    a.prims = b->d_prims;
+    a.shells = shells;
+    a.prims = prims;
     a.T = b->d_T;
     a.grid = g;
     a.p0 = p0;
@@ -142,6 +186,13 @@ int okmi_launch_ao_eval(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64
         if (drv[j] >= 4) a.need2 = 1;
     }
     dim3 blk(256), grd((unsigned)((np + 255) / 256));
+    if (bricks) {
+        if (bricks->nbk <= 0) return 0;
+        a.brick = 1;
+        a.bk0 = bricks->bk0; a.nbk = bricks->nbk;
+        a.nbx = bricks->nbx; a.nby = bricks->nby; a.nbz = bricks->nbz;
+        grd = dim3((unsigned)((bricks->nbk + 3) / 4));
+    }
     OkmiProfScope prof("ao_eval", st);
     if (out_dtype == OKMI_F64)
         hipLaunchKernelGGL(k_ao_eval<double>, grd, blk, 0, st, a, (double*)out);

@@ -32,6 +32,7 @@ struct OkmiBasis {
     int2* d_chunks = nullptr;   // per chunk: first work entry, shells | used rows << 16
     int* d_work = nullptr;
     ShellRec* d_fshells = nullptr;   // row0 = internal row, rcut2 = fused radius
+    ShellRec* d_oshells = nullptr;   // output rows, rcut2 = fused radius
     PrimRec* d_fprims = nullptr;     // rcut2 = fused radius
     int* d_colmap = nullptr;         // [nkpad]
     int nchunk = 0;
@@ -46,6 +47,20 @@ int okmi_hip_fail(hipError_t e, const char* what);
 
 int okmi_launch_ao_eval(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const int* drv, int ndrv,
                         int out_dtype, void* out, int64_t ld, int64_t set_stride, hipStream_t st);
+
+// the same with explicit shell/primitive tables (e.g. the fused screening
+// radii) and, for regular grids, brick order: bricks [bk0, bk0 + nbk) of
+// nbx x nby x nbz 4x4x4 bricks, column 64 (b - bk0) + lane
+struct AoBricks {
+    int64_t bk0, nbk, nbx, nby, nbz;
+};
+int okmi_launch_ao_eval_ex(const OkmiBasis* b, const ShellRec* shells, const PrimRec* prims, const GridDesc& g,
+                           int64_t p0, int64_t np, const AoBricks* bricks, const int* drv, int ndrv, int out_dtype,
+                           void* out, int64_t ld, int64_t set_stride, hipStream_t st);
+
+// fused-path tables (internal row order, fused screening radii) and the
+// output-row shell table with the fused radii (d_oshells); built once
+int okmi_prepare_fused(OkmiBasis* b);
 
 // MO values Phi_s = C A_s for NS sets stored [set][mo][point] (FP64 MFMA)
 int okmi_launch_gemm_f64(const double* A, int64_t ld, int64_t set_stride, int nsets, int nao, const double* C,
@@ -71,7 +86,7 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
 // many-MO density: FP32 AO sets per chunk + MFMA GEMM with density epilogue
 // (S AO sets: 1, 4, 7 or 10, the smallest holding the requested codes)
 size_t okmi_big_ws_bytes(const OkmiBasis* b, int nmo, int prec, int S, int64_t np);
-int okmi_launch_rho_big(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const double* d_c,
+int okmi_launch_rho_big(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const double* d_c,
                         const double* d_occ, int nmo, int prec, const int* drv_out, int nout, double* rho,
                         double* delta, int64_t ld_delta, void* ws, size_t ws_bytes, hipStream_t st);
 

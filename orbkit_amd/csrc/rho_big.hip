@@ -260,15 +260,33 @@ struct OutCodes {
     int c[16];
 };
 
-// rho[p] = sum_mt partial[mt][0][p]; delta[r][p] = sum_mt partial[mt][c_r][p]
-// (code 0: 2 rho, orbkit/core.py:299-300 with drv None)
+struct BrickMap {
+    int on;
+    int64_t bk0, nbx, nby, nbz, nx, ny, nz;
+};
+
+// rho[p] = sum_mt partial[mt][0][col]; delta[r][p] = sum_mt partial[mt][c_r][col]
+// (code 0: 2 rho, orbkit/core.py:299-300 with drv None).  Column col is point
+// p = col (range order) or, in brick order, the grid point of lane col % 64
+// of brick bk0 + col / 64 (columns outside the grid are dropped).
 __global__ void k_reduce_big(const double* partial, int nmt, int S, int64_t ld, int64_t np, OutCodes oc,
-                             double* rho, double* delta, int64_t ld_delta)
+                             double* rho, double* delta, int64_t ld_delta, BrickMap bm)
 {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= np) return;
+    int64_t p = col;
+    if (bm.on) {
+        const int lane = (int)(col & 63);
+        int64_t t = bm.bk0 + (col >> 6);
+        const int64_t bz = t % bm.nbz;
+        t /= bm.nbz;
+        const int64_t by = t % bm.nby, bx = t / bm.nby;
+        const int64_t ix = 4 * bx + (lane >> 4), iy = 4 * by + ((lane >> 2) & 3), iz = 4 * bz + (lane & 3);
+        if (ix >= bm.nx || iy >= bm.ny || iz >= bm.nz) return;
+        p = (ix * bm.ny + iy) * bm.nz + iz;
+    }
     double r = 0.0;
-    for (int m = 0; m < nmt; ++m) r += partial[(int64_t)m * S * ld + p];
+    for (int m = 0; m < nmt; ++m) r += partial[(int64_t)m * S * ld + col];
     rho[p] = r;
     for (int j = 0; j < oc.n; ++j) {
         const int c = oc.c[j];
@@ -276,7 +294,7 @@ __global__ void k_reduce_big(const double* partial, int nmt, int S, int64_t ld, 
         if (c == 0) {
             v = 2.0 * r;
         } else {
-            for (int m = 0; m < nmt; ++m) v += partial[((int64_t)m * S + c) * ld + p];
+            for (int m = 0; m < nmt; ++m) v += partial[((int64_t)m * S + c) * ld + col];
         }
         delta[j * ld_delta + p] = v;
     }
@@ -346,7 +364,7 @@ size_t okmi_big_ws_bytes(const OkmiBasis* b, int nmo, int prec, int S, int64_t n
     return L.fixed + L.per_point * (size_t)n;
 }
 
-int okmi_launch_rho_big(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const double* d_c,
+int okmi_launch_rho_big(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const double* d_c,
                         const double* d_occ, int nmo, int prec, const int* drv_out, int nout, double* rho,
                         double* delta, int64_t ld_delta, void* ws, size_t ws_bytes, hipStream_t st)
 {
@@ -366,13 +384,25 @@ int okmi_launch_rho_big(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64
         }
     const int S = big_sets(drv_out, nout);
     const BigLayout L = big_layout(b, nmo, prec, S);
+    int rc0 = okmi_prepare_fused(b);   // output-row shells and primitives with the fused radii
+    if (rc0) return rc0;
     if (!ws || ws_bytes < L.fixed + L.per_point * 128) {
         okmi_set_error("rho_big: workspace of %zu bytes is too small (need at least %zu)", ws_bytes,
                        L.fixed + L.per_point * 128);
         return 1;
     }
+    // a whole regular grid is processed in 4x4x4 bricks (compact point sets
+    // for screening, 64-column runs per brick); otherwise in range order
+    BrickMap bm{};
+    int64_t total = np;
+    if (g.regular && p0 == 0 && np == g.nx * g.ny * g.nz) {
+        bm.on = 1;
+        bm.nbx = (g.nx + 3) / 4; bm.nby = (g.ny + 3) / 4; bm.nbz = (g.nz + 3) / 4;
+        bm.nx = g.nx; bm.ny = g.ny; bm.nz = g.nz;
+        total = bm.nbx * bm.nby * bm.nbz * 64;
+    }
     int64_t chunk = (int64_t)((ws_bytes - L.fixed) / L.per_point) & ~(int64_t)127;
-    const int64_t npad = (np + 127) & ~(int64_t)127;
+    const int64_t npad = (total + 127) & ~(int64_t)127;
     if (chunk > npad) chunk = npad;
     char* w = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
     void* cpl = w;
@@ -391,9 +421,18 @@ int okmi_launch_rho_big(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64
     oc.n = nout;
     for (int r = 0; r < nout; ++r) oc.c[r] = drv_out[r];
     const int NP = 32 * (S == 1 ? 4 : S == 4 ? 2 : 1);
-    for (int64_t c0 = 0; c0 < np; c0 += chunk) {
-        const int64_t n = np - c0 < chunk ? np - c0 : chunk;
-        int rc = okmi_launch_ao_eval(b, g, p0 + c0, n, codes, S, OKMI_F32, ao, chunk, (int64_t)b->nao * chunk, st);
+    for (int64_t c0 = 0; c0 < total; c0 += chunk) {
+        const int64_t n = total - c0 < chunk ? total - c0 : chunk;
+        int rc;
+        if (bm.on) {
+            AoBricks bk{c0 / 64, n / 64, bm.nbx, bm.nby, bm.nbz};
+            bm.bk0 = c0 / 64;
+            rc = okmi_launch_ao_eval_ex(b, b->d_oshells, b->d_fprims, g, 0, n, &bk, codes, S, OKMI_F32, ao, chunk,
+                                        (int64_t)b->nao * chunk, st);
+        } else {
+            rc = okmi_launch_ao_eval_ex(b, b->d_oshells, b->d_fprims, g, p0 + c0, n, nullptr, codes, S, OKMI_F32, ao,
+                                        chunk, (int64_t)b->nao * chunk, st);
+        }
         if (rc) return rc;
         BigArgs a{};
         a.ao = ao; a.ld = chunk; a.set_stride = (int64_t)b->nao * chunk; a.nao = b->nao; a.np = n;
@@ -406,7 +445,7 @@ int okmi_launch_rho_big(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64
         }
         if (rc) return rc;
         hipLaunchKernelGGL(k_reduce_big, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part, nmt, S, chunk, n, oc,
-                           rho + c0, delta ? delta + c0 : nullptr, ld_delta);
+                           bm.on ? rho : rho + c0, bm.on ? delta : (delta ? delta + c0 : nullptr), ld_delta, bm);
     }
     return okmi_check_launch("rho_big");
 }

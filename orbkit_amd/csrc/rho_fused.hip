@@ -137,28 +137,6 @@ struct Box {
     }
 };
 
-template <int L, int R>
-__device__ __forceinline__ void sph_rows(const double* c, float* dst)
-{
-    if constexpr (L <= 3 && R < 2 * L + 1) {
-        constexpr SphRow row = sph_row(L, R);
-        double v = row.t[0].v * c[row.t[0].c];
-        if constexpr (row.n > 1) v = fma(row.t[1].v, c[row.t[1].c], v);
-        if constexpr (row.n > 2) v = fma(row.t[2].v, c[row.t[2].c], v);
-        dst[R * LDS_LD] = (float)v;
-        sph_rows<L, R + 1>(c, dst);
-    }
-}
-
-template <int L, int R>
-__device__ __forceinline__ void cart_rows(const double* c, float* dst)
-{
-    if constexpr (R < ncart(L)) {
-        dst[R * LDS_LD] = (float)(c[R] * cart_norm(L, R));
-        cart_rows<L, R + 1>(c, dst);
-    }
-}
-
 template <int L, int PASS>
 __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const ShellRec& sh, double d2, double px,
                                                  double py, double pz, int k0, float* lds, int lane)
@@ -210,10 +188,11 @@ __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const Shell
         default: cart_all<L, 9, double>(P, S0, S1, S2, c); break;
         }
         float* dst = lds + s * KC * LDS_LD + (sh.row0 - k0) * LDS_LD + lane;
+        auto store = [&](int r, double v) { dst[r * LDS_LD] = (float)v; };
         if (sh.kind == SHELL_T_SPHERICAL) {
-            sph_rows<L, 0>(c, dst);
+            sph_rows_to<L, 0>(c, store);
         } else if (sh.kind == SHELL_T_CARTESIAN) {
-            cart_rows<L, 0>(c, dst);
+            cart_rows_to<L, 0>(c, store);
         } else {
             const OKMI_CONST double* T = cptr(a.T) + sh.t0;
             for (int r = 0; r < sh.nrow; ++r) {
@@ -553,12 +532,15 @@ int prepare_fused_tables(OkmiBasis* b)
         fs[i].row0 = krow[i];
         fs[i].rcut2 = b->h_frcut2[i];
     }
+    std::vector<ShellRec> os(b->h_shells, b->h_shells + b->nshell);
+    for (int i = 0; i < b->nshell; ++i) os[i].rcut2 = b->h_frcut2[i];
     std::vector<PrimRec> fp(b->h_prims, b->h_prims + b->nprim);
     for (int i = 0; i < b->nprim; ++i) fp[i].rcut2 = b->h_fprcut2[i];
     int rc = upload(&b->d_chunks, ch, "rho_fused: chunk map");
     if (!rc) rc = upload(&b->d_work, work, "rho_fused: work list");
     if (!rc) rc = upload(&b->d_fshells, fs, "rho_fused: shells");
     if (!rc) rc = upload(&b->d_fprims, fp, "rho_fused: primitives");
+    if (!rc) rc = upload(&b->d_oshells, os, "rho_fused: output-row shells");
     if (!rc) rc = upload(&b->d_colmap, colmap, "rho_fused: row map");
     if (rc) return rc;
     b->nchunk = nchunk;
@@ -1025,3 +1007,5 @@ int okmi_batch_set_grid_impl(OkmiBatch* B, int blocks)
 }
 
 int64_t okmi_batch_ntile(const OkmiBatch* B) { return B->ntile; }
+
+int okmi_prepare_fused(OkmiBasis* b) { return prepare_fused_tables(b); }

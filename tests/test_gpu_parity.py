@@ -271,7 +271,7 @@ def test_many_mo_precision(ok, prec, tol):
   widths (rho; rho+grad; +Laplacian; +mixed second derivatives), regular grid
   and a vector grid whose length is no multiple of any tile.'''
   core, grid = ok['core'], ok['grid']
-  qc, axes, _ = ok['W'].make('medium', npts_side=20, nmo=130)
+  qc, axes, _ = ok['W'].make('medium', npts_side=19, nmo=130)   # 19^3: bricks padded at the faces
   print('case many-MO', prec, flush=True)
   grid.set_grid(*axes, is_vector=False)
   errs = {}
