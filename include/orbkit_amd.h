@@ -230,6 +230,21 @@ int okmi_transition_flux(const double* d_mo, int nmo, int nd, int64_t np, int64_
 size_t okmi_cube_format(const double* data, int nsets, int64_t nx, int64_t ny, int64_t nz, char* out, size_t cap);
 
 /*
+ * detCI grid reductions (orbkit/detci/cy_ci.pyx) over MO values
+ * d_mo [nmo][ld] (and d_dmo [nd][nmo][ld] for the fluxes), for terms
+ * (coef[k], ia[k], ib[k]) given in host arrays (indices checked against nmo):
+ *   mode -1: d_out[p]    = sum_k (coef_k mo_a) mo_b                   (get_rho)
+ *   mode  0: d_out[d][p] = -sum_k 1/2 coef_k (mo_a dmo_b - mo_b dmo_a) (get_jab)
+ *   mode  1: d_out[d][p] = sum_k coef_k mo_a dmo_b                      (get_a_nabla_b)
+ * Terms are summed in the given order (orbkit: identical determinants,
+ * then single excitations).
+ */
+size_t okmi_ci_workspace_bytes(int nterm);
+int okmi_ci_reduce(const double* d_mo, const double* d_dmo, int nmo, int nd, int64_t npts, int64_t ld,
+                   const double* coef, const int* ia, const int* ib, int nterm, int mode,
+                   double* d_out, int64_t ld_out, void* d_ws, size_t ws_bytes, void* stream);
+
+/*
  * Many molecules in one launch: density (and gradient) of every molecule on
  * its own grid with the fused reduced-precision kernel (okmi_rho_fused's
  * arithmetic; up to 64 MOs per molecule).  Inputs are bound at creation:

@@ -53,6 +53,9 @@ SIGNATURES = {
     'okmi_mo_pair_products': (c_int, [c_vp, c_int, c_vp, c_int, c_int, c_i64, c_i64, c_vp, c_i64, c_vp]),
     'okmi_transition_flux': (c_int, [c_vp, c_int, c_int, c_i64, c_i64, c_vp, c_int, c_vp, c_i64, c_vp]),
     'okmi_cube_format': (c_sz, [ctypes.POINTER(c_dbl), c_int, c_i64, c_i64, c_i64, ctypes.c_char_p, c_sz]),
+    'okmi_ci_workspace_bytes': (c_sz, [c_int]),
+    'okmi_ci_reduce': (c_int, [c_vp, c_vp, c_int, c_int, c_i64, c_i64, ctypes.POINTER(c_dbl), P_int, P_int, c_int,
+                               c_int, c_vp, c_i64, c_vp, c_sz, c_vp]),
     'okmi_batch_create': (c_int, [c_int, ctypes.POINTER(c_vp), ctypes.POINTER(OkmiGrid), ctypes.POINTER(c_vp),
                                   ctypes.POINTER(c_vp), P_int, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), c_int, c_int,
                                   ctypes.POINTER(c_vp)]),

@@ -13,7 +13,7 @@ OBJ = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'liborbkit_amd.so')
 INCLUDE = os.path.join(os.path.dirname(HERE), 'include')
 
-SOURCES = ['abi.cpp', 'ao_eval.hip', 'rho_gemm.hip', 'rho_gemm_m0.hip', 'rho_gemm_m1.hip', 'rho_fused.hip', 'rho_backward.hip', 'rho_big.hip', 'electrons.hip', 'extras.hip', 'cube.cpp']
+SOURCES = ['abi.cpp', 'ao_eval.hip', 'rho_gemm.hip', 'rho_gemm_m0.hip', 'rho_gemm_m1.hip', 'rho_fused.hip', 'rho_backward.hip', 'rho_big.hip', 'electrons.hip', 'extras.hip', 'cube.cpp', 'detci.hip']
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17', '-x', 'hip',

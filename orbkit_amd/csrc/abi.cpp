@@ -686,3 +686,19 @@ extern "C" int okmi_transition_flux(const double* d_mo, int nmo, int nd, int64_t
     }
     return okmi_launch_jmo(d_mo, nmo, nd, np, ld_in, d_pairs, npair, d_out, ld_out, (hipStream_t)stream);
 }
+
+extern "C" size_t okmi_ci_workspace_bytes(int nterm) { return nterm < 0 ? 0 : okmi_ci_ws_bytes(nterm); }
+
+extern "C" int okmi_ci_reduce(const double* d_mo, const double* d_dmo, int nmo, int nd, int64_t npts, int64_t ld,
+                              const double* coef, const int* ia, const int* ib, int nterm, int mode, double* d_out,
+                              int64_t ld_out, void* d_ws, size_t ws_bytes, void* stream)
+{
+    g_err.clear();
+    if (!d_mo || !d_out || nmo <= 0 || npts < 0 || ld < npts || nterm < 0 || mode < -1 || mode > 1 ||
+        (nterm > 0 && (!coef || !ia || !ib)) || (mode >= 0 && (!d_dmo || nd <= 0 || ld_out < npts))) {
+        okmi_set_error("ci_reduce: invalid arguments");
+        return 1;
+    }
+    return okmi_launch_ci(d_mo, d_dmo, nmo, nd, npts, ld, coef, ia, ib, nterm, mode, d_out, ld_out, d_ws, ws_bytes,
+                          (hipStream_t)stream);
+}

@@ -109,6 +109,12 @@ int okmi_launch_mo_pairs(const double* d_bra, int na, const double* d_ket, int n
 int okmi_launch_jmo(const double* d_mo, int nmo, int nd, int64_t np, int64_t ld_in, const int* d_pairs, int npair,
                     double* d_out, int64_t ld_out, hipStream_t st);
 
+// detCI grid reductions (detci.hip); mode -1 density, 0 jab, 1 a_nabla_b
+size_t okmi_ci_ws_bytes(int nterm);
+int okmi_launch_ci(const double* d_mo, const double* d_dmo, int nmo, int nd, int64_t np, int64_t ld,
+                   const double* coef, const int* ia, const int* ib, int nterm, int mode, double* d_out, int64_t ld_out,
+                   void* ws, size_t ws_bytes, hipStream_t st);
+
 // compensated FP64 electron count (electrons.hip)
 size_t okmi_electrons_ws_bytes();
 int okmi_launch_electrons(const double* d_rho, int64_t n, double d3r, void* ws, size_t ws_bytes, double* d_out,
