@@ -91,6 +91,54 @@ __device__ __forceinline__ void cart_all(const ShellPows<L, R>& P, R S0, R S1, R
     }
 }
 
+// canonical index of component (lx, ly, lz) of angular momentum l
+__host__ __device__ constexpr int cart_index(int l, int lx, int ly, int lz)
+{
+    for (int c = 0; c < ncart(l); ++c)
+        if (cart_exp(l, c, 0) == lx && cart_exp(l, c, 1) == ly && cart_exp(l, c, 2) == lz) return c;
+    return -1;
+}
+
+// Value and first derivatives of all components from shared monomials:
+//   chi_c = m_c S0,   d_a chi_c = R_a (m_c S1) + e_a (m_{c - 1_a} S0),
+// m_c the degree-L monomial of component c, m_{c - 1_a} the degree-(L-1)
+// one with exponent a lowered (the same terms as cart_comp, D = 0..3,
+// evaluated once per shell and point instead of once per derivative).
+template <int L, int C, typename R>
+__device__ __forceinline__ void first_mono(const ShellPows<L, R>& P, R S1, R* m, R* t)
+{
+    if constexpr (C < ncart(L)) {
+        m[C] = mono<L, R>(P, cart_exp(L, C, 0), cart_exp(L, C, 1), cart_exp(L, C, 2));
+        t[C] = m[C] * S1;
+        first_mono<L, C + 1, R>(P, S1, m, t);
+    }
+}
+
+template <int L, int C, typename R>
+__device__ __forceinline__ void lower_mono(const ShellPows<L, R>& P, R S0, R* u)
+{
+    if constexpr (L > 0 && C < ncart(L - 1)) {
+        u[C] = mono<L, R>(P, cart_exp(L - 1, C, 0), cart_exp(L - 1, C, 1), cart_exp(L - 1, C, 2)) * S0;
+        lower_mono<L, C + 1, R>(P, S0, u);
+    }
+}
+
+template <int L, int A, int C, typename R>
+__device__ __forceinline__ void first_deriv(const R* t, const R* u, R RA, R* out)
+{
+    if constexpr (C < ncart(L)) {
+        constexpr int e0 = cart_exp(L, C, 0), e1 = cart_exp(L, C, 1), e2 = cart_exp(L, C, 2);
+        constexpr int ea = A == 0 ? e0 : A == 1 ? e1 : e2;
+        if constexpr (ea > 0) {
+            constexpr int lo = cart_index(L - 1, e0 - (A == 0), e1 - (A == 1), e2 - (A == 2));
+            out[C] = fma(RA, t[C], R(ea) * u[lo]);
+        } else {
+            out[C] = RA * t[C];
+        }
+        first_deriv<L, A, C + 1, R>(t, u, RA, out);
+    }
+}
+
 // runtime (wave-uniform) derivative code -> compile-time evaluation
 template <int L, typename R>
 __device__ __forceinline__ void cart_drv(const ShellPows<L, R>& P, int d, R S0, R S1, R S2, R* out)

@@ -137,6 +137,28 @@ struct Box {
     }
 };
 
+// output rows of one shell and AO set from its canonical Cartesian values,
+// rounded once to FP32 into the LDS tile (row stride LDS_LD)
+template <int L>
+__device__ __forceinline__ void store_rows(const FusedArgs& a, const ShellRec& sh, const double* c, float* dst)
+{
+    constexpr int NC = ncart(L);
+    auto store = [&](int r, double v) { dst[r * LDS_LD] = (float)v; };
+    if (sh.kind == SHELL_T_SPHERICAL) {
+        sph_rows_to<L, 0>(c, store);
+    } else if (sh.kind == SHELL_T_CARTESIAN) {
+        cart_rows_to<L, 0>(c, store);
+    } else {
+        const OKMI_CONST double* T = cptr(a.T) + sh.t0;
+        for (int r = 0; r < sh.nrow; ++r) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) v = fma(T[r * NC + k], c[k], v);
+            store(r, v);
+        }
+    }
+}
+
 template <int L, int PASS>
 __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const ShellRec& sh, double d2, double px,
                                                  double py, double pz, int k0, float* lds, int lane)
@@ -172,6 +194,27 @@ __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const Shell
     ShellPows<L, double> P;
     P.init(X, Y, Z);
     constexpr int NC = ncart(L);
+    if constexpr (PASS == 1) {
+        // value and x, y, z derivatives from shared monomials
+        double m[NC], t[NC], u[L > 0 ? ncart(L - 1) : 1], c[NC];
+        first_mono<L, 0, double>(P, S1, m, t);
+        lower_mono<L, 0, double>(P, S0, u);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (s == 0) {
+#pragma unroll
+                for (int k = 0; k < NC; ++k) c[k] = m[k] * S0;
+            } else if (s == 1) {
+                first_deriv<L, 0, 0, double>(t, u, X, c);
+            } else if (s == 2) {
+                first_deriv<L, 1, 0, double>(t, u, Y, c);
+            } else {
+                first_deriv<L, 2, 0, double>(t, u, Z, c);
+            }
+            store_rows<L>(a, sh, c, lds + s * KC * LDS_LD + (sh.row0 - k0) * LDS_LD + lane);
+        }
+        return;
+    }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         double c[NC];
@@ -187,21 +230,7 @@ __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const Shell
         case 8: cart_all<L, 8, double>(P, S0, S1, S2, c); break;
         default: cart_all<L, 9, double>(P, S0, S1, S2, c); break;
         }
-        float* dst = lds + s * KC * LDS_LD + (sh.row0 - k0) * LDS_LD + lane;
-        auto store = [&](int r, double v) { dst[r * LDS_LD] = (float)v; };
-        if (sh.kind == SHELL_T_SPHERICAL) {
-            sph_rows_to<L, 0>(c, store);
-        } else if (sh.kind == SHELL_T_CARTESIAN) {
-            cart_rows_to<L, 0>(c, store);
-        } else {
-            const OKMI_CONST double* T = cptr(a.T) + sh.t0;
-            for (int r = 0; r < sh.nrow; ++r) {
-                double v = 0.0;
-#pragma unroll
-                for (int k = 0; k < NC; ++k) v = fma(T[r * NC + k], c[k], v);
-                dst[r * LDS_LD] = (float)v;
-            }
-        }
+        store_rows<L>(a, sh, c, lds + s * KC * LDS_LD + (sh.row0 - k0) * LDS_LD + lane);
     }
 }
 
