@@ -343,9 +343,9 @@ extern "C" int okmi_basis_destroy(okmi_basis* b)
     if (b->d_prims) (void)hipFree(b->d_prims);
     if (b->d_T) (void)hipFree(b->d_T);
     if (b->d_chunks) (void)hipFree(b->d_chunks);
-    if (b->d_work) (void)hipFree(b->d_work);
     if (b->d_fshells) (void)hipFree(b->d_fshells);
     if (b->d_fprims) (void)hipFree(b->d_fprims);
+    if (b->d_fgroups) (void)hipFree(b->d_fgroups);
     if (b->d_oshells) (void)hipFree(b->d_oshells);
     if (b->d_colmap) (void)hipFree(b->d_colmap);
     delete[] b->h_shells;

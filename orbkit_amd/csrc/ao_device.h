@@ -100,17 +100,16 @@ __host__ __device__ constexpr int cart_index(int l, int lx, int ly, int lz)
 }
 
 // Value and first derivatives of all components from shared monomials:
-//   chi_c = m_c S0,   d_a chi_c = R_a (m_c S1) + e_a (m_{c - 1_a} S0),
+//   chi_c = m_c S0,   d_a chi_c = (R_a S1) m_c + e_a (m_{c - 1_a} S0),
 // m_c the degree-L monomial of component c, m_{c - 1_a} the degree-(L-1)
 // one with exponent a lowered (the same terms as cart_comp, D = 0..3,
 // evaluated once per shell and point instead of once per derivative).
 template <int L, int C, typename R>
-__device__ __forceinline__ void first_mono(const ShellPows<L, R>& P, R S1, R* m, R* t)
+__device__ __forceinline__ void first_mono(const ShellPows<L, R>& P, R* m)
 {
     if constexpr (C < ncart(L)) {
         m[C] = mono<L, R>(P, cart_exp(L, C, 0), cart_exp(L, C, 1), cart_exp(L, C, 2));
-        t[C] = m[C] * S1;
-        first_mono<L, C + 1, R>(P, S1, m, t);
+        first_mono<L, C + 1, R>(P, m);
     }
 }
 
@@ -123,19 +122,20 @@ __device__ __forceinline__ void lower_mono(const ShellPows<L, R>& P, R S0, R* u)
     }
 }
 
+// out[c] = d_a chi_c for axis A, with RS1 = R_a S1
 template <int L, int A, int C, typename R>
-__device__ __forceinline__ void first_deriv(const R* t, const R* u, R RA, R* out)
+__device__ __forceinline__ void first_deriv(const R* m, const R* u, R RS1, R* out)
 {
     if constexpr (C < ncart(L)) {
         constexpr int e0 = cart_exp(L, C, 0), e1 = cart_exp(L, C, 1), e2 = cart_exp(L, C, 2);
         constexpr int ea = A == 0 ? e0 : A == 1 ? e1 : e2;
         if constexpr (ea > 0) {
             constexpr int lo = cart_index(L - 1, e0 - (A == 0), e1 - (A == 1), e2 - (A == 2));
-            out[C] = fma(RA, t[C], R(ea) * u[lo]);
+            out[C] = fma(RS1, m[C], R(ea) * u[lo]);
         } else {
-            out[C] = RA * t[C];
+            out[C] = RS1 * m[C];
         }
-        first_deriv<L, A, C + 1, R>(t, u, RA, out);
+        first_deriv<L, A, C + 1, R>(m, u, RS1, out);
     }
 }
 

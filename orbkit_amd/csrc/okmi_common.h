@@ -39,6 +39,16 @@ struct PrimRec {
     double rcut2;        // beyond this squared distance this primitive's terms are below 1e-40
 };
 
+// Four primitives of one shell, each field grouped so that a group is read
+// by a few wide scalar loads and one wait.  Unused slots have alpha = d = 0
+// and rcut2 = -1 (never within range).
+struct PrimGroup {
+    double alpha[4];
+    double d[4];
+    double rcut2[4];
+};
+static_assert(sizeof(PrimGroup) == 96, "PrimGroup layout");
+
 // Points: either a vector grid (x[i], y[i], z[i]) or a regular grid given by
 // its axes, flattened with x slowest and z fastest (orbkit/cy_grid.pyx:12-30).
 struct GridDesc {
@@ -100,6 +110,19 @@ __device__ __forceinline__ PrimRec load_prim(const PrimRec* base, int i)
     r.d = p->d;
     r.rcut2 = p->rcut2;
     return r;
+}
+
+__device__ __forceinline__ PrimGroup load_group(const PrimGroup* base, int i)
+{
+    const OKMI_CONST PrimGroup* p = cptr(base) + i;
+    PrimGroup g;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        g.alpha[j] = p->alpha[j];
+        g.d[j] = p->d[j];
+        g.rcut2[j] = p->rcut2[j];
+    }
+    return g;
 }
 
 __device__ __forceinline__ int2 load_int2(const int2* base, int i)

@@ -30,10 +30,10 @@ struct OkmiBasis {
     // pad the chunks), colmap maps internal rows to output rows (-1: pad),
     // and screening radii for the bound OKMI_SCREEN_FUSED.
     int2* d_chunks = nullptr;   // per chunk: first work entry, shells | used rows << 16
-    int* d_work = nullptr;
     ShellRec* d_fshells = nullptr;   // row0 = internal row, rcut2 = fused radius
     ShellRec* d_oshells = nullptr;   // output rows, rcut2 = fused radius
     PrimRec* d_fprims = nullptr;     // rcut2 = fused radius
+    PrimGroup* d_fgroups = nullptr;  // fused-kernel primitives, groups of 4 per shell in work order
     int* d_colmap = nullptr;         // [nkpad]
     int nchunk = 0;
     int nkpad = 0;

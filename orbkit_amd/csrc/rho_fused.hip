@@ -4,8 +4,8 @@
 // A workgroup of four waves owns a tile of 64 grid points: a 4x4x4 brick of a
 // regular grid (compact in space) or 64 consecutive points of a vector grid.
 // The AO rows are processed in chunks of 32 (one K step of the bf16 MFMA).
-// For each chunk the four waves share out the shells overlapping it (a
-// host-balanced work list; lane = point) and evaluate them in FP64
+// For each chunk the four waves take the chunk's shells one at a time from a
+// shared counter, costliest first (lane = point), and evaluate them in FP64
 // (coordinates, exponentials, polynomials, the Cartesian-to-spherical map),
 // storing each value rounded once to FP32 in LDS [row][point] (double
 // buffered, one barrier per chunk).  Each wave then multiplies the rows of its
@@ -41,11 +41,11 @@ constexpr int LDS_LD = 65;    // padded row stride (floats) of the LDS AO tile
 constexpr int PTS = 64;       // points per wave
 
 struct FusedArgs {
-    const ShellRec* shells;
-    const PrimRec* prims;
+    const ShellRec* shells;   // in work order: per chunk its shells, costliest first
+                              // (prim0: first primitive group)
+    const PrimGroup* groups;  // primitives, groups of 4 per shell
     const double* T;          // general transform blocks (kind 0 shells)
-    const int2* chunks;       // per chunk: first entry in `work`, entry count
-    const int* work;          // shell index | (evaluating wave << 24), per chunk
+    const int2* chunks;       // per chunk: first shell, shell count | (used rows << 16)
     GridDesc grid;
     int64_t p0, np;
     int nao, nchunk;
@@ -169,26 +169,26 @@ __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const Shell
     const double X = px - sh.cx, Y = py - sh.cy, Z = pz - sh.cz;
     const double rr = X * X + Y * Y + Z * Z;
     double S0 = 0.0, S1 = 0.0, S2 = 0.0;
-    // primitives four at a time: independent exponential chains (the loop
-    // count is a runtime value); a primitive whose radius misses the tile, or
-    // past the shell's last one, gets t = 1e4 and contributes exactly 0
-    for (int p0 = 0; p0 < sh.nprim; p0 += 4) {
-        double de[4], al[4];
-        bool any = false;
+    // primitives four at a time (one group: independent exponential chains,
+    // one wait for its loads); a primitive whose radius misses the tile, or
+    // an unused slot, gets t = 1e4 and contributes exactly 0
+    for (int q = 0; q < sh.nprim; q += 4) {
+        const PrimGroup pg = load_group(a.groups, sh.prim0 + (q >> 2));
+        bool live[4], any = false;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const PrimRec pr = load_prim(a.prims, sh.prim0 + min(p0 + j, sh.nprim - 1));
-            const bool live = p0 + j < sh.nprim && d2 < pr.rcut2;   // wave-uniform
-            any |= live;
-            al[j] = pr.alpha;
-            de[j] = pr.d * gauss_f64_fast(live ? pr.alpha * rr : 1.0e4);
+            live[j] = d2 < pg.rcut2[j];   // wave-uniform
+            any |= live[j];
         }
         if (!any) continue;
+        double de[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) de[j] = pg.d[j] * gauss_f64_fast(live[j] ? pg.alpha[j] * rr : 1.0e4);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             S0 += de[j];
-            if constexpr (need1) S1 = fma(-2.0 * al[j], de[j], S1);
-            if constexpr (need2) S2 = fma(4.0 * al[j] * al[j], de[j], S2);
+            if constexpr (need1) S1 = fma(-2.0 * pg.alpha[j], de[j], S1);
+            if constexpr (need2) S2 = fma(4.0 * pg.alpha[j] * pg.alpha[j], de[j], S2);
         }
     }
     ShellPows<L, double> P;
@@ -196,8 +196,8 @@ __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const Shell
     constexpr int NC = ncart(L);
     if constexpr (PASS == 1) {
         // value and x, y, z derivatives from shared monomials
-        double m[NC], t[NC], u[L > 0 ? ncart(L - 1) : 1], c[NC];
-        first_mono<L, 0, double>(P, S1, m, t);
+        double m[NC], u[L > 0 ? ncart(L - 1) : 1], c[NC];
+        first_mono<L, 0, double>(P, m);
         lower_mono<L, 0, double>(P, S0, u);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
@@ -205,11 +205,11 @@ __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const Shell
 #pragma unroll
                 for (int k = 0; k < NC; ++k) c[k] = m[k] * S0;
             } else if (s == 1) {
-                first_deriv<L, 0, 0, double>(t, u, X, c);
+                first_deriv<L, 0, 0, double>(m, u, X * S1, c);
             } else if (s == 2) {
-                first_deriv<L, 1, 0, double>(t, u, Y, c);
+                first_deriv<L, 1, 0, double>(m, u, Y * S1, c);
             } else {
-                first_deriv<L, 2, 0, double>(t, u, Z, c);
+                first_deriv<L, 2, 0, double>(m, u, Z * S1, c);
             }
             store_rows<L>(a, sh, c, lds + s * KC * LDS_LD + (sh.row0 - k0) * LDS_LD + lane);
         }
@@ -239,128 +239,112 @@ struct FusedLds {
     static constexpr int BUF = PassCodes<PASS>::n * KC * LDS_LD;   // floats per buffer
 };
 
-// One tile of 64 points: every thread of the workgroup calls this with the
-// same tile.  `lds` holds two AO buffers, `next` two work counters, `live`
-// two chunk tags.  The first barrier orders it after any previous tile's LDS
-// reads.
-template <int PREC, int PASS, int MT>
-__device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float* lds, int* next, int* live)
+// Operand A (MO coefficients) of one chunk, in the lane layout of one wave.
+template <int PREC, int MT>
+struct COperand {
+    Split<PREC> ca[MT];
+    float cf[MT][KC / 4];
+};
+
+template <int PREC, int MT>
+__device__ __forceinline__ void load_c(const FusedArgs& a, int k0, int g, int n16, COperand<PREC, MT>& c)
+{
+    if constexpr (PREC == OKMI_PREC_F32) {
+        const float* C = (const float*)a.C;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int ks = 0; ks < KC / 4; ++ks)
+                c.cf[mt][ks] = C[(int64_t)(mt * 16 + n16) * a.kpad + k0 + 4 * ks + g];
+    } else {
+        const __bf16* C = (const __bf16*)a.C;
+        constexpr int NP = PREC == OKMI_PREC_BF16X6 ? 3 : 2;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int q = 0; q < NP; ++q)
+                c.ca[mt].p[q] = *(const bf8*)(C + ((int64_t)q * a.mpad + mt * 16 + n16) * a.kpad + k0 + 8 * g);
+    }
+}
+
+// AO rows of chunk `ch` for the tile's 64 points (lane = point) into the LDS
+// tile L.  The calling waves take the chunk's shells one at a time from the
+// counter *next (costliest first), so that screening cannot leave one wave
+// with most of the work; a wave that evaluates a shell sets *live = tag.
+// Waves with `pad` set also clear the rows after the chunk's last shell.
+template <int PASS>
+__device__ __forceinline__ void eval_chunk(const FusedArgs& a, int ch, const Box& box, double px, double py, double pz,
+                                           float* L, int* next, int* live, int tag, bool pad, int lane)
 {
     constexpr int NS = PassCodes<PASS>::n;
-    constexpr int BUF = FusedLds<PASS>::BUF;
-    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (threadIdx.x < 2) next[threadIdx.x] = 0;
-    __syncthreads();
-    int64_t pidx;
-    bool pact;
-    double px, py, pz;
-    tile_point(a, tile, lane, pidx, pact, px, py, pz);   // lane l of every wave: point l of the tile
-    Box box;
-    box.lo[0] = wave_min(px); box.hi[0] = wave_max(px);
-    box.lo[1] = wave_min(py); box.hi[1] = wave_max(py);
-    box.lo[2] = wave_min(pz); box.hi[2] = wave_max(pz);
-
-    f4 acc[MT][NS];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-        for (int s = 0; s < NS; ++s) acc[mt][s] = f4{0.f, 0.f, 0.f, 0.f};
-
-    const int g = lane >> 4, n16 = lane & 15;
-    int buf = 0;
-    for (int ch = 0; ch < a.nchunk; ++ch) {
-        const int k0 = ch * KC;
-        const int2 cw = load_int2(a.chunks, ch);
-        const int nsh = cw.y & 0xffff, used = cw.y >> 16;
-        float* L = lds + buf * BUF;
-        // operand A (MO coefficients) for this chunk
-        Split<PREC> ca[MT];
-        float cf[MT][KC / 4];
-        if constexpr (PREC == OKMI_PREC_F32) {
-            const float* C = (const float*)a.C;
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                for (int ks = 0; ks < KC / 4; ++ks)
-                    cf[mt][ks] = C[(int64_t)(mt * 16 + n16) * a.kpad + k0 + 4 * ks + g];
-        } else {
-            const __bf16* C = (const __bf16*)a.C;
-            constexpr int NP = PREC == OKMI_PREC_BF16X6 ? 3 : 2;
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                for (int q = 0; q < NP; ++q)
-                    ca[mt].p[q] = *(const bf8*)(C + ((int64_t)q * a.mpad + mt * 16 + n16) * a.kpad + k0 + 8 * g);
-        }
-        // AO rows [k0, k0+KC): the waves take the chunk's shells one at a
-        // time from a shared counter (costliest first), so that screening
-        // cannot leave one wave with most of the work
-        if (wave == 0 && used < KC) {   // pad rows after the chunk's last shell
-            for (int s = 0; s < NS; ++s)
-                for (int r = used; r < KC; ++r) L[s * KC * LDS_LD + r * LDS_LD + lane] = 0.f;
-        }
-        // the other buffer's counter: every wave finished taking from it
-        // before the previous barrier, the next chunk reads it after the next
-        if (threadIdx.x == 0) next[buf ^ 1] = 0;
-        for (;;) {
-            int e = 0;
-            if (lane == 0) e = atomicAdd(&next[buf], 1);
-            e = __builtin_amdgcn_readfirstlane(__shfl(e, 0));
-            if (e >= nsh) break;
-            const ShellRec sh = load_shell(a.shells, cptr(a.work)[cw.x + e] & 0xffffff);
-            const double d2 = box.dist2(sh);
-            if (d2 < sh.rcut2) {
-                if (lane == 0) live[buf] = ch;
-                switch (sh.l) {
-                case 0: eval_shell_chunk<0, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
-                case 1: eval_shell_chunk<1, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
-                case 2: eval_shell_chunk<2, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
-                case 3: eval_shell_chunk<3, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
-                default: eval_shell_chunk<4, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
-                }
-            } else {
-                for (int s = 0; s < NS; ++s)
-                    for (int r = 0; r < sh.nrow; ++r) L[s * KC * LDS_LD + (sh.row0 - k0 + r) * LDS_LD + lane] = 0.f;
-            }
-        }
-        // All rows of buffer `buf` written; the other buffer is free because
-        // every wave has passed the previous barrier after reading it.
-        __syncthreads();
-        // No shell of the chunk reaches the tile: every row is zero, skip
-        // the contraction (same decision in all waves).  live[buf] is tagged
-        // with the chunk index instead of being reset: it is written before
-        // this barrier and next written (chunk ch + 2) after the following
-        // one, and a stale value can only let a zero chunk through.
-        if (live[buf] != ch) {
-            buf ^= 1;
-            continue;
-        }
-        // contraction of this wave's 16-point column
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const float* src = L + s * KC * LDS_LD + wave * 16 + n16;
-            if constexpr (PREC == OKMI_PREC_F32) {
-#pragma unroll
-                for (int ks = 0; ks < KC / 4; ++ks) {
-                    const float b = src[(4 * ks + g) * LDS_LD];
-#pragma unroll
-                    for (int mt = 0; mt < MT; ++mt)
-                        acc[mt][s] = __builtin_amdgcn_mfma_f32_16x16x4f32(cf[mt][ks], b, acc[mt][s], 0, 0, 0);
-                }
-            } else {
-                float bx[8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) bx[j] = src[(8 * g + j) * LDS_LD];
-                Split<PREC> bs;
-                split8<PREC>(bx, bs);
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt) acc[mt][s] = mfma_split<PREC>(ca[mt], bs, acc[mt][s]);
-            }
-        }
-        buf ^= 1;
+    const int k0 = ch * KC;
+    const int2 cw = load_int2(a.chunks, ch);
+    const int nsh = cw.y & 0xffff, used = cw.y >> 16;
+    if (pad && used < KC) {
+        for (int s = 0; s < NS; ++s)
+            for (int r = used; r < KC; ++r) L[s * KC * LDS_LD + r * LDS_LD + lane] = 0.f;
     }
+    for (;;) {
+        int e = 0;
+        if (lane == 0) e = atomicAdd(next, 1);
+        e = __builtin_amdgcn_readfirstlane(__shfl(e, 0));
+        if (e >= nsh) break;
+        const ShellRec sh = load_shell(a.shells, cw.x + e);
+        const double d2 = box.dist2(sh);
+        if (d2 < sh.rcut2) {
+            if (lane == 0) *live = tag;
+            switch (sh.l) {
+            case 0: eval_shell_chunk<0, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
+            case 1: eval_shell_chunk<1, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
+            case 2: eval_shell_chunk<2, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
+            case 3: eval_shell_chunk<3, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
+            default: eval_shell_chunk<4, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
+            }
+        } else {
+            for (int s = 0; s < NS; ++s)
+                for (int r = 0; r < sh.nrow; ++r) L[s * KC * LDS_LD + (sh.row0 - k0 + r) * LDS_LD + lane] = 0.f;
+        }
+    }
+}
 
-    // epilogue: lane holds Phi[m = 16 mt + 4 g + i][tile point 16 wave + n16]
+// Contraction of one chunk (LDS tile L) for the 16-point column `col` of the
+// tile: acc[mt][s] += C[16 mt .. 16 mt + 15][chunk] * AO_s[chunk][column].
+template <int PREC, int PASS, int MT>
+__device__ __forceinline__ void contract(const float* L, int col, int g, int n16, const COperand<PREC, MT>& c,
+                                         f4 (&acc)[MT][PassCodes<PASS>::n])
+{
+    constexpr int NS = PassCodes<PASS>::n;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const float* src = L + s * KC * LDS_LD + col * 16 + n16;
+        if constexpr (PREC == OKMI_PREC_F32) {
+#pragma unroll
+            for (int ks = 0; ks < KC / 4; ++ks) {
+                const float b = src[(4 * ks + g) * LDS_LD];
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                    acc[mt][s] = __builtin_amdgcn_mfma_f32_16x16x4f32(c.cf[mt][ks], b, acc[mt][s], 0, 0, 0);
+            }
+        } else {
+            float bx[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) bx[j] = src[(8 * g + j) * LDS_LD];
+            Split<PREC> bs;
+            split8<PREC>(bx, bs);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) acc[mt][s] = mfma_split<PREC>(c.ca[mt], bs, acc[mt][s]);
+        }
+    }
+}
+
+// Density terms of the 16-point column `col` of the tile from its MO values:
+// lane holds Phi[m = 16 mt + 4 g + i][tile point 16 col + n16].
+template <int PREC, int PASS, int MT>
+__device__ __forceinline__ void epilogue(const FusedArgs& a, int64_t tile, int col, int lane,
+                                         const f4 (&acc)[MT][PassCodes<PASS>::n])
+{
+    constexpr int NS = PassCodes<PASS>::n;
+    const int g = lane >> 4, n16 = lane & 15;
     double r0 = 0.0, t[NS > 1 ? NS - 1 : 1], u[6];
 #pragma unroll
     for (int s = 0; s < (NS > 1 ? NS - 1 : 1); ++s) t[s] = 0.0;
@@ -405,7 +389,7 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float
     int64_t p;
     bool act;
     double qx, qy, qz;
-    tile_point(a, tile, wave * 16 + n16, p, act, qx, qy, qz);
+    tile_point(a, tile, col * 16 + n16, p, act, qx, qy, qz);
     if (g == 0 && act) {
         if constexpr (PASS <= 1) a.rho[p] = r0;
 #pragma unroll
@@ -419,6 +403,71 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float
                 if (a.need_mask >> (4 + q) & 1) a.term2[(4 + q) * a.ld_term + p] = u[q];
         }
     }
+}
+
+__device__ __forceinline__ Box tile_box(double px, double py, double pz)
+{
+    Box box;
+    box.lo[0] = wave_min(px); box.hi[0] = wave_max(px);
+    box.lo[1] = wave_min(py); box.hi[1] = wave_max(py);
+    box.lo[2] = wave_min(pz); box.hi[2] = wave_max(pz);
+    return box;
+}
+
+// One tile of 64 points by four waves that all evaluate and all contract:
+// every thread of the workgroup calls this with the same tile.  `lds` holds
+// two AO buffers, `next` two work counters, `live` two chunk tags.  The first
+// barrier orders it after any previous tile's LDS reads.
+template <int PREC, int PASS, int MT>
+__device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float* lds, int* next, int* live)
+{
+    constexpr int NS = PassCodes<PASS>::n;
+    constexpr int BUF = FusedLds<PASS>::BUF;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = lane >> 4, n16 = lane & 15;
+    if (threadIdx.x < 2) next[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t pidx;
+    bool pact;
+    double px, py, pz;
+    tile_point(a, tile, lane, pidx, pact, px, py, pz);   // lane l of every wave: point l of the tile
+    const Box box = tile_box(px, py, pz);
+
+    f4 acc[MT][NS];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) acc[mt][s] = f4{0.f, 0.f, 0.f, 0.f};
+
+    // The C operand is loaded before the AO evaluation (its latency hidden
+    // there) unless holding it through the evaluation would exceed the
+    // register budget: second-derivative passes and 4 MO tiles load it
+    // after the barrier.
+    constexpr bool c_early = PASS <= 1 ? MT <= 3 : MT <= 2;
+    int buf = 0;
+    for (int ch = 0; ch < a.nchunk; ++ch) {
+        float* L = lds + buf * BUF;
+        COperand<PREC, MT> c;
+        if constexpr (c_early) load_c<PREC, MT>(a, ch * KC, g, n16, c);
+        // the other buffer's counter: every wave finished taking from it
+        // before the previous barrier, the next chunk reads it after the next
+        if (threadIdx.x == 0) next[buf ^ 1] = 0;
+        eval_chunk<PASS>(a, ch, box, px, py, pz, L, &next[buf], &live[buf], ch, wave == 0, lane);
+        // All rows of buffer `buf` written; the other buffer is free because
+        // every wave has passed the previous barrier after reading it.
+        __syncthreads();
+        // No shell of the chunk reaches the tile: every row is zero, skip
+        // the contraction (same decision in all waves).  live[buf] is tagged
+        // with the chunk index instead of being reset: it is written before
+        // this barrier and next written (chunk ch + 2) after the following
+        // one, and a stale value can only let a zero chunk through.
+        if (live[buf] == ch) {
+            if constexpr (!c_early) load_c<PREC, MT>(a, ch * KC, g, n16, c);
+            contract<PREC, PASS, MT>(L, wave, g, n16, c, acc);
+        }
+        buf ^= 1;
+    }
+    epilogue<PREC, PASS, MT>(a, tile, wave, lane, acc);
 }
 
 template <int PREC, int PASS, int MT>
@@ -554,21 +603,39 @@ int prepare_fused_tables(OkmiBasis* b)
         ch[c] = make_int2((int)work.size(), (int)cost.size() | (used << 16));
         for (auto& cs : cost) work.push_back(cs.second);
     }
-    std::vector<ShellRec> fs(b->h_shells, b->h_shells + b->nshell);
+    // shells in work-list order (entry j of the list is record j), each
+    // with its primitives in groups of 4 (prim0: first group)
+    std::vector<ShellRec> fs(work.size());
+    std::vector<PrimGroup> groups;
     std::vector<int> colmap(nkpad, -1);
-    for (int i = 0; i < b->nshell; ++i) {
-        for (int r = 0; r < fs[i].nrow; ++r) colmap[krow[i] + r] = fs[i].row0 + r;
-        fs[i].row0 = krow[i];
-        fs[i].rcut2 = b->h_frcut2[i];
+    for (size_t j = 0; j < work.size(); ++j) {
+        const int i = work[j];
+        fs[j] = b->h_shells[i];
+        for (int r = 0; r < fs[j].nrow; ++r) colmap[krow[i] + r] = fs[j].row0 + r;
+        fs[j].row0 = krow[i];
+        fs[j].rcut2 = b->h_frcut2[i];
+        fs[j].prim0 = (int)groups.size();
+        const ShellRec& hs = b->h_shells[i];
+        for (int q = 0; q < hs.nprim; q += 4) {
+            PrimGroup pg;
+            for (int k = 0; k < 4; ++k) {
+                const bool used = q + k < hs.nprim;
+                const int pi = hs.prim0 + q + k;
+                pg.alpha[k] = used ? b->h_prims[pi].alpha : 0.0;
+                pg.d[k] = used ? b->h_prims[pi].d : 0.0;
+                pg.rcut2[k] = used ? b->h_fprcut2[pi] : -1.0;
+            }
+            groups.push_back(pg);
+        }
     }
     std::vector<ShellRec> os(b->h_shells, b->h_shells + b->nshell);
     for (int i = 0; i < b->nshell; ++i) os[i].rcut2 = b->h_frcut2[i];
     std::vector<PrimRec> fp(b->h_prims, b->h_prims + b->nprim);
     for (int i = 0; i < b->nprim; ++i) fp[i].rcut2 = b->h_fprcut2[i];
     int rc = upload(&b->d_chunks, ch, "rho_fused: chunk map");
-    if (!rc) rc = upload(&b->d_work, work, "rho_fused: work list");
     if (!rc) rc = upload(&b->d_fshells, fs, "rho_fused: shells");
     if (!rc) rc = upload(&b->d_fprims, fp, "rho_fused: primitives");
+    if (!rc) rc = upload(&b->d_fgroups, groups, "rho_fused: primitive groups");
     if (!rc) rc = upload(&b->d_oshells, os, "rho_fused: output-row shells");
     if (!rc) rc = upload(&b->d_colmap, colmap, "rho_fused: row map");
     if (rc) return rc;
@@ -640,7 +707,7 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
     const bool p1 = (need & ~1) != 0, p2 = (need & (7 << 4)) != 0, p3 = (need & (7 << 7)) != 0;
 
     FusedArgs a{};
-    a.shells = b->d_fshells; a.prims = b->d_fprims; a.T = b->d_T; a.chunks = b->d_chunks; a.work = b->d_work;
+    a.shells = b->d_fshells; a.groups = b->d_fgroups; a.T = b->d_T; a.chunks = b->d_chunks;
     a.grid = g; a.nao = b->nao; a.nchunk = b->nchunk;
     a.C = cpl; a.kpad = kpad; a.mpad = mpad; a.occ = occp;
     a.term = term; a.term2 = term2; a.ld_term = chunk; a.need_mask = need;
@@ -926,8 +993,8 @@ int okmi_batch_build(int nmol, OkmiBasis* const* bases, const GridDesc* grids, c
         const OkmiBasis* bs = bases[m];
         const GridDesc& g = grids[m];
         FusedArgs a{};
-        a.shells = bs->d_fshells; a.prims = bs->d_fprims; a.T = bs->d_T;
-        a.chunks = bs->d_chunks; a.work = bs->d_work;
+        a.shells = bs->d_fshells; a.groups = bs->d_fgroups; a.T = bs->d_T;
+        a.chunks = bs->d_chunks;
         a.grid = g; a.p0 = 0; a.np = npts[m];
         a.nao = bs->nao; a.nchunk = bs->nchunk;
         a.kpad = bs->nkpad; a.mpad = B->mpad;

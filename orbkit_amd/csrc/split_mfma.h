@@ -1,6 +1,7 @@
 // bf16 operand splitting for FP32-accurate products on the bf16 matrix
 // cores, shared by the fused density kernel and the large-MO GEMM.
-//   x = hi + mid (+ lo): BF16X3 keeps (hi, mid) and 3 products,
+//   x = hi + mid (+ lo), each part rounded to nearest bf16 from the exact
+//   residual of the previous ones: BF16X3 keeps (hi, mid) and 3 products,
 //   BF16X6 keeps (hi, mid, lo) and 6 products (terms below 2^-24 dropped).
 #pragma once
 #include "okmi_common.h"
@@ -15,44 +16,52 @@ struct Split {
     bf8 p[PREC == OKMI_PREC_BF16X6 ? 3 : 2];
 };
 
-// x = hi + mid (+ lo) exactly up to the last part: hi and mid keep the top 16
-// bits of x and of the residual x - hi (a truncating split, so every
-// residual is exact in FP32); the last part is the residual rounded to
-// nearest.  Two bf16 of consecutive k share a dword (low half: even k).
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+
+// bf16 pair from its bit pattern, kept in a scalar register.  The compiler
+// would otherwise encode (-1, 0) as the 16-bit inline constant -1.0, which
+// the dot product reads as the FP16 number -1.0 (bit pattern 0xbc00, the
+// bf16 number -2^-7): residuals formed that way were wrong on gfx950.
+__device__ __forceinline__ bf2 pair_bits(unsigned bits)
+{
+    asm volatile("" : "+s"(bits));
+    return __builtin_bit_cast(bf2, bits);
+}
+
+// x - b.x and x - b.y for a bf16 pair b: one bf16 dot product with (-1, 0)
+// or (0, -1) accumulated onto x.  When b holds x rounded to bf16 the
+// difference needs at most 16 significant bits, so it is exact in FP32.
+__device__ __forceinline__ float res_lo(bf2 b, float x)
+{
+    return __builtin_amdgcn_fdot2_f32_bf16(b, pair_bits(0x0000bf80u), x, false);
+}
+__device__ __forceinline__ float res_hi(bf2 b, float x)
+{
+    return __builtin_amdgcn_fdot2_f32_bf16(b, pair_bits(0xbf800000u), x, false);
+}
+
+// x = hi + mid (+ lo) exactly up to the last part: each part is the
+// previous residual rounded to nearest bf16 (two per conversion), and every
+// residual is exact in FP32 (res_lo / res_hi).  Two bf16 of consecutive k
+// share a dword (low half: even k).
 template <int PREC>
 __device__ __forceinline__ void split8(const float (&x)[8], Split<PREC>& o)
 {
-    typedef unsigned u4 __attribute__((ext_vector_type(4)));
-    u4 h, m, lo;
-    float r1[8];
+    bf2 h[4], m[4], lo[4];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const unsigned xb = __float_as_uint(x[j]);
-        r1[j] = x[j] - __uint_as_float(xb & 0xffff0000u);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-        h[i] = __builtin_amdgcn_perm(__float_as_uint(x[2 * i + 1]), __float_as_uint(x[2 * i]), 0x07060302u);
-    if constexpr (PREC == OKMI_PREC_BF16X6) {
-        float r2[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) r2[j] = r1[j] - __uint_as_float(__float_as_uint(r1[j]) & 0xffff0000u);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            m[i] = __builtin_amdgcn_perm(__float_as_uint(r1[2 * i + 1]), __float_as_uint(r1[2 * i]), 0x07060302u);
-            const __bf16 a = (__bf16)r2[2 * i], b = (__bf16)r2[2 * i + 1];
-            lo[i] = (unsigned)__builtin_bit_cast(unsigned short, a) | ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
-        }
-        o.p[2] = __builtin_bit_cast(bf8, lo);
-    } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const __bf16 a = (__bf16)r1[2 * i], b = (__bf16)r1[2 * i + 1];
-            m[i] = (unsigned)__builtin_bit_cast(unsigned short, a) | ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
+    for (int i = 0; i < 4; ++i) {
+        h[i] = bf2{(__bf16)x[2 * i], (__bf16)x[2 * i + 1]};
+        const float r0 = res_lo(h[i], x[2 * i]), r1 = res_hi(h[i], x[2 * i + 1]);
+        m[i] = bf2{(__bf16)r0, (__bf16)r1};
+        if constexpr (PREC == OKMI_PREC_BF16X6) {
+            const float s0 = res_lo(m[i], r0), s1 = res_hi(m[i], r1);
+            lo[i] = bf2{(__bf16)s0, (__bf16)s1};
         }
     }
-    o.p[0] = __builtin_bit_cast(bf8, h);
-    o.p[1] = __builtin_bit_cast(bf8, m);
+    o.p[0] = bf8{h[0].x, h[0].y, h[1].x, h[1].y, h[2].x, h[2].y, h[3].x, h[3].y};
+    o.p[1] = bf8{m[0].x, m[0].y, m[1].x, m[1].y, m[2].x, m[2].y, m[3].x, m[3].y};
+    if constexpr (PREC == OKMI_PREC_BF16X6)
+        o.p[2] = bf8{lo[0].x, lo[0].y, lo[1].x, lo[1].y, lo[2].x, lo[2].y, lo[3].x, lo[3].y};
 }
 
 template <int PREC>
