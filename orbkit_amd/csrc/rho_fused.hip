@@ -159,6 +159,22 @@ __device__ __forceinline__ void store_rows(const FusedArgs& a, const ShellRec& s
     }
 }
 
+// radial sums of the first N primitives of a group:
+// S0 = sum d e, S1 = sum -2 alpha d e, S2 = sum 4 alpha^2 d e, e = exp(-alpha r^2)
+template <int N, bool need1, bool need2>
+__device__ __forceinline__ void prim_sums(const PrimGroup& pg, double rr, double& S0, double& S1, double& S2)
+{
+    double de[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) de[j] = pg.d[j] * gauss_f64_fast(pg.alpha[j] * rr);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        S0 += de[j];
+        if constexpr (need1) S1 = fma(-2.0 * pg.alpha[j], de[j], S1);
+        if constexpr (need2) S2 = fma(4.0 * pg.alpha[j] * pg.alpha[j], de[j], S2);
+    }
+}
+
 template <int L, int PASS>
 __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const ShellRec& sh, double d2, double px,
                                                  double py, double pz, int k0, float* lds, int lane)
@@ -169,27 +185,23 @@ __device__ __forceinline__ void eval_shell_chunk(const FusedArgs& a, const Shell
     const double X = px - sh.cx, Y = py - sh.cy, Z = pz - sh.cz;
     const double rr = X * X + Y * Y + Z * Z;
     double S0 = 0.0, S1 = 0.0, S2 = 0.0;
-    // primitives four at a time (one group: independent exponential chains,
-    // one wait for its loads); a primitive whose radius misses the tile, or
-    // an unused slot, gets t = 1e4 and contributes exactly 0
+    // Primitives four at a time (one group: independent exponential chains,
+    // one wait for its loads).  A shell's primitives are sorted by screening
+    // radius, largest first, so those that reach the tile lead: only they
+    // are evaluated, and a group that is not fully live ends the shell.
     for (int q = 0; q < sh.nprim; q += 4) {
         const PrimGroup pg = load_group(a.groups, sh.prim0 + (q >> 2));
-        bool live[4], any = false;
+        int nl = 0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            live[j] = d2 < pg.rcut2[j];   // wave-uniform
-            any |= live[j];
+        for (int j = 0; j < 4; ++j) nl += d2 < pg.rcut2[j];   // wave-uniform
+        switch (nl) {
+        case 4: prim_sums<4, need1, need2>(pg, rr, S0, S1, S2); break;
+        case 3: prim_sums<3, need1, need2>(pg, rr, S0, S1, S2); break;
+        case 2: prim_sums<2, need1, need2>(pg, rr, S0, S1, S2); break;
+        case 1: prim_sums<1, need1, need2>(pg, rr, S0, S1, S2); break;
+        default: break;
         }
-        if (!any) continue;
-        double de[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) de[j] = pg.d[j] * gauss_f64_fast(live[j] ? pg.alpha[j] * rr : 1.0e4);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            S0 += de[j];
-            if constexpr (need1) S1 = fma(-2.0 * pg.alpha[j], de[j], S1);
-            if constexpr (need2) S2 = fma(4.0 * pg.alpha[j] * pg.alpha[j], de[j], S2);
-        }
+        if (nl < 4) break;
     }
     ShellPows<L, double> P;
     P.init(X, Y, Z);
@@ -616,11 +628,16 @@ int prepare_fused_tables(OkmiBasis* b)
         fs[j].rcut2 = b->h_frcut2[i];
         fs[j].prim0 = (int)groups.size();
         const ShellRec& hs = b->h_shells[i];
+        // primitives by screening radius, largest first (ties keep file order)
+        std::vector<int> order(hs.nprim);
+        for (int k = 0; k < hs.nprim; ++k) order[k] = hs.prim0 + k;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return b->h_fprcut2[x] > b->h_fprcut2[y]; });
         for (int q = 0; q < hs.nprim; q += 4) {
             PrimGroup pg;
             for (int k = 0; k < 4; ++k) {
                 const bool used = q + k < hs.nprim;
-                const int pi = hs.prim0 + q + k;
+                const int pi = used ? order[q + k] : 0;
                 pg.alpha[k] = used ? b->h_prims[pi].alpha : 0.0;
                 pg.d[k] = used ? b->h_prims[pi].d : 0.0;
                 pg.rcut2[k] = used ? b->h_fprcut2[pi] : -1.0;
