@@ -19,9 +19,8 @@ struct AoArgs {
     int64_t ld;          // row stride of out (elements)
     int64_t set_stride;  // stride between derivative sets (elements)
     int nshell;
-    int ndrv;
-    int need1, need2;
-    int drv[OKMI_MAX_SETS];
+    int need1;                 // first derivatives (radial sum S1) needed
+    int dst[OKMI_MAX_SETS];    // output set of derivative code c, -1: not written by this launch
     // brick mode (regular grids): wave w of the launch takes brick bk0 + w
     // (4x4x4 points, z fastest) of the nbx x nby x nbz bricks and writes
     // columns 64 w + lane; lanes of a brick outside the grid repeat its
@@ -53,51 +52,95 @@ __device__ __forceinline__ double gauss_out(double t)
     else return gauss_f64_fast(t);
 }
 
+// rows of one shell from its canonical Cartesian values c, stored into the
+// AO set starting at o (row stride a.ld)
 template <int L, typename OutT>
+__device__ __forceinline__ void store_rows(const AoArgs& a, const ShellRec& sh, const double* c, OutT* o, bool act)
+{
+    constexpr int NC = ncart(L);
+    auto store = [&](int r, double v) {
+        if (act) o[(int64_t)r * a.ld] = (OutT)v;
+    };
+    // standard spherical / Cartesian shells: sparse compile-time rows;
+    // other transforms (ORCA row factors, custom maps): the dense block
+    if (sh.kind == SHELL_T_SPHERICAL) {
+        sph_rows_to<L, 0>(c, store);
+    } else if (sh.kind == SHELL_T_CARTESIAN) {
+        cart_rows_to<L, 0>(c, store);
+    } else {
+        const OKMI_CONST double* T = cptr(a.T) + sh.t0;
+        for (int r = 0; r < sh.nrow; ++r) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) v = fma(T[r * NC + k], c[k], v);
+            store(r, v);
+        }
+    }
+}
+
+// One shell at the lane's point for the derivative codes of group GRP:
+// 0 = value and first derivatives (codes 0-3, from shared monomials),
+// 1 = second derivatives (codes 4-9, cart_all; the mixed ones with orbkit's
+// formula).  Only codes with a destination set are written.
+template <int L, int GRP, typename OutT>
 __device__ __forceinline__ void shell_out(const AoArgs& a, const ShellRec& sh, double d2, double px, double py,
                                           double pz, int64_t i, bool act, OutT* out)
 {
     const double X = px - sh.cx, Y = py - sh.cy, Z = pz - sh.cz;
     const double rr = X * X + Y * Y + Z * Z;
+    const bool need1 = GRP == 1 || a.need1;
     double S0 = 0.0, S1 = 0.0, S2 = 0.0;
     for (int p = 0; p < sh.nprim; ++p) {
         const PrimRec pr = load_prim(a.prims, sh.prim0 + p);
         if (d2 >= pr.rcut2) continue;   // below 1e-40 for every point of the wave
         const double de = pr.d * gauss_out<OutT>(pr.alpha * rr);
         S0 += de;
-        if (a.need1) S1 += (-2.0 * pr.alpha) * de;
-        if (a.need2) S2 += (4.0 * pr.alpha * pr.alpha) * de;
+        if (need1) S1 += (-2.0 * pr.alpha) * de;
+        if constexpr (GRP == 1) S2 += (4.0 * pr.alpha * pr.alpha) * de;
     }
     ShellPows<L, double> P;
     P.init(X, Y, Z);
     constexpr int NC = ncart(L);
-    const OKMI_CONST double* T = cptr(a.T) + sh.t0;
-    for (int j = 0; j < a.ndrv; ++j) {
-        double c[NC];
-        cart_drv<L, double>(P, a.drv[j], S0, S1, S2, c);
-        OutT* o = out + j * a.set_stride + (int64_t)sh.row0 * a.ld + i;
-        auto store = [&](int r, double v) {
-            if (act) o[(int64_t)r * a.ld] = (OutT)v;
-        };
-        // standard spherical / Cartesian shells: sparse compile-time rows;
-        // other transforms (ORCA row factors, custom maps): the dense block
-        if (sh.kind == SHELL_T_SPHERICAL) {
-            sph_rows_to<L, 0>(c, store);
-        } else if (sh.kind == SHELL_T_CARTESIAN) {
-            cart_rows_to<L, 0>(c, store);
-        } else {
-            for (int r = 0; r < sh.nrow; ++r) {
-                double v = 0.0;
+    OutT* o = out + (int64_t)sh.row0 * a.ld + i;
+    double c[NC];
+    if constexpr (GRP == 0) {
+        double m[NC], u[L > 0 ? ncart(L - 1) : 1];
+        first_mono<L, 0, double>(P, m);
+        if (a.dst[0] >= 0) {
 #pragma unroll
-                for (int k = 0; k < NC; ++k) v = fma(T[r * NC + k], c[k], v);
-                store(r, v);
-            }
+            for (int k = 0; k < NC; ++k) c[k] = m[k] * S0;
+            store_rows<L, OutT>(a, sh, c, o + a.dst[0] * a.set_stride, act);
         }
+        if (!need1) return;
+        lower_mono<L, 0, double>(P, S0, u);
+        if (a.dst[1] >= 0) {
+            first_deriv<L, 0, 0, double>(m, u, X * S1, c);
+            store_rows<L, OutT>(a, sh, c, o + a.dst[1] * a.set_stride, act);
+        }
+        if (a.dst[2] >= 0) {
+            first_deriv<L, 1, 0, double>(m, u, Y * S1, c);
+            store_rows<L, OutT>(a, sh, c, o + a.dst[2] * a.set_stride, act);
+        }
+        if (a.dst[3] >= 0) {
+            first_deriv<L, 2, 0, double>(m, u, Z * S1, c);
+            store_rows<L, OutT>(a, sh, c, o + a.dst[3] * a.set_stride, act);
+        }
+    } else {
+#define OKMI_SECOND(D)                                                             \
+        if (a.dst[D] >= 0) {                                                       \
+            cart_all<L, D, double>(P, S0, S1, S2, c);                              \
+            store_rows<L, OutT>(a, sh, c, o + a.dst[D] * a.set_stride, act);      \
+        }
+        OKMI_SECOND(4) OKMI_SECOND(5) OKMI_SECOND(6) OKMI_SECOND(7) OKMI_SECOND(8) OKMI_SECOND(9)
+#undef OKMI_SECOND
     }
 }
 
-template <typename OutT>
-__global__ __launch_bounds__(256) void k_ao_eval(AoArgs a, OutT* __restrict__ out)
+// occupancy (waves/SIMD) the register allocation targets without spilling:
+// FP32 output 4 (group 0) and 3 (group 1), FP64 output 3 and 2
+template <typename OutT, int GRP>
+__global__ __launch_bounds__(256, sizeof(OutT) == 4 ? (GRP == 0 ? 4 : 3) : (GRP == 0 ? 3 : 2))
+void k_ao_eval(AoArgs a, OutT* __restrict__ out)
 {
     int64_t i;
     bool act;
@@ -138,18 +181,19 @@ __global__ __launch_bounds__(256) void k_ao_eval(AoArgs a, OutT* __restrict__ ou
         if (d2 >= sh.rcut2) {
             // every value and derivative of the shell is below 1e-40 here
             if (act)
-                for (int j = 0; j < a.ndrv; ++j) {
-                    OutT* o = out + j * a.set_stride + (int64_t)sh.row0 * a.ld + i;
+                for (int c = 0; c < OKMI_MAX_SETS; ++c) {
+                    if (a.dst[c] < 0) continue;
+                    OutT* o = out + a.dst[c] * a.set_stride + (int64_t)sh.row0 * a.ld + i;
                     for (int r = 0; r < sh.nrow; ++r) o[(int64_t)r * a.ld] = (OutT)0;
                 }
             continue;
         }
         switch (sh.l) {
-        case 0: shell_out<0, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
-        case 1: shell_out<1, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
-        case 2: shell_out<2, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
-        case 3: shell_out<3, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
-        default: shell_out<4, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
+        case 0: shell_out<0, GRP, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
+        case 1: shell_out<1, GRP, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
+        case 2: shell_out<2, GRP, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
+        case 3: shell_out<3, GRP, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
+        default: shell_out<4, GRP, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
         }
     }
 }
@@ -178,25 +222,53 @@ int okmi_launch_ao_eval_ex(const OkmiBasis* b, const ShellRec* shells, const Pri
     a.ld = ld;
     a.set_stride = set_stride;
     a.nshell = b->nshell;
-    a.ndrv = ndrv;
-    a.need1 = a.need2 = 0;
+    // each code's first set; the groups write those, repeats are copies
+    int first[OKMI_MAX_SETS];
+    for (int c = 0; c < OKMI_MAX_SETS; ++c) first[c] = -1;
     for (int j = 0; j < ndrv; ++j) {
-        a.drv[j] = drv[j];
-        if (drv[j] >= 1) a.need1 = 1;
-        if (drv[j] >= 4) a.need2 = 1;
+        if (drv[j] < 0 || drv[j] >= OKMI_MAX_SETS) {
+            okmi_set_error("ao_eval: derivative code %d is not in 0..9", drv[j]);
+            return 1;
+        }
+        if (first[drv[j]] < 0) first[drv[j]] = j;
     }
     dim3 blk(256), grd((unsigned)((np + 255) / 256));
+    int64_t width = np;   // columns written per row
     if (bricks) {
         if (bricks->nbk <= 0) return 0;
         a.brick = 1;
         a.bk0 = bricks->bk0; a.nbk = bricks->nbk;
         a.nbx = bricks->nbx; a.nby = bricks->nby; a.nbz = bricks->nbz;
         grd = dim3((unsigned)((bricks->nbk + 3) / 4));
+        width = bricks->nbk * 64;
     }
     OkmiProfScope prof("ao_eval", st);
-    if (out_dtype == OKMI_F64)
-        hipLaunchKernelGGL(k_ao_eval<double>, grd, blk, 0, st, a, (double*)out);
-    else
-        hipLaunchKernelGGL(k_ao_eval<float>, grd, blk, 0, st, a, (float*)out);
+    for (int grp = 0; grp < 2; ++grp) {
+        const int c0 = grp == 0 ? 0 : 4, c1 = grp == 0 ? 4 : OKMI_MAX_SETS;
+        bool any = false;
+        for (int c = 0; c < OKMI_MAX_SETS; ++c) {
+            a.dst[c] = c >= c0 && c < c1 ? first[c] : -1;
+            any |= a.dst[c] >= 0;
+        }
+        if (!any) continue;
+        a.need1 = first[1] >= 0 || first[2] >= 0 || first[3] >= 0;
+        if (out_dtype == OKMI_F64) {
+            if (grp == 0) hipLaunchKernelGGL((k_ao_eval<double, 0>), grd, blk, 0, st, a, (double*)out);
+            else hipLaunchKernelGGL((k_ao_eval<double, 1>), grd, blk, 0, st, a, (double*)out);
+        } else {
+            if (grp == 0) hipLaunchKernelGGL((k_ao_eval<float, 0>), grd, blk, 0, st, a, (float*)out);
+            else hipLaunchKernelGGL((k_ao_eval<float, 1>), grd, blk, 0, st, a, (float*)out);
+        }
+    }
+    const size_t es = out_dtype == OKMI_F64 ? sizeof(double) : sizeof(float);
+    for (int j = 0; j < ndrv; ++j) {
+        const int f = first[drv[j]];
+        if (f == j) continue;
+        char* base = (char*)out;
+        const hipError_t e = hipMemcpy2DAsync(base + (size_t)j * set_stride * es, (size_t)ld * es,
+                                              base + (size_t)f * set_stride * es, (size_t)ld * es, (size_t)width * es,
+                                              (size_t)b->nao, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return okmi_hip_fail(e, "ao_eval: repeated derivative set");
+    }
     return okmi_check_launch("ao_eval");
 }

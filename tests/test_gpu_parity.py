@@ -153,6 +153,25 @@ def test_mixed_and_mo_derivatives(ok):
   assert ao.shape == rao.shape and rel_err(ao, rao) <= TOL
 
 
+def test_repeated_derivative_codes(ok):
+  '''A derivative listed twice (drv='xx' iterates to ['x', 'x'],
+  orbkit/core.py:401-406; or an explicit list) yields equal sets: the AO
+  kernel writes each code once and the repeats are device copies.'''
+  core, grid, O = ok['core'], ok['grid'], ok['O']
+  qc, axes, _ = ok['W'].make('medium', npts_side=30)
+  x, y, z = _pts(axes, 1200, seed=5)
+  grid.set_grid(x, y, z, is_vector=True)
+  print('case repeated drv', flush=True)
+  for drv in ('xx', ['y', None, 'y', 'xz', 'xz']):
+    ao = core.rho_compute(qc, calc_ao=True, drv=drv)
+    rao = O.rho_compute(qc, x, y, z, True, calc_ao=True, drv=drv)
+    assert ao.shape == rao.shape and rel_err(ao, rao) <= TOL, drv
+    assert numpy.array_equal(ao[0], ao[2 if len(ao) > 2 else 1])
+    rho, d = core.rho_compute(qc, drv=drv)
+    _, rd = O.rho_compute(qc, x, y, z, True, drv=drv)
+    assert rel_err(d, rd) <= TOL, drv
+
+
 def test_mo_creator_and_cart2sph(ok):
   core, O = ok['core'], ok['O']
   qc, axes, _ = ok['W'].make('medium', npts_side=20)
