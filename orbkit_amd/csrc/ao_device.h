@@ -139,6 +139,24 @@ __device__ __forceinline__ void first_deriv(const R* m, const R* u, R RS1, R* ou
     }
 }
 
+// sum_c w_c e_a(c) u_{c - 1_a} for axis A, u the degree-(L-1) monomials
+// (times S0): the lowered-exponent half of sum_c w_c d_a chi_c
+template <int L, int A, int C, typename R>
+__device__ __forceinline__ R lowered_dot(const R* w, const R* u, R acc)
+{
+    if constexpr (C < ncart(L)) {
+        constexpr int e0 = cart_exp(L, C, 0), e1 = cart_exp(L, C, 1), e2 = cart_exp(L, C, 2);
+        constexpr int ea = A == 0 ? e0 : A == 1 ? e1 : e2;
+        if constexpr (ea > 0) {
+            constexpr int lo = cart_index(L - 1, e0 - (A == 0), e1 - (A == 1), e2 - (A == 2));
+            acc = fma(w[C] * R(ea), u[lo], acc);
+        }
+        return lowered_dot<L, A, C + 1, R>(w, u, acc);
+    } else {
+        return acc;
+    }
+}
+
 // runtime (wave-uniform) derivative code -> compile-time evaluation
 template <int L, typename R>
 __device__ __forceinline__ void cart_drv(const ShellPows<L, R>& P, int d, R S0, R S1, R S2, R* out)

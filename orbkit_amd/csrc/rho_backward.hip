@@ -121,22 +121,30 @@ struct ShellGradArgs {
     int64_t ncol;
 };
 
-// per-lane contributions of one point to the shell's position gradient and
-// to the exponent gradients of primitives [pw0, pw0 + MAXP)
-template <int L>
+// Per-lane contributions of one point to the shell's position gradient and
+// to the exponent gradients of primitives [pw0, pw0 + MP), accumulated into
+// gr / ga when `near` (the lane's point is within the shell's radius).
+// Every primitive's d e is computed once (kept for those of the window).
+template <int L, int MP>
 __device__ __forceinline__ void shell_point(const ShellGradArgs& a, const ShellRec& sh, int pw0, int pend,
-                                            double px, double py, double pz, int64_t i, double nfac,
-                                            double (&gr)[3], double (&ga)[MAXP])
+                                            double px, double py, double pz, int64_t i, double nfac, bool near,
+                                            double (&gr)[3], double (&ga)[MP])
 {
     constexpr int NC = ncart(L);
     const double X = px - sh.cx, Y = py - sh.cy, Z = pz - sh.cz;
     const double rr = X * X + Y * Y + Z * Z;
     double S0 = 0.0, S1 = 0.0;
+    double dw[MP];
+#pragma unroll
+    for (int j = 0; j < MP; ++j) dw[j] = 0.0;
     for (int p = 0; p < sh.nprim; ++p) {
         const PrimRec pr = load_prim(a.prims, sh.prim0 + p);
         const double de = pr.d * gauss_f64(pr.alpha * rr);
         S0 += de;
         S1 = fma(-2.0 * pr.alpha, de, S1);
+#pragma unroll
+        for (int j = 0; j < MP; ++j)
+            if (p == pw0 + j) dw[j] = de;
     }
     // w_c = sum_rows G_row T[row][c]
     const OKMI_CONST double* T = cptr(a.T) + sh.t0;
@@ -148,47 +156,43 @@ __device__ __forceinline__ void shell_point(const ShellGradArgs& a, const ShellR
 #pragma unroll
         for (int k = 0; k < NC; ++k) w[k] = fma(g, T[rw * NC + k], w[k]);
     }
+    // q = sum_c w_c m_c and, with the lowered monomials u (times S0),
+    // sum_c w_c d_a chi_c = R_a S1 q + sum_c w_c e_a(c) u_{c - 1_a}
     ShellPows<L, double> P;
     P.init(X, Y, Z);
-    double c[NC];
+    double m[NC];
+    first_mono<L, 0, double>(P, m);
     double q = 0.0;
-    cart_all<L, 0, double>(P, 1.0, 0.0, 0.0, c);
 #pragma unroll
-    for (int k = 0; k < NC; ++k) q = fma(w[k], c[k], q);
-    if (pw0 == 0) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            double v = 0.0;
-            if (d == 0) cart_all<L, 1, double>(P, S0, S1, 0.0, c);
-            else if (d == 1) cart_all<L, 2, double>(P, S0, S1, 0.0, c);
-            else cart_all<L, 3, double>(P, S0, S1, 0.0, c);
-#pragma unroll
-            for (int k = 0; k < NC; ++k) v = fma(w[k], c[k], v);
-            gr[d] -= v;
-        }
+    for (int k = 0; k < NC; ++k) q = fma(w[k], m[k], q);
+    if (pw0 == 0 && near) {
+        double u[L > 0 ? ncart(L - 1) : 1];
+        lower_mono<L, 0, double>(P, S0, u);
+        gr[0] -= fma(X * S1, q, lowered_dot<L, 0, 0, double>(w, u, 0.0));
+        gr[1] -= fma(Y * S1, q, lowered_dot<L, 1, 0, double>(w, u, 0.0));
+        gr[2] -= fma(Z * S1, q, lowered_dot<L, 2, 0, double>(w, u, 0.0));
     }
+    if (!near) q = 0.0;
 #pragma unroll
-    for (int j = 0; j < MAXP; ++j) {
-        const int p = pw0 + j;
-        if (p < pend) {
-            const PrimRec pr = load_prim(a.prims, sh.prim0 + p);
-            const double de = pr.d * gauss_f64(pr.alpha * rr);
-            ga[j] = fma(q * de, nfac / pr.alpha - rr, ga[j]);
+    for (int j = 0; j < MP; ++j) {
+        if (pw0 + j < pend) {
+            const double alpha = load_prim(a.prims, sh.prim0 + pw0 + j).alpha;
+            ga[j] = fma(q * dw[j], nfac / alpha - rr, ga[j]);
         }
     }
 }
 
-template <int L>
+template <int L, int MP>
 __device__ void shell_grad(const ShellGradArgs& a, const ShellRec& sh, int pw0)
 {
-    __shared__ double red[4][3 + MAXP];
+    __shared__ double red[4][3 + MP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int pend = min(sh.nprim, pw0 + MAXP);
+    const int pend = min(sh.nprim, pw0 + MP);
     const double nfac = a.is_normalized ? 0.0 : (2.0 * L + 3.0) / 4.0;
     double gr[3] = {0.0, 0.0, 0.0};
-    double ga[MAXP];
+    double ga[MP];
 #pragma unroll
-    for (int j = 0; j < MAXP; ++j) ga[j] = 0.0;
+    for (int j = 0; j < MP; ++j) ga[j] = 0.0;
     const int64_t r0 = (int64_t)blockIdx.x * RANGE;
     const int64_t r1 = min(a.np, r0 + RANGE);
     for (int64_t b = r0; b < r1; b += 256) {
@@ -199,17 +203,7 @@ __device__ void shell_grad(const ShellGradArgs& a, const ShellRec& sh, int pw0)
         const double X = px - sh.cx, Y = py - sh.cy, Z = pz - sh.cz;
         const bool near = act && X * X + Y * Y + Z * Z < sh.rcut2;
         if (!__builtin_amdgcn_ballot_w64(near)) continue;   // no point of this wave reaches the shell
-        double lr[3] = {0.0, 0.0, 0.0};
-        double la[MAXP];
-#pragma unroll
-        for (int j = 0; j < MAXP; ++j) la[j] = 0.0;
-        shell_point<L>(a, sh, pw0, pend, px, py, pz, act ? i : r0, nfac, lr, la);
-        if (near) {
-#pragma unroll
-            for (int d = 0; d < 3; ++d) gr[d] += lr[d];
-#pragma unroll
-            for (int j = 0; j < MAXP; ++j) ga[j] += la[j];
-        }
+        shell_point<L, MP>(a, sh, pw0, pend, px, py, pz, act ? i : r0, nfac, near, gr, ga);
     }
     // block reduction in a fixed order (deterministic)
 #pragma unroll
@@ -218,7 +212,7 @@ __device__ void shell_grad(const ShellGradArgs& a, const ShellRec& sh, int pw0)
         if (lane == 0) red[wave][d] = v;
     }
 #pragma unroll
-    for (int j = 0; j < MAXP; ++j) {
+    for (int j = 0; j < MP; ++j) {
         if (pw0 + j < pend) {
             const double v = wave_sum(ga[j]);
             if (lane == 0) red[wave][3 + j] = v;
@@ -229,22 +223,25 @@ __device__ void shell_grad(const ShellGradArgs& a, const ShellRec& sh, int pw0)
     double* out = a.partial + (int64_t)blockIdx.x * a.ncol;
     if (t < 3 && pw0 == 0) {
         out[a.nprim_total + 3 * blockIdx.y + t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
-    } else if (t >= 3 && t < 3 + MAXP && pw0 + t - 3 < pend) {
+    } else if (t >= 3 && t < 3 + MP && pw0 + t - 3 < pend) {
         out[sh.prim0 + pw0 + t - 3] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
     }
 }
 
+// MP primitives per block (4 when no shell has more, else MAXP; more
+// primitives than MP: extra windows along grid z)
+template <int MP>
 __global__ __launch_bounds__(256) void k_shell_grad(ShellGradArgs a)
 {
     const ShellRec sh = load_shell(a.shells, blockIdx.y);
-    const int pw0 = blockIdx.z * MAXP;
+    const int pw0 = blockIdx.z * MP;
     if (pw0 >= sh.nprim) return;
     switch (sh.l) {
-    case 0: shell_grad<0>(a, sh, pw0); break;
-    case 1: shell_grad<1>(a, sh, pw0); break;
-    case 2: shell_grad<2>(a, sh, pw0); break;
-    case 3: shell_grad<3>(a, sh, pw0); break;
-    default: shell_grad<4>(a, sh, pw0); break;
+    case 0: shell_grad<0, MP>(a, sh, pw0); break;
+    case 1: shell_grad<1, MP>(a, sh, pw0); break;
+    case 2: shell_grad<2, MP>(a, sh, pw0); break;
+    case 3: shell_grad<3, MP>(a, sh, pw0); break;
+    default: shell_grad<4, MP>(a, sh, pw0); break;
     }
 }
 
@@ -389,9 +386,13 @@ int okmi_launch_rho_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, 
         const int nrange = (int)((n + RANGE - 1) / RANGE);
         {
             OkmiProfScope p3("shell_grad", st);
-            hipLaunchKernelGGL(k_shell_grad,
-                               dim3((unsigned)nrange, (unsigned)b->nshell, (unsigned)((maxp + MAXP - 1) / MAXP)),
-                               dim3(256), 0, st, sa);
+            if (maxp <= 4)
+                hipLaunchKernelGGL(k_shell_grad<4>, dim3((unsigned)nrange, (unsigned)b->nshell, 1u), dim3(256), 0, st,
+                                   sa);
+            else
+                hipLaunchKernelGGL(k_shell_grad<MAXP>,
+                                   dim3((unsigned)nrange, (unsigned)b->nshell, (unsigned)((maxp + MAXP - 1) / MAXP)),
+                                   dim3(256), 0, st, sa);
         }
         hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)((Lo.ncol + 255) / 256)), dim3(256), 0, st, part, nrange,
                            Lo.ncol, cols);
