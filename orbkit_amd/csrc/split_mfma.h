@@ -18,13 +18,16 @@ struct Split {
 
 typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
 
-// bf16 pair from its bit pattern, kept in a scalar register.  The compiler
-// would otherwise encode (-1, 0) as the 16-bit inline constant -1.0, which
-// the dot product reads as the FP16 number -1.0 (bit pattern 0xbc00, the
-// bf16 number -2^-7): residuals formed that way were wrong on gfx950.
+// bf16 pair from its bit pattern, passed through an (empty, side-effect
+// free) asm statement so that it reaches the dot product in a register.  As
+// a compile-time constant the compiler would encode (-1, 0) as the 16-bit
+// inline constant -1.0, which the dot product reads as the FP16 number -1.0
+// (bit pattern 0xbc00, the bf16 number -2^-7): residuals formed that way were
+// wrong on gfx950.  Not volatile, so it is hoisted and does not constrain
+// instruction scheduling.
 __device__ __forceinline__ bf2 pair_bits(unsigned bits)
 {
-    asm volatile("" : "+s"(bits));
+    asm("" : "+s"(bits));
     return __builtin_bit_cast(bf2, bits);
 }
 
