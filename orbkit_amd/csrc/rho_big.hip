@@ -7,10 +7,10 @@
 //      128 MOs x NP points x all S sets; the K loop (AO rows, 32 per step)
 //      stages the C tile (pre-split bf16 planes or FP32) and the AO tile
 //      (FP32) in LDS, double buffered with one barrier per step.  Each wave
-//      holds 64 MOs x NP/2 points x S sets of FP32 accumulators, so the
-//      density epilogue (all sets of one MO and point in one lane) runs in
-//      FP64 without leaving registers; the 128-MO partial sums go to
-//      partial[mo tile][code][p];
+//      holds 64 MOs x NPW 16-point columns x S sets of FP32 accumulators
+//      (BigCfg), so the density epilogue (all sets of one MO and point in one
+//      lane) runs in FP64 without leaving registers; the 128-MO partial sums
+//      go to partial[mo tile][code][p];
 //   3. k_reduce_big sums the MO tiles in a fixed order and writes rho and
 //      orbkit's derivative rows.
 // S is 1 (rho), 4 (+ first derivatives), 7 (+ diagonal second derivatives)
@@ -26,7 +26,17 @@ using namespace okmi_split;
 constexpr int MB = 128;   // MOs per workgroup
 constexpr int KS = 32;    // AO rows per K step
 
-template <int S> struct BigCfg { static constexpr int NPW = S == 1 ? 4 : S == 4 ? 2 : 1; };
+// Workgroup shape per set count: NP points, NW waves in 2 (MO halves) x WN
+// (point columns) with NPW 16-point columns each.  Up to 4 sets a wave holds
+// 64 MOs x 16 points (S = 4) so that two waves fit per SIMD (8 waves share
+// the LDS tile); with 7 or 10 sets four waves of 64 MOs x 16 points.
+template <int S> struct BigCfg {
+    static constexpr int NW = S <= 4 ? 8 : 4;
+    static constexpr int WN = NW / 2;
+    static constexpr int NP = S == 1 ? 128 : S == 4 ? 64 : 32;
+    static constexpr int NPW = NP / 16 / WN;
+    static constexpr int NT = 64 * NW;
+};
 
 struct BigArgs {
     const float* ao;       // [S][nao][ld]
@@ -54,7 +64,7 @@ constexpr int CLD_F = KS + 4;     // floats per C row in LDS (144 bytes)
 
 template <int PREC, int S>
 struct BigLds {
-    static constexpr int NP = 32 * BigCfg<S>::NPW;
+    static constexpr int NP = BigCfg<S>::NP;
     static constexpr int AOLD = AoLd<PREC, NP>::v;
     static constexpr int AO_FLOATS = S * KS * AOLD;
     static constexpr int C_BYTES = PREC == OKMI_PREC_F32 ? MB * CLD_F * 4 : Prec<PREC>::planes * MB * CLD_BF * 2;
@@ -72,10 +82,10 @@ __device__ __forceinline__ int64_t logical_block(int64_t b, int64_t nb)
 }
 
 template <int PREC, int S>
-__global__ __launch_bounds__(256, 1) void k_rho_big(BigArgs a)
+__global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
 {
     using LD = BigLds<PREC, S>;
-    constexpr int NPW = BigCfg<S>::NPW, NP = LD::NP, AOLD = LD::AOLD;
+    constexpr int NPW = BigCfg<S>::NPW, NP = LD::NP, AOLD = LD::AOLD, NT = BigCfg<S>::NT;
     constexpr int NPL = Prec<PREC>::planes;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
 
@@ -90,9 +100,10 @@ __global__ __launch_bounds__(256, 1) void k_rho_big(BigArgs a)
     const int nk = (a.nao + KS - 1) / KS;
 
     // ---- global -> register staging
-    constexpr int AO_V4 = S * KS * NP / 4 / 256;          // float4 per thread per step
-    static_assert(S * KS * NP / 4 % 256 == 0, "AO tile must split evenly over 256 threads");
-    constexpr int C_V4 = PREC == OKMI_PREC_F32 ? MB * KS / 4 / 256 : NPL * MB * KS / 8 / 256;   // 16-byte pieces
+    constexpr int AO_V4 = S * KS * NP / 4 / NT;           // float4 per thread per step
+    static_assert(S * KS * NP / 4 % NT == 0, "AO tile must split evenly over the threads");
+    constexpr int C_V4 = PREC == OKMI_PREC_F32 ? MB * KS / 4 / NT : NPL * MB * KS / 8 / NT;   // 16-byte pieces
+    static_assert((PREC == OKMI_PREC_F32 ? MB * KS / 4 : NPL * MB * KS / 8) % NT == 0, "C tile split");
     typedef float fl4 __attribute__((ext_vector_type(4)));
     fl4 rao[AO_V4];
     fl4 rc[C_V4];
@@ -100,7 +111,7 @@ __global__ __launch_bounds__(256, 1) void k_rho_big(BigArgs a)
         const int k0 = ks * KS;
 #pragma unroll
         for (int v = 0; v < AO_V4; ++v) {
-            const int e = v * 256 + threadIdx.x;          // float4 index in [S][KS][NP/4]
+            const int e = v * NT + threadIdx.x;          // float4 index in [S][KS][NP/4]
             const int pc = e % (NP / 4), r = (e / (NP / 4)) % KS, s = e / (NP / 4 * KS);
             const int k = k0 + r;
             if (k < a.nao)
@@ -110,7 +121,7 @@ __global__ __launch_bounds__(256, 1) void k_rho_big(BigArgs a)
         }
 #pragma unroll
         for (int v = 0; v < C_V4; ++v) {
-            const int e = v * 256 + threadIdx.x;
+            const int e = v * NT + threadIdx.x;
             if constexpr (PREC == OKMI_PREC_F32) {
                 const int kc = e % (KS / 4), r = e / (KS / 4);
                 rc[v] = *(const fl4*)((const float*)a.C + (int64_t)(m0 + r) * a.kpad + k0 + 4 * kc);
@@ -126,7 +137,7 @@ __global__ __launch_bounds__(256, 1) void k_rho_big(BigArgs a)
         unsigned char* cl = smem + buf * LD::STAGE + LD::AO_FLOATS * 4;
 #pragma unroll
         for (int v = 0; v < AO_V4; ++v) {
-            const int e = v * 256 + threadIdx.x;
+            const int e = v * NT + threadIdx.x;
             const int pc = e % (NP / 4), r = (e / (NP / 4)) % KS, s = e / (NP / 4 * KS);
             float* d = ao + (s * KS + r) * AOLD + 4 * pc;   // 8-byte aligned (AOLD even)
             *(float2*)d = float2{rao[v][0], rao[v][1]};
@@ -134,7 +145,7 @@ __global__ __launch_bounds__(256, 1) void k_rho_big(BigArgs a)
         }
 #pragma unroll
         for (int v = 0; v < C_V4; ++v) {
-            const int e = v * 256 + threadIdx.x;
+            const int e = v * NT + threadIdx.x;
             if constexpr (PREC == OKMI_PREC_F32) {
                 const int kc = e % (KS / 4), r = e / (KS / 4);
                 *(fl4*)((float*)cl + r * CLD_F + 4 * kc) = rc[v];
@@ -241,7 +252,7 @@ __global__ __launch_bounds__(256, 1) void k_rho_big(BigArgs a)
         }
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < S * NP; e += 256) {
+    for (int e = threadIdx.x; e < S * NP; e += NT) {
         const int s = e / NP, pt = e % NP;
         const int64_t p = p0 + pt;
         if (p < a.np)
@@ -312,7 +323,7 @@ int launch_big_s(const BigArgs& a, hipStream_t st)
         attr_set = true;
     }
     OkmiProfScope prof("rho_big", st);
-    hipLaunchKernelGGL((k_rho_big<PREC, S>), dim3((unsigned)a.nblocks), dim3(256), LD::TOTAL, st, a);
+    hipLaunchKernelGGL((k_rho_big<PREC, S>), dim3((unsigned)a.nblocks), dim3(BigCfg<S>::NT), LD::TOTAL, st, a);
     return 0;
 }
 
