@@ -69,16 +69,26 @@ template <> struct PassCodes<1> { static constexpr int n = 4; static constexpr i
 template <> struct PassCodes<2> { static constexpr int n = 4; static constexpr int c[4] = {0, 4, 5, 6}; };
 template <> struct PassCodes<3> { static constexpr int n = 4; static constexpr int c[4] = {0, 7, 8, 9}; };
 
+// brick coordinates of a tile (32-bit arithmetic: the host keeps the brick
+// count below 2^32)
+__device__ __forceinline__ void brick_of(const FusedArgs& a, int64_t tile, int64_t& bx, int64_t& by, int64_t& bz)
+{
+    unsigned t = (unsigned)tile;
+    const unsigned nbz = (unsigned)a.nbz, nby = (unsigned)a.nby;
+    bz = t % nbz;
+    t /= nbz;
+    by = t % nby;
+    bx = t / nby;
+}
+
 // point of lane `ln` of this block's tile: index relative to p0, validity,
 // and the (clamped) coordinates
 __device__ __forceinline__ void tile_point(const FusedArgs& a, int64_t tile, int ln, int64_t& p, bool& act, double& px,
                                            double& py, double& pz)
 {
     if (a.brick) {
-        int64_t t = tile;
-        const int64_t bz = t % a.nbz;
-        t /= a.nbz;
-        const int64_t by = t % a.nby, bx = t / a.nby;
+        int64_t bx, by, bz;
+        brick_of(a, tile, bx, by, bz);
         int64_t ix = (bx << a.tlx) + (ln >> (a.tly + a.tlz));
         int64_t iy = (by << a.tly) + ((ln >> a.tlz) & ((1 << a.tly) - 1));
         int64_t iz = (bz << a.tlz) + (ln & ((1 << a.tlz) - 1));
@@ -417,9 +427,36 @@ __device__ __forceinline__ void epilogue(const FusedArgs& a, int64_t tile, int c
     }
 }
 
-__device__ __forceinline__ Box tile_box(double px, double py, double pz)
+// (min, max) of the axis values of brick b along one axis (2^tl values,
+// fewer at the grid's end), wave-uniform
+__device__ __forceinline__ double2 axis_span(const double* ax, int64_t b, int tl, int64_t n)
+{
+    const int64_t i0 = b << tl, i1 = min(i0 + (int64_t(1) << tl), n);
+    double lo = ax[i0], hi = lo;
+    for (int64_t i = i0 + 1; i < i1; ++i) {
+        lo = fmin(lo, ax[i]);
+        hi = fmax(hi, ax[i]);
+    }
+    return double2{uniform(lo), uniform(hi)};
+}
+
+// Bounding box of the tile's points: for a brick of a regular grid from the
+// (at most 2^tl) axis values it spans, otherwise by wave reductions over the
+// lanes' points (px, py, pz: lane l holds point l of the tile).
+__device__ __forceinline__ Box tile_box(const FusedArgs& a, int64_t tile, double px, double py, double pz)
 {
     Box box;
+    if (a.brick) {
+        int64_t bx, by, bz;
+        brick_of(a, tile, bx, by, bz);
+        const double2 sx = axis_span(a.grid.x, bx, a.tlx, a.grid.nx);
+        const double2 sy = axis_span(a.grid.y, by, a.tly, a.grid.ny);
+        const double2 sz = axis_span(a.grid.z, bz, a.tlz, a.grid.nz);
+        box.lo[0] = sx.x; box.hi[0] = sx.y;
+        box.lo[1] = sy.x; box.hi[1] = sy.y;
+        box.lo[2] = sz.x; box.hi[2] = sz.y;
+        return box;
+    }
     box.lo[0] = wave_min(px); box.hi[0] = wave_max(px);
     box.lo[1] = wave_min(py); box.hi[1] = wave_max(py);
     box.lo[2] = wave_min(pz); box.hi[2] = wave_max(pz);
@@ -443,7 +480,7 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float
     bool pact;
     double px, py, pz;
     tile_point(a, tile, lane, pidx, pact, px, py, pz);   // lane l of every wave: point l of the tile
-    const Box box = tile_box(px, py, pz);
+    const Box box = tile_box(a, tile, px, py, pz);
 
     f4 acc[MT][NS];
 #pragma unroll
@@ -520,6 +557,10 @@ template <int PREC, int PASS>
 int launch_mt(const FusedArgs& a, int mt, hipStream_t st)
 {
     const int64_t ntile = a.brick ? a.nbx * a.nby * a.nbz : (a.np + PTS - 1) / PTS;
+    if (ntile >= ((int64_t)1 << 31)) {   // one workgroup per tile; brick indices are 32-bit
+        okmi_set_error("rho_fused: %lld tiles of 64 points (at most 2^31 - 1 per call)", (long long)ntile);
+        return 1;
+    }
     dim3 blk(256), grd((unsigned)ntile);
     OkmiProfScope prof("rho_fused", st);
     switch (mt) {
@@ -922,6 +963,10 @@ static int batch_set_shape(OkmiBatch* B, int lx, int ly, int lz)
             a.nby = (a.grid.ny + (1 << ly) - 1) >> ly;
             a.nbz = (a.grid.nz + (1 << lz) - 1) >> lz;
             tiles = a.nbx * a.nby * a.nbz;
+            if (tiles >= ((int64_t)1 << 31)) {   // brick indices are 32-bit
+                okmi_set_error("batch: molecule %d has %lld tiles (at most 2^31 - 1)", j, (long long)tiles);
+                return 1;
+            }
         } else {
             tiles = (a.np + PTS - 1) / PTS;
         }
