@@ -7,8 +7,9 @@
 // For each chunk the four waves take the chunk's shells one at a time from a
 // shared counter, costliest first (lane = point), and evaluate them in FP64
 // (coordinates, exponentials, polynomials, the Cartesian-to-spherical map),
-// storing each value rounded once to FP32 in LDS [row][point] (double
-// buffered, one barrier per chunk).  Each wave then multiplies the rows of its
+// storing each value rounded once to FP32 in LDS [row][point] (one buffer
+// and three workgroups per CU for rho / rho + grad rho with l <= 3, else
+// double buffered with one barrier per chunk; see FusedPlan).  Each wave then multiplies the rows of its
 // 16-point column with the matching 32 columns of C:
 //   F32     v_mfma_f32_16x16x4_f32, C and AO in FP32
 //   BF16X3  v_mfma_f32_16x16x32_bf16 on (hi, lo) splits, 3 products
@@ -294,7 +295,7 @@ __device__ __forceinline__ void load_c(const FusedArgs& a, int k0, int g, int n1
 // counter *next (costliest first), so that screening cannot leave one wave
 // with most of the work; a wave that evaluates a shell sets *live = tag.
 // Waves with `pad` set also clear the rows after the chunk's last shell.
-template <int PASS>
+template <int PASS, int LM>
 __device__ __forceinline__ void eval_chunk(const FusedArgs& a, int ch, const Box& box, double px, double py, double pz,
                                            float* L, int* next, int* live, int tag, bool pad, int lane)
 {
@@ -320,7 +321,9 @@ __device__ __forceinline__ void eval_chunk(const FusedArgs& a, int ch, const Box
             case 1: eval_shell_chunk<1, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
             case 2: eval_shell_chunk<2, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
             case 3: eval_shell_chunk<3, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
-            default: eval_shell_chunk<4, PASS>(a, sh, d2, px, py, pz, k0, L, lane); break;
+            default:
+                if constexpr (LM >= 4) eval_shell_chunk<4, PASS>(a, sh, d2, px, py, pz, k0, L, lane);
+                break;
             }
         } else {
             for (int s = 0; s < NS; ++s)
@@ -463,11 +466,23 @@ __device__ __forceinline__ Box tile_box(const FusedArgs& a, int64_t tile, double
     return box;
 }
 
+// Whether an instance uses one LDS buffer and three workgroups per CU: rho
+// and rho + grad rho with l <= 3 and up to 48 MOs (32 in the batch kernel,
+// which needs a few more registers) fit in 168 VGPRs without spilling (33 KB
+// LDS) and run at 3 waves/SIMD; the others keep two buffers (66.5 KB,
+// 2 waves/SIMD), where holding C through the evaluation hides its latency.
+template <int PASS, int LM, int MT, bool BATCH> struct FusedPlan {
+    static constexpr bool single = PASS <= 1 && LM <= 3 && MT <= (BATCH ? 2 : 3);
+    static constexpr int buffers = single ? 1 : 2;
+    static constexpr int occupancy = single ? 3 : 2;
+};
+
 // One tile of 64 points by four waves that all evaluate and all contract:
 // every thread of the workgroup calls this with the same tile.  `lds` holds
-// two AO buffers, `next` two work counters, `live` two chunk tags.  The first
-// barrier orders it after any previous tile's LDS reads.
-template <int PREC, int PASS, int MT>
+// FusedPlan::buffers AO buffers, `next` two work counters, `live` two chunk
+// tags.  The first barrier orders it after any previous tile's LDS reads.
+// LM: highest angular momentum of the basis (3 or 4).
+template <int PREC, int PASS, int MT, int LM, bool BATCH>
 __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float* lds, int* next, int* live)
 {
     constexpr int NS = PassCodes<PASS>::n;
@@ -488,10 +503,28 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float
 #pragma unroll
         for (int s = 0; s < NS; ++s) acc[mt][s] = f4{0.f, 0.f, 0.f, 0.f};
 
-    // The C operand is loaded before the AO evaluation (its latency hidden
-    // there) unless holding it through the evaluation would exceed the
-    // register budget: second-derivative passes and 4 MO tiles load it
-    // after the barrier.
+    if constexpr (FusedPlan<PASS, LM, MT, BATCH>::single) {
+        // one buffer: evaluate, barrier, contract (C loaded now), barrier
+        for (int ch = 0; ch < a.nchunk; ++ch) {
+            eval_chunk<PASS, LM>(a, ch, box, px, py, pz, lds, &next[0], &live[0], ch, wave == 0, lane);
+            __syncthreads();
+            // every wave finished taking from the counter before the barrier
+            // and takes from it again after the next one
+            if (threadIdx.x == 0) next[0] = 0;
+            if (live[0] == ch) {
+                COperand<PREC, MT> c;
+                load_c<PREC, MT>(a, ch * KC, g, n16, c);
+                contract<PREC, PASS, MT>(lds, wave, g, n16, c, acc);
+            }
+            __syncthreads();
+        }
+        epilogue<PREC, PASS, MT>(a, tile, wave, lane, acc);
+        return;
+    }
+    // Two buffers.  The C operand is loaded before the AO evaluation (its
+    // latency hidden there) unless holding it through the evaluation would
+    // exceed the register budget: second-derivative passes and 4 MO tiles
+    // load it after the barrier.
     constexpr bool c_early = PASS <= 1 ? MT <= 3 : MT <= 2;
     int buf = 0;
     for (int ch = 0; ch < a.nchunk; ++ch) {
@@ -501,7 +534,7 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float
         // the other buffer's counter: every wave finished taking from it
         // before the previous barrier, the next chunk reads it after the next
         if (threadIdx.x == 0) next[buf ^ 1] = 0;
-        eval_chunk<PASS>(a, ch, box, px, py, pz, L, &next[buf], &live[buf], ch, wave == 0, lane);
+        eval_chunk<PASS, LM>(a, ch, box, px, py, pz, L, &next[buf], &live[buf], ch, wave == 0, lane);
         // All rows of buffer `buf` written; the other buffer is free because
         // every wave has passed the previous barrier after reading it.
         __syncthreads();
@@ -519,13 +552,13 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float
     epilogue<PREC, PASS, MT>(a, tile, wave, lane, acc);
 }
 
-template <int PREC, int PASS, int MT>
-__global__ __launch_bounds__(256, 2) void k_rho_fused(FusedArgs a)
+template <int PREC, int PASS, int MT, int LM>
+__global__ __launch_bounds__(256, (FusedPlan<PASS, LM, MT, false>::occupancy)) void k_rho_fused(FusedArgs a)
 {
-    __shared__ float lds[2 * FusedLds<PASS>::BUF];
+    __shared__ float lds[FusedPlan<PASS, LM, MT, false>::buffers * FusedLds<PASS>::BUF];
     __shared__ int next[2];   // per buffer: next work entry to evaluate
     __shared__ int live[2];   // per buffer: index of the last chunk with a live shell
-    rho_tile<PREC, PASS, MT>(a, blockIdx.x, lds, next, live);
+    rho_tile<PREC, PASS, MT, LM, false>(a, blockIdx.x, lds, next, live);
 }
 
 struct GatherCodes {
@@ -553,7 +586,7 @@ __global__ void k_pad_occ(const double* occ, int nmo, int mpad, double* out)
     if (i < mpad) out[i] = i < nmo ? occ[i] : 0.0;
 }
 
-template <int PREC, int PASS>
+template <int PREC, int PASS, int LM>
 int launch_mt(const FusedArgs& a, int mt, hipStream_t st)
 {
     const int64_t ntile = a.brick ? a.nbx * a.nby * a.nbz : (a.np + PTS - 1) / PTS;
@@ -564,24 +597,28 @@ int launch_mt(const FusedArgs& a, int mt, hipStream_t st)
     dim3 blk(256), grd((unsigned)ntile);
     OkmiProfScope prof("rho_fused", st);
     switch (mt) {
-    case 1: hipLaunchKernelGGL((k_rho_fused<PREC, PASS, 1>), grd, blk, 0, st, a); return 0;
-    case 2: hipLaunchKernelGGL((k_rho_fused<PREC, PASS, 2>), grd, blk, 0, st, a); return 0;
-    case 3: hipLaunchKernelGGL((k_rho_fused<PREC, PASS, 3>), grd, blk, 0, st, a); return 0;
-    case 4: hipLaunchKernelGGL((k_rho_fused<PREC, PASS, 4>), grd, blk, 0, st, a); return 0;
+    case 1: hipLaunchKernelGGL((k_rho_fused<PREC, PASS, 1, LM>), grd, blk, 0, st, a); return 0;
+    case 2: hipLaunchKernelGGL((k_rho_fused<PREC, PASS, 2, LM>), grd, blk, 0, st, a); return 0;
+    case 3: hipLaunchKernelGGL((k_rho_fused<PREC, PASS, 3, LM>), grd, blk, 0, st, a); return 0;
+    case 4: hipLaunchKernelGGL((k_rho_fused<PREC, PASS, 4, LM>), grd, blk, 0, st, a); return 0;
     default: break;
     }
     okmi_set_error("rho_fused: %d MO tiles (up to 4 supported)", mt);
     return 1;
 }
 
+// lm: highest angular momentum of the basis (l <= 3 selects the l <= 3
+// instances; only rho and rho + grad rho have them, the second-derivative
+// passes use the general ones)
 template <int PREC>
-int launch_pass(const FusedArgs& a, int pass, int mt, hipStream_t st)
+int launch_pass(const FusedArgs& a, int pass, int mt, int lm, hipStream_t st)
 {
+    const bool l3 = lm <= 3;
     switch (pass) {
-    case 0: return launch_mt<PREC, 0>(a, mt, st);
-    case 1: return launch_mt<PREC, 1>(a, mt, st);
-    case 2: return launch_mt<PREC, 2>(a, mt, st);
-    default: return launch_mt<PREC, 3>(a, mt, st);
+    case 0: return l3 ? launch_mt<PREC, 0, 3>(a, mt, st) : launch_mt<PREC, 0, 4>(a, mt, st);
+    case 1: return l3 ? launch_mt<PREC, 1, 3>(a, mt, st) : launch_mt<PREC, 1, 4>(a, mt, st);
+    case 2: return launch_mt<PREC, 2, 4>(a, mt, st);
+    default: return launch_mt<PREC, 3, 4>(a, mt, st);
     }
 }
 
@@ -763,6 +800,8 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
         gc.c[r] = drv_out[r];
     }
     const bool p1 = (need & ~1) != 0, p2 = (need & (7 << 4)) != 0, p3 = (need & (7 << 7)) != 0;
+    int lm = 0;   // highest angular momentum of the basis
+    for (int i = 0; i < b->nshell; ++i) lm = std::max(lm, b->h_shells[i].l);
 
     FusedArgs a{};
     a.shells = b->d_fshells; a.groups = b->d_fgroups; a.T = b->d_T; a.chunks = b->d_chunks;
@@ -779,17 +818,17 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
         a.rho = rho + c0;
         int r;
         if (prec == OKMI_PREC_F32) {
-            r = launch_pass<OKMI_PREC_F32>(a, p1 ? 1 : 0, mt, st);
-            if (!r && p2) r = launch_pass<OKMI_PREC_F32>(a, 2, mt, st);
-            if (!r && p3) r = launch_pass<OKMI_PREC_F32>(a, 3, mt, st);
+            r = launch_pass<OKMI_PREC_F32>(a, p1 ? 1 : 0, mt, lm, st);
+            if (!r && p2) r = launch_pass<OKMI_PREC_F32>(a, 2, mt, lm, st);
+            if (!r && p3) r = launch_pass<OKMI_PREC_F32>(a, 3, mt, lm, st);
         } else if (prec == OKMI_PREC_BF16X3) {
-            r = launch_pass<OKMI_PREC_BF16X3>(a, p1 ? 1 : 0, mt, st);
-            if (!r && p2) r = launch_pass<OKMI_PREC_BF16X3>(a, 2, mt, st);
-            if (!r && p3) r = launch_pass<OKMI_PREC_BF16X3>(a, 3, mt, st);
+            r = launch_pass<OKMI_PREC_BF16X3>(a, p1 ? 1 : 0, mt, lm, st);
+            if (!r && p2) r = launch_pass<OKMI_PREC_BF16X3>(a, 2, mt, lm, st);
+            if (!r && p3) r = launch_pass<OKMI_PREC_BF16X3>(a, 3, mt, lm, st);
         } else {
-            r = launch_pass<OKMI_PREC_BF16X6>(a, p1 ? 1 : 0, mt, st);
-            if (!r && p2) r = launch_pass<OKMI_PREC_BF16X6>(a, 2, mt, st);
-            if (!r && p3) r = launch_pass<OKMI_PREC_BF16X6>(a, 3, mt, st);
+            r = launch_pass<OKMI_PREC_BF16X6>(a, p1 ? 1 : 0, mt, lm, st);
+            if (!r && p2) r = launch_pass<OKMI_PREC_BF16X6>(a, 2, mt, lm, st);
+            if (!r && p3) r = launch_pass<OKMI_PREC_BF16X6>(a, 3, mt, lm, st);
         }
         if (r) return r;
         if (nout > 0)
@@ -855,16 +894,16 @@ __device__ __forceinline__ int find_item(const int64_t* start, int n, int64_t t)
     return lo;
 }
 
-template <int PREC, int PASS, int MT>
-__global__ __launch_bounds__(256, 2) void k_rho_batch(BatchArgs b)
+template <int PREC, int PASS, int MT, int LM>
+__global__ __launch_bounds__(256, (FusedPlan<PASS, LM, MT, true>::occupancy)) void k_rho_batch(BatchArgs b)
 {
-    __shared__ float lds[2 * FusedLds<PASS>::BUF];
+    __shared__ float lds[FusedPlan<PASS, LM, MT, true>::buffers * FusedLds<PASS>::BUF];
     __shared__ int next[2];
     __shared__ int live[2];
     for (int64_t t = blockIdx.x; t < b.ntile; t += gridDim.x) {
         const int m = find_item(b.tile_start, b.nmol, t);
         const FusedArgs a = load_fused_args(b.mols, m);
-        rho_tile<PREC, PASS, MT>(a, t - cptr(b.tile_start)[m], lds, next, live);
+        rho_tile<PREC, PASS, MT, LM, true>(a, t - cptr(b.tile_start)[m], lds, next, live);
     }
 }
 
@@ -894,30 +933,33 @@ __global__ void k_prep_batch(const PrepItem* items, const int64_t* start, int ni
     }
 }
 
-template <int PREC, int PASS>
+template <int PREC, int PASS, int LM>
 int launch_batch_mt(const BatchArgs& a, int mt, unsigned grid, hipStream_t st)
 {
     switch (mt) {
-    case 1: hipLaunchKernelGGL((k_rho_batch<PREC, PASS, 1>), dim3(grid), dim3(256), 0, st, a); return 0;
-    case 2: hipLaunchKernelGGL((k_rho_batch<PREC, PASS, 2>), dim3(grid), dim3(256), 0, st, a); return 0;
-    case 3: hipLaunchKernelGGL((k_rho_batch<PREC, PASS, 3>), dim3(grid), dim3(256), 0, st, a); return 0;
-    case 4: hipLaunchKernelGGL((k_rho_batch<PREC, PASS, 4>), dim3(grid), dim3(256), 0, st, a); return 0;
+    case 1: hipLaunchKernelGGL((k_rho_batch<PREC, PASS, 1, LM>), dim3(grid), dim3(256), 0, st, a); return 0;
+    case 2: hipLaunchKernelGGL((k_rho_batch<PREC, PASS, 2, LM>), dim3(grid), dim3(256), 0, st, a); return 0;
+    case 3: hipLaunchKernelGGL((k_rho_batch<PREC, PASS, 3, LM>), dim3(grid), dim3(256), 0, st, a); return 0;
+    case 4: hipLaunchKernelGGL((k_rho_batch<PREC, PASS, 4, LM>), dim3(grid), dim3(256), 0, st, a); return 0;
     default: break;
     }
     okmi_set_error("batch: %d MO tiles (up to 4)", mt);
     return 1;
 }
 
+// lm: highest angular momentum over the batch's bases
 template <int PREC>
-int launch_batch(const BatchArgs& a, int pass, int mt, unsigned grid, hipStream_t st)
+int launch_batch(const BatchArgs& a, int pass, int mt, int lm, unsigned grid, hipStream_t st)
 {
-    return pass == 0 ? launch_batch_mt<PREC, 0>(a, mt, grid, st) : launch_batch_mt<PREC, 1>(a, mt, grid, st);
+    if (lm <= 3)
+        return pass == 0 ? launch_batch_mt<PREC, 0, 3>(a, mt, grid, st) : launch_batch_mt<PREC, 1, 3>(a, mt, grid, st);
+    return pass == 0 ? launch_batch_mt<PREC, 0, 4>(a, mt, grid, st) : launch_batch_mt<PREC, 1, 4>(a, mt, grid, st);
 }
 
 }  // namespace
 
 struct OkmiBatch {
-    int nmol = 0, prec = 0, pass = 0, mt = 0;
+    int nmol = 0, prec = 0, pass = 0, mt = 0, lm = 0;   // lm: highest angular momentum
     int64_t mpad = 0, ntile = 0, prep_total = 0;
     unsigned grid = 0;                    // persistent workgroups (0: one per tile)
     FusedArgs* d_mols = nullptr;
@@ -1022,6 +1064,7 @@ int okmi_batch_build(int nmol, OkmiBasis* const* bases, const GridDesc* grids, c
         }
         mt = std::max(mt, (nmo[m] + 15) / 16);
         grad = grad || d_grad[m] != nullptr;
+        for (int s = 0; s < bases[m]->nshell; ++s) B->lm = std::max(B->lm, bases[m]->h_shells[s].l);
         double prims = 0;
         for (int s = 0; s < bases[m]->nshell; ++s) prims += bases[m]->h_shells[s].nprim + bases[m]->h_shells[s].nrow;
         cost[m] = prims * (double)npts[m];
@@ -1099,9 +1142,9 @@ int okmi_batch_enqueue(OkmiBatch* B, hipStream_t st)
     a.mols = B->d_mols; a.tile_start = B->d_tile_start; a.nmol = B->nmol; a.ntile = B->ntile;
     const unsigned grid = B->grid ? (unsigned)std::min<int64_t>(B->grid, B->ntile) : (unsigned)B->ntile;
     switch (B->prec) {
-    case OKMI_PREC_F32: return launch_batch<OKMI_PREC_F32>(a, B->pass, B->mt, grid, st);
-    case OKMI_PREC_BF16X3: return launch_batch<OKMI_PREC_BF16X3>(a, B->pass, B->mt, grid, st);
-    default: return launch_batch<OKMI_PREC_BF16X6>(a, B->pass, B->mt, grid, st);
+    case OKMI_PREC_F32: return launch_batch<OKMI_PREC_F32>(a, B->pass, B->mt, B->lm, grid, st);
+    case OKMI_PREC_BF16X3: return launch_batch<OKMI_PREC_BF16X3>(a, B->pass, B->mt, B->lm, grid, st);
+    default: return launch_batch<OKMI_PREC_BF16X6>(a, B->pass, B->mt, B->lm, grid, st);
     }
 }
 

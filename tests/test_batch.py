@@ -160,3 +160,52 @@ def test_batch_tile_shapes(mols, shape):
   plan.launch()
   for m in range(len(mols)):
     _close(ref.result(m), plan.result(m), (shape, m))
+
+
+def _g_molecule(nmo=20, side=14, seed=11):
+  '''Four atoms with Cartesian [s, p, d, f, g] shells (l = 4: the fused
+  kernels' general instances) and nmo orthonormal MOs on a side^3 grid.'''
+  from orbkit_amd import workloads as W
+  from orbkit_amd.qcinfo import QCinfo
+  rng = numpy.random.default_rng(seed)
+  geo = W._atoms(rng, 4, 5.0)
+  ao = W._basis(rng, geo, [['s', 'p', 'd', 'f', 'g']] * 4, 0.1, 1e3, False)
+  qc = QCinfo()
+  qc.geo_spec = geo
+  qc.geo_info = numpy.array([['X', str(i + 1), '0'] for i in range(len(geo))])
+  qc.ao_spec = ao
+  qc.mo_spec = W._mos(rng, W.ao_overlap(geo, ao), nmo)
+  return qc, W._grid(geo, side)
+
+
+def test_g_shells_and_many_tiles(mols):
+  '''Fused instances other than the one-buffer l <= 3 ones: a basis with g
+  shells (fused path and a batch holding it next to l <= 3 molecules) and
+  64 MOs (4 MO tiles) on the medium basis; each against the FP64 path.'''
+  import torch
+  from orbkit_amd import core, grid, workloads
+  from orbkit_amd.batch import BatchPlan
+  qc, axes = _g_molecule()
+  print('case g shells', flush=True)
+  grid.set_grid(*axes, is_vector=False)
+  r64, d64 = core.rho_compute(qc, drv=['x', 'y', 'z'])
+  for prec in ('bf16x6', 'f32'):
+    r, d = core.rho_compute(qc, drv=['x', 'y', 'z'], precision=prec)
+    assert numpy.linalg.norm(r - r64) / numpy.linalg.norm(r64) <= 1e-5, prec
+    assert numpy.linalg.norm(d - d64) / numpy.linalg.norm(d64) <= 1e-5, prec
+  items = [mols[0], (qc, axes), mols[1]]
+  plan = BatchPlan(items, precision='bf16x6')
+  plan.launch()
+  torch.cuda.synchronize()
+  for m, item in enumerate(items):
+    rs, ds, _ = _single(item, 'bf16x6')
+    rb, db = plan.result(m)
+    assert numpy.array_equal(rb.reshape(-1), rs), m
+    assert numpy.array_equal(db.reshape(3, -1), ds), m
+  print('case 64 MOs', flush=True)
+  qc, axes, _ = workloads.make('medium', npts_side=16, nmo=64)
+  grid.set_grid(*axes, is_vector=False)
+  r64, d64 = core.rho_compute(qc, drv=['x', 'y', 'z'])
+  r, d = core.rho_compute(qc, drv=['x', 'y', 'z'], precision='bf16x6')
+  assert numpy.linalg.norm(r - r64) / numpy.linalg.norm(r64) <= 1e-5
+  assert numpy.linalg.norm(d - d64) / numpy.linalg.norm(d64) <= 1e-5
