@@ -290,6 +290,68 @@ __device__ __forceinline__ void load_c(const FusedArgs& a, int k0, int g, int n1
     }
 }
 
+// The split C operand of one chunk, shared by the four waves through LDS in
+// the one-buffer schedule: [plane][MT * 16 rows] of 80-byte rows, each the
+// chunk's 32 bf16 (four 16-byte slots) and a 16-byte pad slot, so that the
+// contraction's 16-byte reads (lanes n16 = 0..15 on rows 20 dwords apart)
+// fall in distinct banks.  The workgroup loads it cooperatively when the
+// chunk's evaluation starts (one copy instead of one per wave, its latency
+// hidden behind the evaluation) and writes it to LDS before the barrier.
+template <int PREC, int MT> struct CStage {
+    static constexpr bool on = PREC != OKMI_PREC_F32;
+    static constexpr int NP = PREC == OKMI_PREC_BF16X6 ? 3 : 2;
+    static constexpr int data = on ? NP * MT * 16 * 4 : 0;    // 16-byte data slots
+    static constexpr int slots = on ? NP * MT * 16 * 5 : 0;   // with the pad slots
+    static constexpr int rounds = (data + 255) / 256;          // per thread (256 threads)
+};
+
+template <int PREC, int MT>
+struct CRegs {
+    bf8 v[CStage<PREC, MT>::rounds > 0 ? CStage<PREC, MT>::rounds : 1];
+};
+
+// Data slot d = 4 r + c (row r = plane q, MO m; k0 + 8c .. k0 + 8c + 7) for
+// d = 256 t + thread.  The loads are unconditional (threads past the end
+// reload the last slot) and use the global address space, so that they are
+// counted by vmcnt alone and the evaluation never waits for them.
+template <int PREC, int MT>
+__device__ __forceinline__ void fetch_c(const FusedArgs& a, int k0, CRegs<PREC, MT>& cr)
+{
+    using S = CStage<PREC, MT>;
+    typedef const __attribute__((address_space(1))) __bf16 gbf16;
+    typedef const __attribute__((address_space(1))) bf8 gbf8;
+    gbf16* C = (gbf16*)a.C;
+#pragma unroll
+    for (int t = 0; t < S::rounds; ++t) {
+        const int d = min(t * 256 + (int)threadIdx.x, S::data - 1);
+        const int r = d >> 2, c = d & 3;
+        const int q = r / (MT * 16), m = r - q * (MT * 16);
+        cr.v[t] = *(gbf8*)(C + ((int64_t)q * a.mpad + m) * a.kpad + k0 + 8 * c);
+    }
+}
+
+template <int PREC, int MT>
+__device__ __forceinline__ void put_c(const CRegs<PREC, MT>& cr, bf8* cs)
+{
+    using S = CStage<PREC, MT>;
+#pragma unroll
+    for (int t = 0; t < S::rounds; ++t) {
+        const int d = t * 256 + (int)threadIdx.x;
+        if (d < S::data) cs[(d >> 2) * 5 + (d & 3)] = cr.v[t];
+    }
+}
+
+// the lane's operand A fragments (MO row 16 mt + n16, k = 8g .. 8g + 7)
+template <int PREC, int MT>
+__device__ __forceinline__ void read_c(const bf8* cs, int g, int n16, COperand<PREC, MT>& c)
+{
+    constexpr int NP = CStage<PREC, MT>::NP;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int q = 0; q < NP; ++q) c.ca[mt].p[q] = cs[((q * MT + mt) * 16 + n16) * 5 + g];
+}
+
 // AO rows of chunk `ch` for the tile's 64 points (lane = point) into the LDS
 // tile L.  The calling waves take the chunk's shells one at a time from the
 // counter *next (costliest first), so that screening cannot leave one wave
@@ -479,11 +541,12 @@ template <int PASS, int LM, int MT, bool BATCH> struct FusedPlan {
 
 // One tile of 64 points by four waves that all evaluate and all contract:
 // every thread of the workgroup calls this with the same tile.  `lds` holds
-// FusedPlan::buffers AO buffers, `next` two work counters, `live` two chunk
+// FusedPlan::buffers AO buffers, `cst` the shared C operand (one-buffer
+// schedule, split precisions), `next` two work counters, `live` two chunk
 // tags.  The first barrier orders it after any previous tile's LDS reads.
 // LM: highest angular momentum of the basis (3 or 4).
 template <int PREC, int PASS, int MT, int LM, bool BATCH>
-__device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float* lds, int* next, int* live)
+__device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float* lds, bf8* cst, int* next, int* live)
 {
     constexpr int NS = PassCodes<PASS>::n;
     constexpr int BUF = FusedLds<PASS>::BUF;
@@ -504,16 +567,26 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float
         for (int s = 0; s < NS; ++s) acc[mt][s] = f4{0.f, 0.f, 0.f, 0.f};
 
     if constexpr (FusedPlan<PASS, LM, MT, BATCH>::single) {
-        // one buffer: evaluate, barrier, contract (C loaded now), barrier
+        // One buffer: evaluate (C fetched meanwhile), C to LDS, barrier,
+        // contract, barrier.  The point coordinates are consumed here so
+        // that the evaluation does not count their loads as outstanding
+        // (which would make it wait for the C loads issued after them).
+        asm volatile("" ::"v"(px), "v"(py), "v"(pz));
         for (int ch = 0; ch < a.nchunk; ++ch) {
+            CRegs<PREC, MT> cr;
+            if constexpr (CStage<PREC, MT>::on) fetch_c<PREC, MT>(a, ch * KC, cr);
             eval_chunk<PASS, LM>(a, ch, box, px, py, pz, lds, &next[0], &live[0], ch, wave == 0, lane);
+            // the staging area is free: every wave read it before the
+            // previous barrier
+            if constexpr (CStage<PREC, MT>::on) put_c<PREC, MT>(cr, cst);
             __syncthreads();
             // every wave finished taking from the counter before the barrier
             // and takes from it again after the next one
             if (threadIdx.x == 0) next[0] = 0;
             if (live[0] == ch) {
                 COperand<PREC, MT> c;
-                load_c<PREC, MT>(a, ch * KC, g, n16, c);
+                if constexpr (CStage<PREC, MT>::on) read_c<PREC, MT>(cst, g, n16, c);
+                else load_c<PREC, MT>(a, ch * KC, g, n16, c);
                 contract<PREC, PASS, MT>(lds, wave, g, n16, c, acc);
             }
             __syncthreads();
@@ -555,10 +628,12 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, float
 template <int PREC, int PASS, int MT, int LM>
 __global__ __launch_bounds__(256, (FusedPlan<PASS, LM, MT, false>::occupancy)) void k_rho_fused(FusedArgs a)
 {
-    __shared__ float lds[FusedPlan<PASS, LM, MT, false>::buffers * FusedLds<PASS>::BUF];
+    using P = FusedPlan<PASS, LM, MT, false>;
+    __shared__ float lds[P::buffers * FusedLds<PASS>::BUF];
+    __shared__ bf8 cst[P::single && CStage<PREC, MT>::on ? CStage<PREC, MT>::slots : 1];
     __shared__ int next[2];   // per buffer: next work entry to evaluate
     __shared__ int live[2];   // per buffer: index of the last chunk with a live shell
-    rho_tile<PREC, PASS, MT, LM, false>(a, blockIdx.x, lds, next, live);
+    rho_tile<PREC, PASS, MT, LM, false>(a, blockIdx.x, lds, cst, next, live);
 }
 
 struct GatherCodes {
@@ -897,13 +972,15 @@ __device__ __forceinline__ int find_item(const int64_t* start, int n, int64_t t)
 template <int PREC, int PASS, int MT, int LM>
 __global__ __launch_bounds__(256, (FusedPlan<PASS, LM, MT, true>::occupancy)) void k_rho_batch(BatchArgs b)
 {
-    __shared__ float lds[FusedPlan<PASS, LM, MT, true>::buffers * FusedLds<PASS>::BUF];
+    using P = FusedPlan<PASS, LM, MT, true>;
+    __shared__ float lds[P::buffers * FusedLds<PASS>::BUF];
+    __shared__ bf8 cst[P::single && CStage<PREC, MT>::on ? CStage<PREC, MT>::slots : 1];
     __shared__ int next[2];
     __shared__ int live[2];
     for (int64_t t = blockIdx.x; t < b.ntile; t += gridDim.x) {
         const int m = find_item(b.tile_start, b.nmol, t);
         const FusedArgs a = load_fused_args(b.mols, m);
-        rho_tile<PREC, PASS, MT, LM, true>(a, t - cptr(b.tile_start)[m], lds, next, live);
+        rho_tile<PREC, PASS, MT, LM, true>(a, t - cptr(b.tile_start)[m], lds, cst, next, live);
     }
 }
 
