@@ -22,10 +22,15 @@ FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17', '-x', 'hip',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
          '-munsafe-fp-atomics', '-I', INCLUDE, '-I', CSRC]
 
+# per-source additions: the fused kernel's only atomic is a work counter
+# taken by one lane, which the atomic optimizer would wrap in a
+# wave-reduction sequence on every claim
+EXTRA = {'rho_fused.hip': ['-mllvm', '-amdgpu-atomic-optimizer-strategy=None']}
+
 
 def _compile(src, verbose):
   obj = os.path.join(OBJ, os.path.splitext(src)[0] + '.o')
-  cmd = [HIPCC] + FLAGS + ['-c', os.path.join(CSRC, src), '-o', obj]
+  cmd = [HIPCC] + FLAGS + EXTRA.get(src, []) + ['-c', os.path.join(CSRC, src), '-o', obj]
   r = subprocess.run(cmd, capture_output=True, text=True)
   out = '\n'.join(l for l in (r.stdout + r.stderr).splitlines()
                   if 'packed-fp32-ops' not in l)

@@ -372,7 +372,7 @@ __device__ __forceinline__ void eval_chunk(const FusedArgs& a, int ch, const Box
     for (;;) {
         int e = 0;
         if (lane == 0) e = atomicAdd(next, 1);
-        e = __builtin_amdgcn_readfirstlane(__shfl(e, 0));
+        e = __builtin_amdgcn_readfirstlane(e);   // all lanes active again: lane 0's value
         if (e >= nsh) break;
         const ShellRec sh = load_shell(a.shells, cw.x + e);
         const double d2 = box.dist2(sh);
