@@ -283,6 +283,24 @@ def test_fused_precision(ok, prec, tol):
     assert errs[k] <= tol, (k, errs[k])
 
 
+@pytest.mark.parametrize('nmo', [5, 20, 40])
+@pytest.mark.parametrize('prec,tol', [('bf16x6', 1e-5), ('bf16x3', 1e-3)])
+def test_fused_mo_tiles(ok, prec, tol, nmo):
+  '''rho and grad rho (the schedule that shares the C operand through LDS)
+  with one, two and three 16-MO tiles and two or three bf16 planes, against
+  the FP64 path.'''
+  core, grid = ok['core'], ok['grid']
+  qc, axes, _ = ok['W'].make('medium', npts_side=17, nmo=nmo)
+  grid.set_grid(*axes, is_vector=False)
+  print('case fused tiles', prec, nmo, flush=True)
+  rho, g = core.rho_compute(qc, drv=['x', 'y', 'z'])
+  frho, fg = core.rho_compute(qc, drv=['x', 'y', 'z'], precision=prec)
+  e_rho = numpy.linalg.norm(frho - rho) / numpy.linalg.norm(rho)
+  e_g = numpy.linalg.norm(fg - g) / numpy.linalg.norm(g)
+  print(prec, nmo, e_rho, e_g)
+  assert e_rho <= tol and e_g <= tol, (e_rho, e_g)
+
+
 @pytest.mark.parametrize('prec,tol', [('f32', 1e-5), ('bf16x6', 1e-5), ('bf16x3', 1e-3)])
 def test_many_mo_precision(ok, prec, tol):
   '''More MOs than the fused kernel holds (130: two 128-MO tiles, the second
