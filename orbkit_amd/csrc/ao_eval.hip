@@ -58,8 +58,10 @@ template <int L, typename OutT>
 __device__ __forceinline__ void store_rows(const AoArgs& a, const ShellRec& sh, const double* c, OutT* o, bool act)
 {
     constexpr int NC = ncart(L);
+    // streaming (non-temporal) stores: the sets are read back by a later
+    // kernel from HBM, so the rows are not kept in L2
     auto store = [&](int r, double v) {
-        if (act) o[(int64_t)r * a.ld] = (OutT)v;
+        if (act) __builtin_nontemporal_store((OutT)v, &o[(int64_t)r * a.ld]);
     };
     // standard spherical / Cartesian shells: sparse compile-time rows;
     // other transforms (ORCA row factors, custom maps): the dense block
@@ -184,7 +186,7 @@ void k_ao_eval(AoArgs a, OutT* __restrict__ out)
                 for (int c = 0; c < OKMI_MAX_SETS; ++c) {
                     if (a.dst[c] < 0) continue;
                     OutT* o = out + a.dst[c] * a.set_stride + (int64_t)sh.row0 * a.ld + i;
-                    for (int r = 0; r < sh.nrow; ++r) o[(int64_t)r * a.ld] = (OutT)0;
+                    for (int r = 0; r < sh.nrow; ++r) __builtin_nontemporal_store((OutT)0, &o[(int64_t)r * a.ld]);
                 }
             continue;
         }
