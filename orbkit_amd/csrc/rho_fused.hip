@@ -8,9 +8,10 @@
 // shared counter, costliest first (lane = point), and evaluate them in FP64
 // (coordinates, exponentials, polynomials, the Cartesian-to-spherical map),
 // storing each value rounded once to FP32 in LDS [row][point] (one buffer
-// and three workgroups per CU for rho / rho + grad rho with l <= 3, else
-// double buffered with one barrier per chunk; see FusedPlan).  Each wave then multiplies the rows of its
-// 16-point column with the matching 32 columns of C:
+// and three workgroups per CU for rho / rho + grad rho with l <= 3, with the
+// chunk's C columns shared through LDS; else double buffered with one
+// barrier per chunk; see FusedPlan).  Each wave then multiplies the rows of
+// its 16-point column with the matching 32 columns of C:
 //   F32     v_mfma_f32_16x16x4_f32, C and AO in FP32
 //   BF16X3  v_mfma_f32_16x16x32_bf16 on (hi, lo) splits, 3 products
 //   BF16X6  (hi, mid, lo) splits, 6 products (terms below 2^-24 dropped)
@@ -531,8 +532,9 @@ __device__ __forceinline__ Box tile_box(const FusedArgs& a, int64_t tile, double
 // Whether an instance uses one LDS buffer and three workgroups per CU: rho
 // and rho + grad rho with l <= 3 and up to 48 MOs (32 in the batch kernel,
 // which needs a few more registers) fit in 168 VGPRs without spilling (33 KB
-// LDS) and run at 3 waves/SIMD; the others keep two buffers (66.5 KB,
-// 2 waves/SIMD), where holding C through the evaluation hides its latency.
+// AO tile plus up to 11.5 KB of C stage) and run at 3 waves/SIMD; the others
+// keep two buffers (66.5 KB, 2 waves/SIMD), where holding C through the
+// evaluation hides its latency.
 template <int PASS, int LM, int MT, bool BATCH> struct FusedPlan {
     static constexpr bool single = PASS <= 1 && LM <= 3 && MT <= (BATCH ? 2 : 3);
     static constexpr int buffers = single ? 1 : 2;
