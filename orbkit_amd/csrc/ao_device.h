@@ -195,12 +195,14 @@ __device__ __forceinline__ float gauss_f32(double t)
     return __builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f(-f), -(int)n);
 }
 
-// exp(-t) for t >= 0 in FP64 with ~1e-11 relative error, for the paths whose
-// AO values end in FP32 or narrower.  Cody-Waite reduction t = k ln2 + r,
-// |r| <= ln2/2 (ln2 split in a high part exact in 32 bits and a low part),
-// then the degree-9 Taylor polynomial of exp(-r) (truncation error
-// r^10/10! < 7e-12) and 2^-k by exponent adjustment, which underflows to +0
-// for large t.  Branch-free: t is clamped to 1e4 (exp(-1e4) is +0 in FP64).
+// exp(-t) for t >= 0 in FP64 with 2.6e-9 relative error (under 1/20 of an
+// FP32 half-ulp), for the paths whose AO values end in FP32 or narrower.
+// Cody-Waite reduction t = k ln2 + r, |r| <= ln2/2 (ln2 split in a high part
+// exact in 32 bits and a low part), then exp(s), s = -r, by a degree-6
+// polynomial: the Chebyshev interpolant of exp on [-ln2/2, ln2/2] (7
+// Chebyshev nodes, converted to monomial coefficients), and 2^-k by exponent
+// adjustment, which underflows to +0 for large t.  Branch-free: t is
+// clamped to 1e4 (exp(-1e4) is +0 in FP64).
 __device__ __forceinline__ double gauss_f64_fast(double t)
 {
     t = fmin(t, 1.0e4);
@@ -208,16 +210,13 @@ __device__ __forceinline__ double gauss_f64_fast(double t)
     double r = fma(-k, 6.93147180369123816490e-01, t);
     r = fma(-k, 1.90821492927058770002e-10, r);
     const double s = -r;
-    double p = 2.7557319223985893e-06;             // 1/9!
-    p = fma(p, s, 2.4801587301587302e-05);         // 1/8!
-    p = fma(p, s, 1.9841269841269841e-04);         // 1/7!
-    p = fma(p, s, 1.3888888888888889e-03);         // 1/6!
-    p = fma(p, s, 8.3333333333333333e-03);         // 1/5!
-    p = fma(p, s, 4.1666666666666667e-02);         // 1/4!
-    p = fma(p, s, 1.6666666666666667e-01);         // 1/3!
-    p = fma(p, s, 0.5);
-    p = fma(p, s, 1.0);
-    p = fma(p, s, 1.0);
+    double p = 0.0013941118940759135;
+    p = fma(p, s, 0.0083751347746787615);
+    p = fma(p, s, 0.041666352770387056);
+    p = fma(p, s, 0.16666415414041169);
+    p = fma(p, s, 0.50000000471463202);
+    p = fma(p, s, 1.0000000377388703);
+    p = fma(p, s, 0.99999999999999989);
     return ldexp(p, -(int)k);
 }
 
