@@ -22,10 +22,13 @@ FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17', '-x', 'hip',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
          '-munsafe-fp-atomics', '-I', INCLUDE, '-I', CSRC]
 
-# per-source additions: the fused kernel's only atomic is a work counter
-# taken by one lane, which the atomic optimizer would wrap in a
-# wave-reduction sequence on every claim
-EXTRA = {'rho_fused.hip': ['-mllvm', '-amdgpu-atomic-optimizer-strategy=None']}
+# per-source additions:
+#   rho_fused.hip: its only atomic is a work counter taken by one lane, which
+#     the atomic optimizer would wrap in a wave-reduction sequence per claim;
+#   rho_gemm_m1.hip: FP64 MFMAs in the VGPR form (by default each K step
+#     copied the accumulators into AGPRs and back around every MFMA).
+EXTRA = {'rho_fused.hip': ['-mllvm', '-amdgpu-atomic-optimizer-strategy=None'],
+         'rho_gemm_m1.hip': ['-mllvm', '-amdgpu-mfma-vgpr-form']}
 
 
 def _compile(src, verbose):

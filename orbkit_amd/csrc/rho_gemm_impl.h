@@ -155,7 +155,10 @@ __device__ __forceinline__ void rho_terms(const d4 (&acc)[NSC], int i, double f0
 // MFMAs (uniform branches).  Per point, with w = occ:
 //   rho   += w f0^2
 //   d_c   += 2 w (f0 f_c + e_c),  e_c = f_a^2 (c = aa), f_a f_b (c = ab), 0 (c = a)
-template <int MG, int NSC>
+// FULL: every code 0..NSC-1 present (no per-set branches) and the loads
+// unconditional (rows clamped, the zeroing done by selects), a straight
+// sequence of MG x NSC MFMAs per K step
+template <int MG, int NSC, bool FULL>
 __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -189,15 +192,35 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
             const int k = k0 + kq;
             const bool kin = k < a.nao;
             double av[MG];
+            if constexpr (FULL) {
+                // rows past the end read row nao - 1 and are zeroed through
+                // av; points past the end read point 0 and are zeroed
+                // through bv (they must add nothing to mo_norm)
+                const int kc = kin ? k : a.nao - 1;
 #pragma unroll
-            for (int mt = 0; mt < MG; ++mt) av[mt] = (kin && min_[mt]) ? cbase[mt][k] : 0.0;
+                for (int mt = 0; mt < MG; ++mt) {
+                    const double c = cbase[mt][kc];
+                    av[mt] = (kin && min_[mt]) ? c : 0.0;
+                }
 #pragma unroll
-            for (int d = 0; d < NSC; ++d) {
-                if (a.set_mask & (1 << d)) {
-                    const double bv = (kin && pin) ? abase[a.set_off[d] + (int64_t)k * a.ld] : 0.0;
+                for (int d = 0; d < NSC; ++d) {
+                    const double b = abase[a.set_off[d] + (int64_t)kc * a.ld];
+                    const double bv = pin ? b : 0.0;
 #pragma unroll
                     for (int mt = 0; mt < MG; ++mt)
                         acc[mt][d] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bv, acc[mt][d], 0, 0, 0);
+                }
+            } else {
+#pragma unroll
+                for (int mt = 0; mt < MG; ++mt) av[mt] = (kin && min_[mt]) ? cbase[mt][k] : 0.0;
+#pragma unroll
+                for (int d = 0; d < NSC; ++d) {
+                    if (a.set_mask & (1 << d)) {
+                        const double bv = (kin && pin) ? abase[a.set_off[d] + (int64_t)k * a.ld] : 0.0;
+#pragma unroll
+                        for (int mt = 0; mt < MG; ++mt)
+                            acc[mt][d] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bv, acc[mt][d], 0, 0, 0);
+                    }
                 }
             }
         }
@@ -244,23 +267,32 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
     }
 }
 
-template <int MG>
-int launch_rho_nsc(const GemmArgs& a, int nsc, hipStream_t st)
+template <int MG, bool FULL>
+int launch_rho_full(const GemmArgs& a, int nsc, hipStream_t st)
 {
     dim3 blk(256), grd((unsigned)((a.np + 63) / 64));
     switch (nsc) {
-    case 1: hipLaunchKernelGGL((k_rho_f64<MG, 1>), grd, blk, 0, st, a); return 0;
-    case 4: hipLaunchKernelGGL((k_rho_f64<MG, 4>), grd, blk, 0, st, a); return 0;
-    case 7: hipLaunchKernelGGL((k_rho_f64<MG, 7>), grd, blk, 0, st, a); return 0;
+    case 1: hipLaunchKernelGGL((k_rho_f64<MG, 1, FULL>), grd, blk, 0, st, a); return 0;
+    case 4: hipLaunchKernelGGL((k_rho_f64<MG, 4, FULL>), grd, blk, 0, st, a); return 0;
+    case 7: hipLaunchKernelGGL((k_rho_f64<MG, 7, FULL>), grd, blk, 0, st, a); return 0;
     default:
         if constexpr (MG <= 3) {
-            hipLaunchKernelGGL((k_rho_f64<MG, 10>), grd, blk, 0, st, a);
+            hipLaunchKernelGGL((k_rho_f64<MG, 10, FULL>), grd, blk, 0, st, a);
             return 0;
         }
         break;
     }
     okmi_set_error("rho_f64: unsupported code range %d for MO group %d", nsc, MG);
     return 1;
+}
+
+// All codes 0..nsc-1 present and at least 4 of them: the branch-free
+// instance (measured faster there; with one set the masked loop is faster).
+template <int MG>
+int launch_rho_nsc(const GemmArgs& a, int nsc, hipStream_t st)
+{
+    if (nsc >= 4 && a.set_mask == (1 << nsc) - 1) return launch_rho_full<MG, true>(a, nsc, st);
+    return launch_rho_full<MG, false>(a, nsc, st);
 }
 
 inline int launch_rho(const GemmArgs& a, int nsc, hipStream_t st)

@@ -248,15 +248,20 @@ def test_empty_grid(ok):
   assert ao.shape == (25, 0)
 
 
-def test_slice_rho_direct(ok):
+@pytest.mark.parametrize('drv', [['x', 'yy'], ['x', 'y', 'z'], ['x', 'y', 'z', 'xx', 'yy', 'zz']])
+def test_slice_rho_direct(ok, drv):
+  '''slice_rho's three outputs, MO norms included, over an 800-point slice
+  (the last 64-point tile is partial); ['x', 'y', 'z'] and the six-label
+  list use every code 0..3 / 0..6, the FP64 kernel without set branches.'''
   core, grid, O = ok['core'], ok['grid'], ok['O']
   qc, axes, _ = ok['W'].make('small', npts_side=12)
   grid.set_grid(*axes, is_vector=False)
   grid.grid2vector()
-  core.initializer({'qc': qc, 'calc_ao': False, 'calc_mo': False, 'derivative': ['x', 'yy']})
+  print('case slice_rho', drv, flush=True)
+  core.initializer({'qc': qc, 'calc_ao': False, 'calc_mo': False, 'derivative': drv})
   rho, mo_norm, delta = core.slice_rho([100, 900])
   r = O.slice_rho(qc.geo_spec, qc.ao_spec, qc.mo_spec, grid.x[100:900], grid.y[100:900], grid.z[100:900],
-                  ['x', 'yy'], False, False, ok['tools'].get_cart2sph)
+                  drv, False, False, ok['tools'].get_cart2sph)
   assert rel_err(rho, r[0]) <= TOL and rel_err(mo_norm, r[1]) <= TOL and rel_err(delta, r[2]) <= TOL
 
 
