@@ -193,19 +193,19 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
             const bool kin = k < a.nao;
             double av[MG];
             if constexpr (FULL) {
-                // rows past the end read row nao - 1 and are zeroed through
+                // Rows past the end read row nao - 1 and are zeroed through
                 // av; points past the end read point 0 and are zeroed
-                // through bv (they must add nothing to mo_norm)
+                // through bv (they must add nothing to mo_norm).  The zeroing
+                // multiplies by 0 or 1 (the operands are finite) instead of
+                // selecting: the compiler turns a select of a load into a
+                // masked load in its own branch with its own wait, which
+                // serialised the K step's loads.
                 const int kc = kin ? k : a.nao - 1;
 #pragma unroll
-                for (int mt = 0; mt < MG; ++mt) {
-                    const double c = cbase[mt][kc];
-                    av[mt] = (kin && min_[mt]) ? c : 0.0;
-                }
+                for (int mt = 0; mt < MG; ++mt) av[mt] = cbase[mt][kc] * ((kin && min_[mt]) ? 1.0 : 0.0);
 #pragma unroll
                 for (int d = 0; d < NSC; ++d) {
-                    const double b = abase[a.set_off[d] + (int64_t)kc * a.ld];
-                    const double bv = pin ? b : 0.0;
+                    const double bv = abase[a.set_off[d] + (int64_t)kc * a.ld] * (pin ? 1.0 : 0.0);
 #pragma unroll
                     for (int mt = 0; mt < MG; ++mt)
                         acc[mt][d] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bv, acc[mt][d], 0, 0, 0);
