@@ -286,13 +286,12 @@ int launch_rho_full(const GemmArgs& a, int nsc, hipStream_t st)
     return 1;
 }
 
-// All codes 0..nsc-1 present, at least 4 of them, and at most 3 MO tiles
-// per group: the branch-free instance (measured faster there; with one set
-// or four MO tiles per group the masked loop is faster).
+// All codes 0..nsc-1 present and at least 4 of them: the branch-free
+// instance (measured faster there; with one set the masked loop is faster).
 template <int MG>
 int launch_rho_nsc(const GemmArgs& a, int nsc, hipStream_t st)
 {
-    if (MG <= 3 && nsc >= 4 && a.set_mask == (1 << nsc) - 1) return launch_rho_full<MG, true>(a, nsc, st);
+    if (nsc >= 4 && a.set_mask == (1 << nsc) - 1) return launch_rho_full<MG, true>(a, nsc, st);
     return launch_rho_full<MG, false>(a, nsc, st);
 }
 
