@@ -4,10 +4,14 @@ Default (the headline line): step = one evaluation of rho and grad(rho) over
 the whole 100^3 regular grid of the medium workload (20 atoms, spherical
 [s,s,p,p,d,f] shells, 400 AOs, 3 primitives per shell, exponents in
 [0.05, 1e5], 40 doubly occupied MOs), inputs resident in HBM, outputs written
-to HBM (FP64).  Prints one JSON line.
+to HBM, all arithmetic in FP64 (the fused FP64 kernel).  Beside it, in the
+same JSON line under "legs": the staged FP64 path (AO sets through HBM, then
+the FP64 GEMM) and the bf16x6 fused path, each with its time, roofline
+fraction, error against the FP64 result and speed-up over the headline.
+Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--precision bf16x6|bf16x3|f32|f64]
-                    [--workload medium|small|large|gradient|batch] [--laplacian]
+    python bench.py [--steps K] [--warmup W] [--precision f64|bf16x6|bf16x3|f32]
+                    [--workload medium|small|large|gradient|batch] [--laplacian] [--no-legs]
 
 Other workloads (parity cases of the same path, measured the same way):
   large     200 atoms, 2,950 spherical AOs, 400 MOs, 128^3 grid (many-MO path)
@@ -40,11 +44,12 @@ def parse():
   ap.add_argument('--gpus', type=int, default=1)
   ap.add_argument('--steps', type=int, default=20)
   ap.add_argument('--warmup', type=int, default=3)
-  ap.add_argument('--precision', default='bf16x6', choices=['bf16x6', 'bf16x3', 'f32', 'f64'])
+  ap.add_argument('--precision', default='f64', choices=['bf16x6', 'bf16x3', 'f32', 'f64'])
   ap.add_argument('--workload', default='medium', choices=['medium', 'small', 'large', 'gradient', 'batch'])
   ap.add_argument('--side', type=int, default=None, help='grid points per axis (default: workload)')
   ap.add_argument('--laplacian', action='store_true', help='also the Laplacian (7 AO sets)')
   ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+  ap.add_argument('--no-legs', action='store_true', help='time only the headline path')
   ap.add_argument('--cpu-threads', type=int, default=None)
   ap.add_argument('--errors', action='store_true', help='also report FP64-relative errors')
   ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
@@ -65,23 +70,65 @@ def measured_traffic(kernel, workload, precision):
   return None if v is None else v['bytes_per_launch']
 
 
-def cpu_baseline(qc, axes, threads, slice_length, codes, slices_per_thread=4):
-  '''orbkit's algorithm (C restatement in oracle/, OpenMP over slices) on a
-  bounded sample: slices_per_thread slices of slice_length points per
-  thread, a contiguous run of the flattened grid from its middle.'''
+def cpu_info():
+  """CPU model, logical CPUs of the machine, CPUs this process may run on,
+  and the thread budget of the job (OMP_NUM_THREADS where the environment
+  sets one: the CPU share that comes with one GPU)."""
+  model = 'unknown'
+  try:
+    with open('/proc/cpuinfo') as f:
+      for line in f:
+        if line.startswith('model name'):
+          model = line.split(':', 1)[1].strip()
+          break
+  except OSError:
+    pass
+  nproc = os.cpu_count() or 1
+  avail = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else nproc
+  env = os.environ.get('OMP_NUM_THREADS')
+  share = min(avail, int(env)) if env and env.isdigit() and int(env) > 0 else avail
+  return {'model': model, 'nproc': nproc, 'affinity': avail, 'threads': share}
+
+
+def cpu_baseline(qc, axes, threads, slice_length, codes, slices_per_thread=2, reps=5):
+  """orbkit's algorithm (C restatement in oracle/: orbkit's AO kernel, the
+  MO triple loop and slice_rho's assembly, OpenMP over slices standing in
+  for orbkit's Pool over slices) on a bounded sample of the grid: a
+  contiguous run from the middle of the flattened grid, slices_per_thread
+  slices of slice_length points per thread.  One warm-up, then the median of
+  `reps` timed runs; the same on one thread (one warm-up, median of 3) on
+  a sample of one slice."""
   from oracle import core_oracle
   X, Y, Z = numpy.meshgrid(*axes, indexing='ij')
   P = numpy.stack([X.ravel(), Y.ravel(), Z.ravel()])
-  n = int(min(P.shape[1], threads * slice_length * slices_per_thread))
-  a = max(0, P.shape[1] // 2 - n // 2)
-  x, y, z = P[0, a:a + n].copy(), P[1, a:a + n].copy(), P[2, a:a + n].copy()
-  core_oracle.rho_slices_c(qc, x[:64], y[:64], z[:64], codes, slice_length=64, nthreads=1)
-  t = time.perf_counter()
-  core_oracle.rho_slices_c(qc, x, y, z, codes, slice_length=slice_length, nthreads=threads)
-  dt = time.perf_counter() - t
+
+  def sample(n):
+    n = int(min(P.shape[1], n))
+    a = max(0, P.shape[1] // 2 - n // 2)
+    return n, P[0, a:a + n].copy(), P[1, a:a + n].copy(), P[2, a:a + n].copy()
+
+  def run(nthreads, n, reps):
+    n, x, y, z = sample(n)
+    core_oracle.rho_slices_c(qc, x[:slice_length // 4], y[:slice_length // 4], z[:slice_length // 4], codes,
+                             slice_length=slice_length, nthreads=nthreads)
+    ts = []
+    for _ in range(reps):
+      t = time.perf_counter()
+      core_oracle.rho_slices_c(qc, x, y, z, codes, slice_length=slice_length, nthreads=nthreads)
+      ts.append(time.perf_counter() - t)
+    return n, float(numpy.median(ts)), ts
+
+  info = cpu_info()
+  n, dt, ts = run(threads, threads * slice_length * slices_per_thread, reps)
+  n1, dt1, _ = run(1, slice_length, 3)
   return {'value': n / dt, 'unit': 'grid points/s', 'cores': threads, 'kind': 'port',
-          'sample': '%d of the %d grid points (%d slices of %d, rho + derivative codes %s, FP64) in %.1f s'
-                    % (n, P.shape[1], -(-n // slice_length), slice_length, list(codes), dt)}
+          'sample': '%d of the %d grid points (%d slices of %d, rho + derivative codes %s, FP64); median of %d '
+                    'runs after one warm-up: %.2f s (runs %s)'
+                    % (n, P.shape[1], -(-n // slice_length), slice_length, [int(c) for c in codes], reps, dt,
+                       ', '.join('%.2f' % t for t in ts)),
+          'one_core': {'value': n1 / dt1, 'sample': '%d points, median of 3 after one warm-up: %.2f s' % (n1, dt1)},
+          'cpu_model': info['model'], 'nproc': info['nproc'], 'affinity_cpus': info['affinity'],
+          'threads_note': 'threads = the CPUs this job may use (affinity, capped by OMP_NUM_THREADS where set)'}
 
 
 def cpu_baseline_grad(qc, axes, n=3000):
@@ -129,20 +176,18 @@ def timed(step, args, L, names, sync):
 
 # ---------------------------------------------------------------- workloads
 
-def density_step(args, torch, info, qc, axes):
-  '''rho (+ grad, + Laplacian) over one regular grid; fused / many-MO or FP64.'''
+def density_leg(args, torch, info, basis, dg, C, occ, codes, path):
+  """One way of computing rho (+ grad, + Laplacian) over the whole grid,
+  timed: path 'f64' (fused FP64), 'f64_staged' (AO sets through HBM, then
+  the FP64 GEMM with density epilogue), or a reduced precision of the fused
+  kernel ('bf16x6', 'bf16x3', 'f32'; more than 64 MOs: the many-MO path).
+  Returns (measurements, result getter)."""
   from orbkit_amd import _native
-  from orbkit_amd.basis import device_basis
-  from orbkit_amd.core import DeviceGrid
   from orbkit_amd.fused import FusedPlan
-  basis = device_basis(qc.geo_spec, qc.ao_spec)
-  dg = DeviceGrid(*axes, regular=True)
-  C, occ = qc.mo_spec.get_coeffs(), qc.mo_spec.get_occ()
-  codes = numpy.array([1, 2, 3] + ([4, 5, 6] if args.laplacian else []), dtype=numpy.intc)
-  nsets = 7 if args.laplacian else 4
   L = _native.lib()
   npts, nao, nmo = dg.npts, info['nao'], info['nmo']
-  if args.precision == 'f64':
+  nsets = 7 if args.laplacian else 4
+  if path == 'f64_staged' or (path == 'f64' and nmo > 64):
     dev = dg.x.device
     dC, docc = torch.as_tensor(C, device=dev), torch.as_tensor(occ, device=dev)
     rho = torch.empty(npts, dtype=torch.float64, device=dev)
@@ -158,14 +203,17 @@ def density_step(args, torch, info, qc, axes):
     names, result = ['ao_eval', 'rho_f64'], (lambda: (rho.cpu().numpy(), delta.cpu().numpy()))
     dom = ('rho_f64', 'k_rho_f64', 'f64', 1)
   else:
-    plan = FusedPlan(basis, dg, C, occ, codes, precision=args.precision)
+    plan = FusedPlan(basis, dg, C, occ, codes, precision=path)
     step, result = plan.launch, plan.result
-    if nmo > 64:
+    if path == 'f64':
+      names = ['rho_fused_f64']
+      dom = ('rho_fused_f64', 'k_rho_fused', 'f64', 1)
+    elif nmo > 64:
       names = ['ao_eval', 'rho_big']
-      dom = ('rho_big', 'k_rho_big', 'f32' if args.precision == 'f32' else 'bf16', PRODUCTS[args.precision])
+      dom = ('rho_big', 'k_rho_big', 'f32' if path == 'f32' else 'bf16', PRODUCTS[path])
     else:
       names = ['rho_fused']
-      dom = ('rho_fused', 'k_rho_fused', 'f32' if args.precision == 'f32' else 'bf16', PRODUCTS[args.precision])
+      dom = ('rho_fused', 'k_rho_fused', 'f32' if path == 'f32' else 'bf16', PRODUCTS[path])
   dt, times = timed(step, args, L, names, torch.cuda.synchronize)
   flops = 2.0 * nmo * nao * nsets * npts   # algorithmic, one pass over the grid
   cnt, ms = times[dom[0]]
@@ -173,35 +221,72 @@ def density_step(args, torch, info, qc, axes):
   achieved = flops / t_launch / 1e12
   peak = PEAK_TFLOPS[dom[2]] / dom[3]
   roof = {'bound': 'mfma', 'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s', 'frac': achieved / peak,
-          'traffic': measured_traffic(dom[1], args.workload, args.precision), 'kernel': dom[1],
+          'traffic': measured_traffic(dom[1], args.workload, 'f64' if path == 'f64_staged' else path),
+          'kernel': dom[1] + ('<f64>' if path == 'f64' else ''),
           'kernel_ms_per_step': t_launch * 1e3, 'launches_per_step': cnt / args.steps,
           'algorithmic_flop_per_step': flops}
   if dom[3] > 1:
     roof.update(executed_tflops=achieved * dom[3], mfma_dtype=dom[2], products=dom[3])
   roof['other_kernels_ms_per_step'] = {k: v[1] / args.steps for k, v in times.items() if k != dom[0]}
-  out = {'value': npts * args.steps / dt, 'ms_per_step': dt / args.steps * 1e3,
+  return {'value': npts * args.steps / dt, 'ms_per_step': dt / args.steps * 1e3, 'roofline': roof}, result
+
+
+def error_vs(r, d, rr, dd, axes, laplacian):
+  """Errors of (r, d) against the FP64 result (rr, dd): rho pointwise max
+  over rho > 1e-10 max rho and L2, gradient L2 and max, electron counts."""
+  mask = rr > 1e-10 * rr.max()
+  d3r = numpy.prod([a[1] - a[0] for a in axes])
+  err = {'rho_max_rel': float(numpy.max(numpy.abs(r - rr)[mask] / rr[mask])),
+         'rho_l2_rel': float(numpy.linalg.norm(r - rr) / numpy.linalg.norm(rr)),
+         'grad_l2_rel': float(numpy.linalg.norm(d[:3] - dd[:3]) / numpy.linalg.norm(dd[:3])),
+         'grad_max_rel': float(numpy.max(numpy.abs(d[:3] - dd[:3])) / numpy.max(numpy.abs(dd[:3]))),
+         'electrons': float(numpy.sum(r) * d3r), 'electrons_f64': float(numpy.sum(rr) * d3r)}
+  if laplacian:
+    lap, lap0 = d[3:].sum(axis=0), dd[3:].sum(axis=0)
+    err['lap_l2_rel'] = float(numpy.linalg.norm(lap - lap0) / numpy.linalg.norm(lap0))
+    err['lap_max_rel'] = float(numpy.max(numpy.abs(lap - lap0)) / numpy.max(numpy.abs(lap0)))
+  return err
+
+
+def density_step(args, torch, info, qc, axes):
+  """rho (+ grad, + Laplacian) over one regular grid.  The headline is the
+  precision asked for (default FP64, fused); beside it, unless --no-legs,
+  the staged FP64 path and the bf16x6 fused path are timed the same way,
+  with their errors against the FP64 result and their speed-up over it."""
+  from orbkit_amd.basis import device_basis
+  from orbkit_amd.core import DeviceGrid
+  basis = device_basis(qc.geo_spec, qc.ao_spec)
+  dg = DeviceGrid(*axes, regular=True)
+  C, occ = qc.mo_spec.get_coeffs(), qc.mo_spec.get_occ()
+  codes = numpy.array([1, 2, 3] + ([4, 5, 6] if args.laplacian else []), dtype=numpy.intc)
+  nsets = 7 if args.laplacian else 4
+  nao, nmo = info['nao'], info['nmo']
+  head, head_result = density_leg(args, torch, info, basis, dg, C, occ, codes, args.precision)
+  out = {'value': head['value'], 'ms_per_step': head['ms_per_step'],
          'dtype': {'bf16x6': 'bf16x6 (fp32 acc, fp64 epilogue)', 'bf16x3': 'bf16x3 (fp32 acc, fp64 epilogue)',
                    'f32': 'f32 (fp64 epilogue)', 'f64': 'f64'}[args.precision],
          'config': {'workload': '%s: %d atoms, %d spherical AOs (l<=3, 3 prims/shell), %d MOs, %d^3 regular grid, '
                                 'rho + grad rho%s' % (args.workload, info['natoms'], nao, nmo, info['side'],
                                                       ' + laplacian' if args.laplacian else ''),
-                    'npts': npts, 'nao': nao, 'nmo': nmo, 'ao_sets': nsets, 'precision': args.precision},
-         'roofline': roof}
-  if args.errors or args.precision != 'f64':
-    r, d = result()
+                    'npts': dg.npts, 'nao': nao, 'nmo': nmo, 'ao_sets': nsets,
+                    'path': 'fused' if args.precision != 'f64' or nmo <= 64 else 'staged'},
+         'roofline': head['roofline']}
+  legs = [] if args.no_legs else [p for p in ('f64_staged', 'bf16x6') if p != args.precision]
+  ref = None
+  if args.precision != 'f64' or legs or args.errors:
     from orbkit_amd.core import eval_rho
-    rr, dd, _ = eval_rho(basis, dg, C, occ, codes)
-    mask = rr > 1e-10 * rr.max()
-    d3r = numpy.prod([a[1] - a[0] for a in axes])
-    err = {'rho_max_rel': float(numpy.max(numpy.abs(r - rr)[mask] / rr[mask])),
-           'rho_l2_rel': float(numpy.linalg.norm(r - rr) / numpy.linalg.norm(rr)),
-           'grad_l2_rel': float(numpy.linalg.norm(d[:3] - dd[:3]) / numpy.linalg.norm(dd[:3])),
-           'grad_max_rel': float(numpy.max(numpy.abs(d[:3] - dd[:3])) / numpy.max(numpy.abs(dd[:3]))),
-           'electrons': float(numpy.sum(r) * d3r), 'electrons_f64': float(numpy.sum(rr) * d3r)}
-    if args.laplacian:
-      lap, lap0 = d[3:].sum(axis=0), dd[3:].sum(axis=0)
-      err['lap_l2_rel'] = float(numpy.linalg.norm(lap - lap0) / numpy.linalg.norm(lap0))
-    out['error_vs_f64'] = err
+    rr, dd, _ = eval_rho(basis, dg, C, occ, codes)   # staged FP64 path
+    ref = (rr, dd)
+  if ref is not None:
+    r, d = head_result()
+    out['error_vs_f64'] = error_vs(r, d, rr, dd, axes, args.laplacian)
+  out['legs'] = {}
+  for p in legs:
+    leg, res = density_leg(args, torch, info, basis, dg, C, occ, codes, p)
+    r, d = res()
+    leg['error_vs_f64'] = error_vs(r, d, rr, dd, axes, args.laplacian)
+    leg['speedup_vs_headline'] = head['ms_per_step'] / leg['ms_per_step']
+    out['legs'][p] = leg
   return out, codes
 
 
@@ -304,13 +389,13 @@ def main():
   if not args.no_cpu and args.workload == 'gradient':
     out['cpu_baseline'] = cpu_baseline_grad(qc, axes)
   elif not args.no_cpu and qc is not None:
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads or cpu_info()['threads']
     sl = int(info['slice_length'])
-    per = 4 if info['nao'] <= 1000 else 1
+    per = 2 if info['nao'] <= 1000 else 1
     if args.workload == 'large':
       sl = 512
     out['cpu_baseline'] = cpu_baseline(qc, axes, threads, sl, list(codes) if len(codes) else [1, 2, 3],
-                                       slices_per_thread=per)
+                                       slices_per_thread=per, reps=5 if info['nao'] <= 1000 else 3)
   print(json.dumps(out))
 
 

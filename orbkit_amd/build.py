@@ -13,7 +13,7 @@ OBJ = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'liborbkit_amd.so')
 INCLUDE = os.path.join(os.path.dirname(HERE), 'include')
 
-SOURCES = ['abi.cpp', 'ao_eval.hip', 'rho_gemm.hip', 'rho_gemm_m0.hip', 'rho_gemm_m1.hip', 'rho_fused.hip', 'rho_backward.hip', 'rho_big.hip', 'electrons.hip', 'extras.hip', 'cube.cpp', 'detci.hip']
+SOURCES = ['abi.cpp', 'ao_eval.hip', 'rho_gemm.hip', 'rho_gemm_m0.hip', 'rho_gemm_m1.hip', 'rho_fused.hip', 'rho_fused_f64.hip', 'rho_backward.hip', 'rho_big.hip', 'electrons.hip', 'extras.hip', 'cube.cpp', 'detci.hip']
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17', '-x', 'hip',
@@ -23,11 +23,14 @@ FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17', '-x', 'hip',
          '-munsafe-fp-atomics', '-I', INCLUDE, '-I', CSRC]
 
 # per-source additions:
-#   rho_fused.hip: its only atomic is a work counter taken by one lane, which
-#     the atomic optimizer would wrap in a wave-reduction sequence per claim;
-#   rho_gemm_m1.hip: FP64 MFMAs in the VGPR form (by default each K step
-#     copied the accumulators into AGPRs and back around every MFMA).
+#   rho_fused.hip, rho_fused_f64.hip: their only atomic is a work counter
+#     taken by one lane, which the atomic optimizer would wrap in a
+#     wave-reduction sequence per claim;
+#   rho_gemm_m1.hip, rho_fused_f64.hip: FP64 MFMAs in the VGPR form (by
+#     default each K step copied the accumulators into AGPRs and back around
+#     every MFMA).
 EXTRA = {'rho_fused.hip': ['-mllvm', '-amdgpu-atomic-optimizer-strategy=None'],
+         'rho_fused_f64.hip': ['-mllvm', '-amdgpu-atomic-optimizer-strategy=None', '-mllvm', '-amdgpu-mfma-vgpr-form'],
          'rho_gemm_m1.hip': ['-mllvm', '-amdgpu-mfma-vgpr-form']}
 
 

@@ -85,10 +85,12 @@ __device__ __forceinline__ f4 mfma_split(const Split<PREC>& A, const Split<PREC>
 namespace {
 
 // one entry of the padded operand planes: [plane][mpad][kpad] bf16 (hi and
-// mid rounded to nearest; nc = mpad * kpad entries per plane) or FP32
+// mid rounded to nearest; nc = mpad * kpad entries per plane), FP32 or FP64
 __device__ inline void store_c_split(double v, int prec, int64_t i, int64_t nc, void* out)
 {
-    if (prec == OKMI_PREC_F32) {
+    if (prec == OKMI_PREC_F64) {
+        ((double*)out)[i] = v;
+    } else if (prec == OKMI_PREC_F32) {
         ((float*)out)[i] = (float)v;
     } else {
         __bf16* o = (__bf16*)out;

@@ -1,8 +1,11 @@
-'''Reduced-precision fused density path (okmi_rho_fused).
+'''Fused density path (okmi_rho_fused): AO evaluation, MO contraction and
+density assembly in one kernel, the AO values never stored in HBM.
 
-``precision``: 'f32' (FP32 MFMA), 'bf16x3' (two-part bf16 split, 3 MFMA
-products) or 'bf16x6' (three-part split, 6 products); FP32 accumulation,
-FP64 density assembly.  See include/orbkit_amd.h.
+``precision``: 'f64' (FP64 AO values and FP64 MFMA, FP64 accumulation; up to
+64 MOs), 'f32' (FP32 MFMA), 'bf16x3' (two-part bf16 split, 3 MFMA products)
+or 'bf16x6' (three-part split, 6 products); the reduced precisions
+accumulate in FP32, every precision assembles the density in FP64.  See
+include/orbkit_amd.h.
 '''
 import ctypes
 
@@ -23,8 +26,8 @@ class FusedPlan:
 
   def __init__(self, basis, dgrid, coeffs, occ, codes, precision='bf16x6', p0=0, np_=None):
     import torch
-    if precision not in ('f32', 'bf16x3', 'bf16x6'):
-      raise ValueError('precision must be f32, bf16x3 or bf16x6, not %r' % (precision,))
+    if precision not in ('f64', 'f32', 'bf16x3', 'bf16x6'):
+      raise ValueError('precision must be f64, f32, bf16x3 or bf16x6, not %r' % (precision,))
     self.L = _native.lib()
     self.basis, self.dgrid = basis, dgrid
     self.prec = _native.PREC[precision]

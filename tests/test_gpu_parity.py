@@ -265,6 +265,42 @@ def test_slice_rho_direct(ok, drv):
   assert rel_err(rho, r[0]) <= TOL and rel_err(mo_norm, r[1]) <= TOL and rel_err(delta, r[2]) <= TOL
 
 
+@pytest.mark.parametrize('name,nmo,side', [('small', None, 21), ('medium', 5, 17), ('medium', 40, 19),
+                                           ('medium', 64, 13)])
+def test_fused_f64_vs_oracle(ok, name, nmo, side):
+  """FP64 fused kernel (AO values in FP64 in LDS, FP64 MFMA, FP64
+  accumulation) against the oracle at the FP64 bar: rho, the three first
+  derivatives and every second-derivative code (all four passes), on a
+  regular grid tiled in bricks (side not a multiple of 4) and on a vector
+  grid of ragged length; one to four 16-MO tiles."""
+  from orbkit_amd.basis import device_basis
+  from orbkit_amd.core import DeviceGrid
+  from orbkit_amd.fused import rho_fused
+  O = ok['O']
+  kw = {} if nmo is None else {'nmo': nmo}
+  qc, axes, _ = ok['W'].make(name, npts_side=side, **kw)
+  basis = device_basis(qc.geo_spec, qc.ao_spec)
+  C, occ = qc.mo_spec.get_coeffs(), qc.mo_spec.get_occ()
+  codes = [1, 2, 3, 4, 5, 6, 7, 8, 9]
+  drv = ['x', 'y', 'z', 'xx', 'yy', 'zz', 'xy', 'xz', 'yz']
+  x, y, z = _pts(axes, 2113, seed=5)
+  for regular in (True, False):
+    print('case fused f64', name, nmo, 'regular' if regular else 'vector', flush=True)
+    if regular:
+      dg = DeviceGrid(*axes, regular=True)
+      rx, ry, rz = O.expand(*axes)
+    else:
+      dg = DeviceGrid(x, y, z, regular=False)
+      rx, ry, rz = x, y, z
+    rho, d = rho_fused(basis, dg, C, occ, codes, precision='f64')
+    r_rho, r_d = O.rho_compute(qc, rx, ry, rz, True, drv=drv)
+    e_rho, e_d = rel_err(rho, r_rho), max(rel_err(d[i], r_d[i]) for i in range(len(drv)))
+    print(name, nmo, regular, e_rho, e_d)
+    assert e_rho <= TOL and e_d <= TOL, (regular, e_rho, e_d)
+    rho1, _ = rho_fused(basis, dg, C, occ, [], precision='f64')
+    assert rel_err(rho1, r_rho) <= TOL
+
+
 @pytest.mark.parametrize('prec,tol', [('f32', 1e-5), ('bf16x6', 1e-5), ('bf16x3', 1e-3)])
 def test_fused_precision(ok, prec, tol):
   '''Reduced-precision fused path vs the FP64 path: norm-wise relative error
