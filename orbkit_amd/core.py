@@ -25,7 +25,7 @@ from . import grid, _native
 from .basis import device_basis, spherical_matrix
 from .display import display
 from .qcinfo import QCinfo
-from .tools import validate_drv, require, convert, zeros, reshape
+from .tools import validate_drv, require, convert
 
 #: device bytes for one staged chunk of AO values (FP64 path)
 WORKSPACE_BYTES = 1 << 30

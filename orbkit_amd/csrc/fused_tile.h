@@ -738,7 +738,8 @@ int launch_pass(const FusedArgs& a, int pass, int mt, int lm, hipStream_t st)
     }
 }
 
-// FP64 instances (rho_fused_f64.hip, compiled with FP64 MFMAs in the VGPR form)
-int launch_pass_f64(const FusedArgs& a, int pass, int mt, int lm, hipStream_t st);
+// FP64 kernel (rho_fused_f64.hip, compiled with FP64 MFMAs in the VGPR form);
+// mpad: MOs padded to a multiple of 4, a.mpad the same
+int launch_pass_f64(const FusedArgs& a, int pass, int mpad, int lm, hipStream_t st);
 
 }  // namespace okmi_fused

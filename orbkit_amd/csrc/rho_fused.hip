@@ -56,22 +56,28 @@ int upload(T** d, const std::vector<T>& h, const char* what)
     return 0;
 }
 
-// Internal row layout of the fused kernels: shells in order, and a shell
-// that would cross a 32-row chunk boundary starts the next chunk, so every
-// chunk holds whole shells.  Returns the padded row count (-1 if a shell
-// has more than 32 rows) and each shell's first internal row.
+// Internal row layout of the fused kernels: every 32-row chunk holds whole
+// shells, packed from its first row.  Shells are taken in basis order, each
+// placed in the first chunk with room for it (first fit), so the chunks are
+// nearly full (every padding row still costs MFMA work) and the shells of a
+// chunk come from neighbouring atoms.  E.g. 20 atoms x [s,s,p,p,d,f]
+// (spherical, 400 rows): 13 chunks, where starting a new chunk at every
+// shell that does not fit took 14.  Returns the padded row count (-1 if a
+// shell has more than 32 rows) and each shell's first internal row.
 int fused_layout(const OkmiBasis* b, std::vector<int>* krow)
 {
-    int k = 0;
+    std::vector<int> fill;   // rows used per chunk
     if (krow) krow->assign(b->nshell, 0);
     for (int s = 0; s < b->nshell; ++s) {
         const int n = b->h_shells[s].nrow;
         if (n > KC) return -1;
-        if (k % KC + n > KC) k = (k + KC - 1) / KC * KC;
-        if (krow) (*krow)[s] = k;
-        k += n;
+        size_t c = 0;
+        while (c < fill.size() && fill[c] + n > KC) ++c;
+        if (c == fill.size()) fill.push_back(0);
+        if (krow) (*krow)[s] = (int)c * KC + fill[c];
+        fill[c] += n;
     }
-    return (k + KC - 1) / KC * KC;
+    return (int)fill.size() * KC;
 }
 
 size_t fused_fixed_bytes(const OkmiBasis* b, int nmo, int prec)
@@ -98,19 +104,18 @@ int prepare_fused_tables(OkmiBasis* b)
     const int nchunk = nkpad / KC;
     std::vector<int2> ch(nchunk);
     std::vector<int> work;
-    int s = 0;
+    std::vector<std::vector<std::pair<int, int>>> cost(nchunk);   // per chunk: (cost, shell)
+    std::vector<int> used(nchunk, 0);                              // rows used per chunk
+    for (int s = 0; s < b->nshell; ++s) {
+        const ShellRec& sh = b->h_shells[s];
+        const int c = krow[s] / KC;
+        cost[c].push_back({16 * sh.nprim + 8 * (ncart(sh.l) + sh.nrow), s});
+        used[c] = std::max(used[c], krow[s] + sh.nrow - c * KC);
+    }
     for (int c = 0; c < nchunk; ++c) {
-        std::vector<std::pair<int, int>> cost;
-        int used = 0;
-        while (s < b->nshell && krow[s] < (c + 1) * KC) {
-            const ShellRec& sh = b->h_shells[s];
-            cost.push_back({16 * sh.nprim + 8 * (ncart(sh.l) + sh.nrow), s});
-            used = krow[s] + sh.nrow - c * KC;
-            ++s;
-        }
-        std::sort(cost.begin(), cost.end(), [](auto& x, auto& y) { return x.first > y.first; });
-        ch[c] = make_int2((int)work.size(), (int)cost.size() | (used << 16));
-        for (auto& cs : cost) work.push_back(cs.second);
+        std::stable_sort(cost[c].begin(), cost[c].end(), [](auto& x, auto& y) { return x.first > y.first; });
+        ch[c] = make_int2((int)work.size(), (int)cost[c].size() | (used[c] << 16));
+        for (auto& cs : cost[c]) work.push_back(cs.second);
     }
     // shells in work-list order (entry j of the list is record j), each
     // with its primitives in groups of 4 (prim0: first group)
@@ -199,7 +204,9 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
     const bool brick = g.regular && p0 == 0 && np == g.nx * g.ny * g.nz && chunk >= np;
     if (chunk >= np) chunk = np;
     else chunk &= ~(int64_t)63;
-    const int64_t mpad = mt * 16, kpad = b->nkpad;
+    // MOs padded to 16 (the FP32-accumulating MFMA tiles) or, for FP64, to 4
+    // (16-MO tiles plus 4-MO groups)
+    const int64_t mpad = prec == OKMI_PREC_F64 ? (nmo + 3) / 4 * 4 : mt * 16, kpad = b->nkpad;
     char* w = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
     void* cpl = w;
     double* occp = (double*)(w + align256(c_elem_bytes(prec) * mpad * kpad));
@@ -240,7 +247,7 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
         a.rho = rho + c0;
         auto pass = [&](int ps) {
             switch (prec) {
-            case OKMI_PREC_F64: return launch_pass_f64(a, ps, mt, lm, st);
+            case OKMI_PREC_F64: return launch_pass_f64(a, ps, (int)mpad, lm, st);
             case OKMI_PREC_F32: return launch_pass<OKMI_PREC_F32>(a, ps, mt, lm, st);
             case OKMI_PREC_BF16X3: return launch_pass<OKMI_PREC_BF16X3>(a, ps, mt, lm, st);
             default: return launch_pass<OKMI_PREC_BF16X6>(a, ps, mt, lm, st);

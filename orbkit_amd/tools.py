@@ -6,19 +6,20 @@ Mirrors the parts of ``orbkit/tools.py`` that the density path reads:
 * ``exp`` (Cartesian component order)  orbkit/tools.py:117-135 (molden order)
 * ``cart2sph`` / ``get_cart2sph``      orbkit/tools.py:148-220
 * ``validate_drv``                     orbkit/tools.py:225-239
-* ``require/convert/zeros/reshape``    orbkit/tools.py:290-314
+* ``require`` / ``convert``            orbkit/tools.py:290-302 (array helpers)
 
 The numeric content of ``exp`` and ``cart2sph`` has to equal the reference's
 (they define which AO row is which and the real-solid-harmonic combinations),
 including the two g-shell rows whose norm is not one in the reference; the
 representation here is a per-(l, m) list of (lx, ly, lz, coefficient) terms.
 '''
-import string
 from math import sqrt
 
 import numpy
 
-orbit = 'spd' + string.ascii_lowercase[5:].replace('s', '').replace('p', '')
+#: shell letter of each angular momentum l = 0, 1, 2, ...: s, p, d, then the
+#: alphabet from f on without the letters already taken (s, p)
+orbit = 'spdfghijklmnoqrtuvwxyz'
 lquant = {c: i for i, c in enumerate(orbit)}
 
 
@@ -117,29 +118,17 @@ def validate_drv(drv):
   raise ValueError('The selection `drv=%s` is not valid!' % drv)
 
 
+_DTYPES = {'f': numpy.float64, 'i': numpy.intc}
+
+
 def require(data, dtype='f', requirements='CA'):
-  if dtype == 'f':
-    dtype = numpy.float64
-  elif dtype == 'i':
-    dtype = numpy.intc
-  return numpy.require(data, dtype=dtype, requirements=requirements)
+  """C-contiguous, aligned array of float64 ('f'), C int ('i') or any
+  numpy dtype, copied only when ``data`` is not one already."""
+  return numpy.require(data, dtype=_DTYPES.get(dtype, dtype), requirements=requirements)
 
 
 def convert(data, was_vector, N):
-  data = numpy.array(data, order='C')
-  if not was_vector:
-    data = data.reshape(data.shape[:-1] + tuple(N), order='C')
-  return data
-
-
-def zeros(shape, name, hdf5_file=None, chunks=True):
-  if hdf5_file is None:
-    return numpy.zeros(shape)
-  return hdf5_file.create_dataset(name, shape, dtype=numpy.float64, chunks=chunks)
-
-
-def reshape(data, shape, save_hdf5):
-  if not save_hdf5:
-    return data.reshape(shape)
-  data.attrs['shape'] = shape
-  return data[...].reshape(shape)
+  """A fresh C-ordered array; for a regular grid (``was_vector`` false) the
+  last axis is unfolded into the grid shape ``N``."""
+  out = numpy.array(data, order='C')
+  return out if was_vector else out.reshape(out.shape[:-1] + tuple(N))

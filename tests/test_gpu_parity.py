@@ -265,14 +265,16 @@ def test_slice_rho_direct(ok, drv):
   assert rel_err(rho, r[0]) <= TOL and rel_err(mo_norm, r[1]) <= TOL and rel_err(delta, r[2]) <= TOL
 
 
-@pytest.mark.parametrize('name,nmo,side', [('small', None, 21), ('medium', 5, 17), ('medium', 40, 19),
-                                           ('medium', 64, 13)])
+@pytest.mark.parametrize('name,nmo,side', [('small', None, 21), ('medium', 5, 17), ('medium', 12, 14),
+                                           ('medium', 20, 15), ('medium', 40, 19), ('medium', 64, 13)])
 def test_fused_f64_vs_oracle(ok, name, nmo, side):
   """FP64 fused kernel (AO values in FP64 in LDS, FP64 MFMA, FP64
   accumulation) against the oracle at the FP64 bar: rho, the three first
   derivatives and every second-derivative code (all four passes), on a
   regular grid tiled in bricks (side not a multiple of 4) and on a vector
-  grid of ragged length; one to four 16-MO tiles."""
+  grid of ragged length; MOs in 16-MO tiles (16x16x4 MFMA) and 4-MO groups
+  (4x4x4 MFMA): 5 -> 2 groups, 12 -> 3 groups, 20 -> 1 tile + 1 group,
+  40 -> 2 tiles + 2 groups, 64 -> 4 tiles."""
   from orbkit_amd.basis import device_basis
   from orbkit_amd.core import DeviceGrid
   from orbkit_amd.fused import rho_fused

@@ -25,6 +25,23 @@ def test_validate_drv_table():
       tools.validate_drv(bad)
 
 
+def test_cart2sph_matches_orbkit_table(golden):
+  """Every (l, m) row of the cart2sph table, l <= 4, equals orbkit's own
+  table (tests/golden/cart2sph_table.json, read from orbkit/tools.py by
+  make_cart2sph_fixture.py): the same Cartesian components in the same
+  order, the same coefficients with their signs and the same global factor.
+  Orthonormality checks alone would pass a row with its sign flipped."""
+  import json
+  rows = json.load(open(os.path.join(golden, 'cart2sph_table.json')))['rows']
+  assert len(rows) == sum(2 * l + 1 for l in range(5))
+  for r in rows:
+    exps, coeffs, glob = tools.get_cart2sph(r['l'], r['m'])
+    assert [tuple(e) for e in exps] == [tuple(e) for e in r['exps']], (r['l'], r['m'])
+    assert numpy.allclose(coeffs, r['coeffs'], rtol=1e-15, atol=0), (r['l'], r['m'])
+    assert numpy.sign(coeffs).tolist() == numpy.sign(r['coeffs']).tolist()
+    assert abs(glob - r['global']) <= 1e-15 * abs(r['global']), (r['l'], r['m'])
+
+
 def test_readers(golden):
   qc = read.main_read(os.path.join(golden, 'h2o_rhf_sph.molden'), all_mo=True)
   assert not qc.ao_spec.spherical and qc.ao_spec.get_ao_num() == 25 and len(qc.mo_spec) == 24
