@@ -196,10 +196,11 @@ def density_leg(args, torch, info, basis, dg, C, occ, codes, path):
     st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
     def step():
-      _native.check(L.okmi_rho(basis.handle, ctypes.byref(dg.desc), 0, npts, ctypes.c_void_p(dC.data_ptr()),
-                               ctypes.c_void_p(docc.data_ptr()), nmo, codes.ctypes.data_as(_native.P_int), len(codes),
-                               ctypes.c_void_p(rho.data_ptr()), ctypes.c_void_p(delta.data_ptr()), npts, None,
-                               ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, st), 'okmi_rho')
+      _native.check(L.okmi_rho_staged(basis.handle, ctypes.byref(dg.desc), 0, npts, ctypes.c_void_p(dC.data_ptr()),
+                                      ctypes.c_void_p(docc.data_ptr()), nmo, codes.ctypes.data_as(_native.P_int),
+                                      len(codes), ctypes.c_void_p(rho.data_ptr()), ctypes.c_void_p(delta.data_ptr()),
+                                      npts, None, ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, st),
+                    'okmi_rho_staged')
     names, result = ['ao_eval', 'rho_f64'], (lambda: (rho.cpu().numpy(), delta.cpu().numpy()))
     dom = ('rho_f64', 'k_rho_f64', 'f64', 1)
   else:
@@ -275,7 +276,7 @@ def density_step(args, torch, info, qc, axes):
   ref = None
   if args.precision != 'f64' or legs or args.errors:
     from orbkit_amd.core import eval_rho
-    rr, dd, _ = eval_rho(basis, dg, C, occ, codes)   # staged FP64 path
+    rr, dd, _ = eval_rho(basis, dg, C, occ, codes, staged=True)   # the staged FP64 path
     ref = (rr, dd)
   if ref is not None:
     r, d = head_result()

@@ -117,11 +117,28 @@ int okmi_mo_eval(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np
  *                     (orbkit/core.py:280-304); row stride ld_delta
  *   d_mo_norm[i]   += sum_p phi_i(p)^2 when non-NULL (orbkit/core.py:266-274)
  * Replaces slice_rho's density branch (orbkit/core.py:179-308).
+ * Up to 64 MOs, and when the workspace holds at least
+ * okmi_rho_fused_workspace_bytes(b, nmo, OKMI_PREC_F64, 64) bytes, it runs
+ * the fused FP64 kernel (AO values in FP64 in LDS, FP64 MFMA; points in
+ * passes as large as the workspace allows); otherwise okmi_rho_staged.  The
+ * basis's fused-kernel tables are built on first use.
  */
 int okmi_rho(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np,
              const double* d_c, const double* d_occ, int nmo,
              const int* drv_out, int nout, double* d_rho, double* d_delta, int64_t ld_delta,
              double* d_mo_norm, void* d_ws, size_t ws_bytes, void* stream);
+
+/*
+ * The same as okmi_rho in orbkit's own two stages, any number of MOs: the
+ * FP64 AO sets of a chunk of points written to the workspace (okmi_ao_eval's
+ * kernel, the workspace sized by okmi_workspace_bytes), then the FP64 MFMA
+ * contraction with the density assembled in its epilogue.  Replaces the same
+ * code as okmi_rho (orbkit/core.py:179-308, orbkit/cy_core.pyx:49-101).
+ */
+int okmi_rho_staged(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np,
+                    const double* d_c, const double* d_occ, int nmo,
+                    const int* drv_out, int nout, double* d_rho, double* d_delta, int64_t ld_delta,
+                    double* d_mo_norm, void* d_ws, size_t ws_bytes, void* stream);
 
 /*
  * Workspace bytes okmi_rho_fused needs to process `chunk` points per pass

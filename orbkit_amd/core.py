@@ -133,9 +133,11 @@ def eval_mos(basis, dgrid, codes, coeffs, p0=0, np_=None):
   return out
 
 
-def eval_rho(basis, dgrid, coeffs, occ, codes, p0=0, np_=None, mo_norm=False):
+def eval_rho(basis, dgrid, coeffs, occ, codes, p0=0, np_=None, mo_norm=False, staged=False):
   '''Density (np,), derivative outputs (ncodes, np) and optionally the MO
-  norms sum_p phi^2 (nmo,), all FP64.'''
+  norms sum_p phi^2 (nmo,), all FP64: okmi_rho (the fused FP64 kernel up to
+  64 MOs), or with ``staged`` okmi_rho_staged (AO sets through device memory,
+  then the FP64 GEMM).'''
   torch = _torch()
   L = _native.lib()
   np_ = dgrid.npts - p0 if np_ is None else np_
@@ -155,11 +157,12 @@ def eval_rho(basis, dgrid, coeffs, occ, codes, p0=0, np_=None, mo_norm=False):
   ws_chunk = max(64, min(np_, WORKSPACE_BYTES // max(1, nsets * basis.nao * 8)))
   ws = torch.empty(L.okmi_workspace_bytes(basis.handle, nsets, ws_chunk) // 8 + 1,
                    dtype=torch.float64, device=dev)
-  _native.check(L.okmi_rho(basis.handle, ctypes.byref(dgrid.desc), p0, np_, ctypes.c_void_p(dC.data_ptr()),
-                           ctypes.c_void_p(docc.data_ptr()), nmo, codes.ctypes.data_as(_native.P_int), nd,
-                           ctypes.c_void_p(rho.data_ptr()), ctypes.c_void_p(delta.data_ptr()),
-                           delta.shape[1], ctypes.c_void_p(nrm.data_ptr()) if mo_norm else None,
-                           ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream()), 'okmi_rho')
+  fn = L.okmi_rho_staged if staged else L.okmi_rho
+  _native.check(fn(basis.handle, ctypes.byref(dgrid.desc), p0, np_, ctypes.c_void_p(dC.data_ptr()),
+                   ctypes.c_void_p(docc.data_ptr()), nmo, codes.ctypes.data_as(_native.P_int), nd,
+                   ctypes.c_void_p(rho.data_ptr()), ctypes.c_void_p(delta.data_ptr()),
+                   delta.shape[1], ctypes.c_void_p(nrm.data_ptr()) if mo_norm else None,
+                   ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream()), 'okmi_rho')
   r = rho[:np_].cpu().numpy()
   d = delta[:nd, :np_].cpu().numpy()
   return r, d, (nrm.cpu().numpy() if mo_norm else None)

@@ -464,9 +464,10 @@ int okmi_rho_sets(const int* drv_out, int nout, int* codes, int* ncodes)
     return 0;
 }
 
-extern "C" int okmi_rho(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np, const double* d_c,
-                        const double* d_occ, int nmo, const int* drv_out, int nout, double* d_rho, double* d_delta,
-                        int64_t ld_delta, double* d_mo_norm, void* d_ws, size_t ws_bytes, void* stream)
+extern "C" int okmi_rho_staged(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np, const double* d_c,
+                               const double* d_occ, int nmo, const int* drv_out, int nout, double* d_rho,
+                               double* d_delta, int64_t ld_delta, double* d_mo_norm, void* d_ws, size_t ws_bytes,
+                               void* stream)
 {
     g_err.clear();
     if (!b || !d_c || !d_occ || !d_rho || nmo <= 0 || (nout > 0 && (!d_delta || ld_delta < np))) {
@@ -504,6 +505,32 @@ extern "C" int okmi_rho(const okmi_basis* b, const okmi_grid* g, int64_t p0, int
     return 0;
 }
 
+extern "C" int okmi_rho(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np, const double* d_c,
+                        const double* d_occ, int nmo, const int* drv_out, int nout, double* d_rho, double* d_delta,
+                        int64_t ld_delta, double* d_mo_norm, void* d_ws, size_t ws_bytes, void* stream)
+{
+    g_err.clear();
+    if (!b || !d_c || !d_occ || !d_rho || nmo <= 0 || (nout > 0 && (!d_delta || ld_delta < np))) {
+        okmi_set_error("rho: invalid arguments");
+        return 1;
+    }
+    if (check_grid(g, p0, np)) return 1;
+    int codes[OKMI_MAX_SETS];
+    int nsets = 0;
+    if (okmi_rho_sets(drv_out, nout, codes, &nsets)) return 1;
+    if (np == 0) return 0;
+    // Up to 64 MOs: the fused FP64 kernel (same arithmetic type, no HBM round
+    // trip of the AO sets) when the workspace holds its buffers for at least
+    // one 64-point tile; otherwise the staged path.  The fused tables are a
+    // cache built once per basis (the basis is not otherwise changed).
+    okmi_basis* mb = const_cast<okmi_basis*>(b);
+    if (nmo <= 64 && d_ws && ws_bytes >= okmi_fused_ws_bytes(mb, nmo, OKMI_PREC_F64, 64))
+        return okmi_launch_rho_fused(mb, to_desc(g), p0, np, d_c, d_occ, nmo, OKMI_PREC_F64, drv_out, nout, d_rho,
+                                     d_delta, ld_delta, d_mo_norm, d_ws, ws_bytes, (hipStream_t)stream);
+    return okmi_rho_staged(b, g, p0, np, d_c, d_occ, nmo, drv_out, nout, d_rho, d_delta, ld_delta, d_mo_norm, d_ws,
+                           ws_bytes, stream);
+}
+
 extern "C" size_t okmi_rho_fused_workspace_bytes(const okmi_basis* b, int nmo, int precision, int64_t chunk)
 {
     if (!b || nmo <= 0 || chunk <= 0) return 0;
@@ -522,7 +549,7 @@ extern "C" int okmi_rho_fused(okmi_basis* b, const okmi_grid* g, int64_t p0, int
     }
     if (check_grid(g, p0, np)) return 1;
     return okmi_launch_rho_fused(b, to_desc(g), p0, np, d_c, d_occ, nmo, precision, drv_out, nout, d_rho, d_delta,
-                                 ld_delta, d_ws, ws_bytes, (hipStream_t)stream);
+                                 ld_delta, nullptr, d_ws, ws_bytes, (hipStream_t)stream);
 }
 
 extern "C" size_t okmi_rho_backward_workspace_bytes(const okmi_basis* b, int nmo, int64_t chunk)

@@ -91,6 +91,7 @@ struct FusedArgs {
     int tlx, tly, tlz;        // brick extents 2^tlx x 2^tly x 2^tlz (tlx + tly + tlz = 6)
     int tl_pad;
     int64_t nbx, nby, nbz;    // bricks per axis
+    double* norm_part;        // FP64 kernel: per tile sum_p Phi_m(p)^2, [tile][mpad] (null: not wanted)
 };
 
 template <int PASS> struct PassCodes;
@@ -741,5 +742,7 @@ int launch_pass(const FusedArgs& a, int pass, int mt, int lm, hipStream_t st)
 // FP64 kernel (rho_fused_f64.hip, compiled with FP64 MFMAs in the VGPR form);
 // mpad: MOs padded to a multiple of 4, a.mpad the same
 int launch_pass_f64(const FusedArgs& a, int pass, int mpad, int lm, hipStream_t st);
+// mo_norm[m] += sum_t part[t][m] for m < nmo (the FP64 kernel's per-tile norms)
+int launch_norm_reduce(const double* part, int64_t ntile, int mpad, int nmo, double* mo_norm, hipStream_t st);
 
 }  // namespace okmi_fused

@@ -79,9 +79,11 @@ int okmi_rho_sets(const int* drv_out, int nout, int* codes, int* ncodes);
 
 // fused AO -> MFMA -> density (reduced-precision contraction)
 size_t okmi_fused_ws_bytes(const OkmiBasis* b, int nmo, int prec, int64_t np);
+// (mo_norm, FP64 only: += sum_p phi_i(p)^2, or null)
 int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const double* d_c,
                           const double* d_occ, int nmo, int prec, const int* drv_out, int nout, double* rho,
-                          double* delta, int64_t ld_delta, void* ws, size_t ws_bytes, hipStream_t st);
+                          double* delta, int64_t ld_delta, double* mo_norm, void* ws, size_t ws_bytes,
+                          hipStream_t st);
 
 // many-MO density: FP32 AO sets per chunk + MFMA GEMM with density epilogue
 // (S AO sets: 1, 4, 7 or 10, the smallest holding the requested codes)

@@ -164,17 +164,28 @@ int prepare_fused_tables(OkmiBasis* b)
 }
 }  // namespace
 
+// Per point: up to 20 FP64 term rows; FP64 with MO norms also one row of
+// per-tile norms per 64 points (at most 64 MOs: 8 bytes per point)
+constexpr size_t FUSED_PT_BYTES = 20 * 8, NORM_PT_BYTES = 8;
+
 size_t okmi_fused_ws_bytes(const OkmiBasis* b, int nmo, int prec, int64_t np)
 {
     if (nmo > 64) return okmi_big_ws_bytes(b, nmo, prec, 7, np);
-    return fused_fixed_bytes(b, nmo, prec) + align256((size_t)20 * np * 8) + 256;
+    size_t n = fused_fixed_bytes(b, nmo, prec) + align256(FUSED_PT_BYTES * np) + 256;
+    if (prec == OKMI_PREC_F64) n += align256(NORM_PT_BYTES * (np + 64)) + 256;
+    return n;
 }
 
 int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const double* d_c,
                           const double* d_occ, int nmo, int prec, const int* drv_out, int nout, double* rho,
-                          double* delta, int64_t ld_delta, void* ws, size_t ws_bytes, hipStream_t st)
+                          double* delta, int64_t ld_delta, double* mo_norm, void* ws, size_t ws_bytes,
+                          hipStream_t st)
 {
     if (np <= 0) return 0;
+    if (mo_norm && prec != OKMI_PREC_F64) {
+        okmi_set_error("rho_fused: MO norms are computed by the FP64 kernel only");
+        return 1;
+    }
     if (prec != OKMI_PREC_F64 && prec != OKMI_PREC_F32 && prec != OKMI_PREC_BF16X3 && prec != OKMI_PREC_BF16X6) {
         okmi_set_error("rho_fused: unknown precision %d", prec);
         return 1;
@@ -193,25 +204,33 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
     }
     int rc = prepare_fused_tables(b);
     if (rc) return rc;
-    const size_t fixed = fused_fixed_bytes(b, nmo, prec);
-    if (!ws || ws_bytes < fixed + 256 + 20 * 8 * 64) {
-        okmi_set_error("rho_fused: workspace of %zu bytes is too small (need at least %zu)", ws_bytes,
-                       fixed + 256 + 20 * 8 * 64);
-        return 1;
-    }
-    int64_t chunk = (int64_t)((ws_bytes - fixed - 256) / (20 * 8));
-    // a whole regular grid whose term buffers fit is tiled in 4x4x4 bricks
-    const bool brick = g.regular && p0 == 0 && np == g.nx * g.ny * g.nz && chunk >= np;
-    if (chunk >= np) chunk = np;
-    else chunk &= ~(int64_t)63;
     // MOs padded to 16 (the FP32-accumulating MFMA tiles) or, for FP64, to 4
     // (16-MO tiles plus 4-MO groups)
     const int64_t mpad = prec == OKMI_PREC_F64 ? (nmo + 3) / 4 * 4 : mt * 16, kpad = b->nkpad;
+    const size_t fixed = fused_fixed_bytes(b, nmo, prec);
+    // per point: the term rows, plus the per-tile MO norms (mpad doubles per
+    // 64-point tile) when wanted
+    const size_t per_pt = FUSED_PT_BYTES + (mo_norm ? NORM_PT_BYTES : 0);
+    const size_t slack = 256 + (mo_norm ? 256 + (size_t)mpad * 8 : 0);
+    if (!ws || ws_bytes < fixed + slack + per_pt * 64) {
+        okmi_set_error("rho_fused: workspace of %zu bytes is too small (need at least %zu)", ws_bytes,
+                       fixed + slack + per_pt * 64);
+        return 1;
+    }
+    const size_t avail = ws_bytes - fixed - slack;
+    // a whole regular grid whose buffers fit is tiled in 4x4x4 bricks (the
+    // faces may add tiles beyond np / 64)
+    const int64_t nbrick = ((g.nx + 3) / 4) * ((g.ny + 3) / 4) * ((g.nz + 3) / 4);
+    const bool brick = g.regular && p0 == 0 && np == g.nx * g.ny * g.nz &&
+                       FUSED_PT_BYTES * (size_t)np + (mo_norm ? (size_t)nbrick * mpad * 8 : 0) <= avail;
+    int64_t chunk = brick ? np : std::min<int64_t>(np, (int64_t)(avail / per_pt) & ~(int64_t)63);
+    if (chunk <= 0) chunk = np;   // unreachable: avail holds 64 points
     char* w = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
     void* cpl = w;
     double* occp = (double*)(w + align256(c_elem_bytes(prec) * mpad * kpad));
     double* term = (double*)(w + fixed);
     double* term2 = term + 10 * chunk;
+    double* norm_part = mo_norm ? (double*)(w + fixed + align256(FUSED_PT_BYTES * (size_t)chunk)) : nullptr;
 
     hipLaunchKernelGGL(k_prep_c, dim3((unsigned)((mpad * kpad + 255) / 256)), dim3(256), 0, st, d_c, nmo, b->nao,
                        prec, mpad, kpad, (const int*)b->d_colmap, cpl);
@@ -253,9 +272,12 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
             default: return launch_pass<OKMI_PREC_BF16X6>(a, ps, mt, lm, st);
             }
         };
+        a.norm_part = norm_part;   // read by the first pass (rho) only
         int r = pass(p1 ? 1 : 0);
         if (!r && p2) r = pass(2);
         if (!r && p3) r = pass(3);
+        if (!r && mo_norm) r = launch_norm_reduce(norm_part, brick ? nbrick : (n + PTS - 1) / PTS, (int)mpad, nmo,
+                                                  mo_norm, st);
         if (r) return r;
         if (nout > 0)
             hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rho + c0, term, term2,

@@ -79,6 +79,36 @@ __device__ __forceinline__ void contract_f64(const double* L, const double* cs, 
     }
 }
 
+// Per-tile MO norms sum_p Phi_m(p)^2 over the tile's points (padding points
+// excluded): column sums over the 16 lanes of each MO, the four columns
+// added through LDS (red: 4 x MP doubles, free after the chunk loop's last
+// barrier), one row of norm_part per tile.  Called by every thread.
+template <int PASS, int MT, int MR>
+__device__ __forceinline__ void tile_norms(const FusedArgs& a, int64_t tile, int col, int lane, bool act, double* red,
+                                           const d4 (&acc)[MoSplit<MT, MR>::T][PassCodes<PASS>::n],
+                                           const double (&accr)[MoSplit<MT, MR>::R][PassCodes<PASS>::n])
+{
+    constexpr int MP = MoSplit<MT, MR>::MP;
+    const int g = lane >> 4, n16 = lane & 15;
+    auto put = [&](int m, double f) {
+        double v = act ? f * f : 0.0;
+        v += __shfl_xor(v, 1);
+        v += __shfl_xor(v, 2);
+        v += __shfl_xor(v, 4);
+        v += __shfl_xor(v, 8);
+        if (n16 == 0) red[col * MP + m] = v;
+    };
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) put(16 * mt + 4 * i + g, acc[mt][0][i]);
+#pragma unroll
+    for (int r = 0; r < MR; ++r) put(16 * MT + 4 * r + g, accr[r][0]);
+    __syncthreads();
+    const int m = threadIdx.x;
+    if (m < MP) a.norm_part[tile * MP + m] = (red[m] + red[MP + m]) + (red[2 * MP + m] + red[3 * MP + m]);
+}
+
 // Density terms of the column from its MO values: item i of acc[mt][s] is
 // Phi_s[16 mt + 4 i + g][point n16], accr[r][s] is Phi_s[16 MT + 4 r + g][point n16]
 template <int PASS, int MT, int MR>
@@ -221,6 +251,33 @@ __global__ __launch_bounds__(256, 2) void k_rho_fused_f64(FusedArgs a)
         __syncthreads();
     }
     epilogue_f64<PASS, MT, MR>(a, tile, wave, lane, acc, accr);
+    if constexpr (PASS <= 1) {
+        if (a.norm_part) {   // uniform: every thread takes the branch or none
+            int64_t p;
+            bool act;
+            double qx, qy, qz;
+            tile_point(a, tile, wave * 16 + n16, p, act, qx, qy, qz);
+            tile_norms<PASS, MT, MR>(a, tile, wave, lane, act, lds, acc, accr);
+        }
+    }
+}
+
+// mo_norm[m] += sum over tiles of part[tile][m], m < nmo: one workgroup per
+// MO, each thread a fixed stride of tiles, then a tree in LDS (the order of
+// the additions does not depend on timing)
+__global__ void k_norm_reduce(const double* part, int64_t ntile, int mpad, int nmo, double* mo_norm)
+{
+    __shared__ double red[256];
+    const int m = blockIdx.x;
+    double s = 0.0;
+    for (int64_t t = threadIdx.x; t < ntile; t += 256) s += part[t * mpad + m];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && m < nmo) mo_norm[m] += red[0];
 }
 
 template <int PASS, int LM, int MT, int MR>
@@ -257,6 +314,13 @@ int launch_f64_m(const FusedArgs& a, int mpad, hipStream_t st)
 }
 
 }  // namespace
+
+int launch_norm_reduce(const double* part, int64_t ntile, int mpad, int nmo, double* mo_norm, hipStream_t st)
+{
+    if (ntile <= 0 || nmo <= 0) return 0;
+    hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)nmo), dim3(256), 0, st, part, ntile, mpad, nmo, mo_norm);
+    return 0;
+}
 
 int launch_pass_f64(const FusedArgs& a, int pass, int mpad, int lm, hipStream_t st)
 {

@@ -265,6 +265,59 @@ def test_slice_rho_direct(ok, drv):
   assert rel_err(rho, r[0]) <= TOL and rel_err(mo_norm, r[1]) <= TOL and rel_err(delta, r[2]) <= TOL
 
 
+@pytest.mark.parametrize('regular,ws_points', [(True, None), (False, None), (False, 256), (True, 320)])
+def test_okmi_rho_fused_and_staged(ok, regular, ws_points):
+  """okmi_rho (fused FP64 kernel up to 64 MOs) against okmi_rho_staged and
+  the oracle: rho, first and second derivatives and the MO norms, on a
+  regular grid (bricks) and a vector grid, with a workspace that holds the
+  whole grid or only a few hundred points (several passes, the MO norms
+  accumulated pass by pass; a regular grid then runs as vector passes)."""
+  import ctypes
+  import torch
+  from orbkit_amd import _native
+  from orbkit_amd.basis import device_basis
+  from orbkit_amd.core import DeviceGrid
+  O = ok['O']
+  qc, axes, _ = ok['W'].make('medium', npts_side=13, nmo=20)
+  basis = device_basis(qc.geo_spec, qc.ao_spec)
+  C, occ = qc.mo_spec.get_coeffs(), qc.mo_spec.get_occ()
+  if regular:
+    dg = DeviceGrid(*axes, regular=True)
+    x, y, z = O.expand(*axes)
+  else:
+    x, y, z = _pts(axes, 1777, seed=7)
+    dg = DeviceGrid(x, y, z, regular=False)
+  codes = numpy.array([1, 2, 3, 4, 6, 8], dtype=numpy.intc)
+  L, dev = _native.lib(), dg.x.device
+  dC, docc = torch.as_tensor(C, device=dev), torch.as_tensor(occ, device=dev)
+  st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+  out = {}
+  for name in ('okmi_rho', 'okmi_rho_staged'):
+    print('case', name, 'regular' if regular else 'vector', ws_points, flush=True)
+    if name == 'okmi_rho' and ws_points is not None:
+      nb = L.okmi_rho_fused_workspace_bytes(basis.handle, C.shape[0], 0, ws_points)
+    else:
+      nb = L.okmi_workspace_bytes(basis.handle, 7, dg.npts if ws_points is None else ws_points)
+    ws = torch.empty(nb // 8 + 1, dtype=torch.float64, device=dev)
+    rho = torch.empty(dg.npts, dtype=torch.float64, device=dev)
+    delta = torch.empty((len(codes), dg.npts), dtype=torch.float64, device=dev)
+    nrm = torch.full((C.shape[0],), 0.5, dtype=torch.float64, device=dev)   # accumulated onto
+    _native.check(getattr(L, name)(basis.handle, ctypes.byref(dg.desc), 0, dg.npts, ctypes.c_void_p(dC.data_ptr()),
+                                   ctypes.c_void_p(docc.data_ptr()), C.shape[0], codes.ctypes.data_as(_native.P_int),
+                                   len(codes), ctypes.c_void_p(rho.data_ptr()), ctypes.c_void_p(delta.data_ptr()),
+                                   dg.npts, ctypes.c_void_p(nrm.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
+                                   nb, st), name)
+    out[name] = (rho.cpu().numpy(), delta.cpu().numpy(), nrm.cpu().numpy() - 0.5)
+  r = O.slice_rho(qc.geo_spec, qc.ao_spec, qc.mo_spec, x, y, z, ['x', 'y', 'z', 'xx', 'zz', 'xz'], False, False,
+                  ok['tools'].get_cart2sph)
+  for name, (rho, d, nrm) in out.items():
+    e = (rel_err(rho, r[0]), rel_err(d, r[2]), rel_err(nrm, r[1]))
+    print(name, e)
+    assert max(e) <= TOL, (name, e)
+  f, s_ = out['okmi_rho'], out['okmi_rho_staged']
+  assert rel_err(f[0], s_[0]) <= TOL and rel_err(f[1], s_[1]) <= TOL and rel_err(f[2], s_[2]) <= TOL
+
+
 @pytest.mark.parametrize('name,nmo,side', [('small', None, 21), ('medium', 5, 17), ('medium', 12, 14),
                                            ('medium', 20, 15), ('medium', 40, 19), ('medium', 64, 13)])
 def test_fused_f64_vs_oracle(ok, name, nmo, side):
