@@ -129,6 +129,25 @@ int okmi_rho(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np,
              double* d_mo_norm, void* d_ws, size_t ws_bytes, void* stream);
 
 /*
+ * okmi_rho without a workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  It
+ * calls no allocating routine: its workspace is a device array of
+ * okmi_rho_noalloc_scratch_bytes() (64 MiB) held by each GPU the library
+ * runs on until the process ends, and the basis's fused tables are built by
+ * okmi_basis_create.  Requests whose fixed buffers plus one 64-point pass
+ * exceed the scratch (MO coefficients of very large bases) return an error;
+ * larger grids run in several passes.  Calls that overlap on one GPU share
+ * the scratch and race: serialise them (one stream per GPU, or wait).
+ * Kernel timing (okmi_profile_enable) is not recorded inside this call.
+ */
+size_t okmi_rho_noalloc_scratch_bytes(void);
+int okmi_rho_noalloc(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np,
+                     const double* d_c, const double* d_occ, int nmo,
+                     const int* drv_out, int nout, double* d_rho, double* d_delta, int64_t ld_delta,
+                     double* d_mo_norm, void* stream);
+
+/*
  * The same as okmi_rho in orbkit's own two stages, any number of MOs: the
  * FP64 AO sets of a chunk of points written to the workspace (okmi_ao_eval's
  * kernel, the workspace sized by okmi_workspace_bytes), then the FP64 MFMA

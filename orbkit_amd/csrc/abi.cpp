@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -165,11 +166,12 @@ struct ProfEvent {
 };
 bool g_prof = false;
 std::vector<ProfEvent> g_prof_events;
+thread_local bool g_no_alloc = false;   // inside okmi_rho_noalloc: no event creation either
 }  // namespace
 
 OkmiProfScope::OkmiProfScope(const char* name, hipStream_t s) : st(s)
 {
-    if (!g_prof) return;
+    if (!g_prof || g_no_alloc) return;
     ProfEvent e{name, nullptr, nullptr};
     if (hipEventCreate(&e.start) != hipSuccess || hipEventCreate(&e.stop) != hipSuccess) return;
     (void)hipEventRecord(e.start, st);
@@ -328,6 +330,10 @@ extern "C" int okmi_basis_create(int ncont, const int* ncart_per_cont, const int
     int rc = dev_upload(&b->d_shells, shells.data(), shells.size(), "basis_create: shells");
     if (!rc) rc = dev_upload(&b->d_prims, prims.data(), prims.size(), "basis_create: primitives");
     if (!rc) rc = dev_upload(&b->d_T, T.data(), T.size(), "basis_create: transform");
+    // the fused kernels' tables (chunk layout, work lists), built here so
+    // that no later call has to allocate device memory
+    for (int i = 0; i < ncont; ++i) b->max_nrow = std::max(b->max_nrow, shells[i].nrow);
+    if (!rc && b->max_nrow <= 32) rc = okmi_prepare_fused(b);
     if (rc) {
         okmi_basis_destroy(b);
         return rc;
@@ -524,11 +530,84 @@ extern "C" int okmi_rho(const okmi_basis* b, const okmi_grid* g, int64_t p0, int
     // one 64-point tile; otherwise the staged path.  The fused tables are a
     // cache built once per basis (the basis is not otherwise changed).
     okmi_basis* mb = const_cast<okmi_basis*>(b);
-    if (nmo <= 64 && d_ws && ws_bytes >= okmi_fused_ws_bytes(mb, nmo, OKMI_PREC_F64, 64))
+    if (nmo <= 64 && b->max_nrow <= 32 && d_ws && ws_bytes >= okmi_fused_ws_bytes(mb, nmo, OKMI_PREC_F64, 64))
         return okmi_launch_rho_fused(mb, to_desc(g), p0, np, d_c, d_occ, nmo, OKMI_PREC_F64, drv_out, nout, d_rho,
                                      d_delta, ld_delta, d_mo_norm, d_ws, ws_bytes, (hipStream_t)stream);
     return okmi_rho_staged(b, g, p0, np, d_c, d_occ, nmo, drv_out, nout, d_rho, d_delta, ld_delta, d_mo_norm, d_ws,
                            ws_bytes, stream);
+}
+
+// ---------------------------------------------------------------------------
+// okmi_rho for callers that cannot allocate device memory.  Its workspace is
+// `scratch`, a file-scope device array of OKMI_NOALLOC_SCRATCH_BYTES (each GPU
+// the library runs on holds its own copy until the process ends).  In host
+// code the array's name is not a device address: the address of the current
+// GPU's copy comes from hipGetSymbolAddress, taken on the first call on that
+// GPU (with it current) and kept per device.  Every buffer the kernels use in
+// it is written before it is read (C planes, occupations, term rows, per-tile
+// norms, or the staged path's AO sets), so nothing is cleared; calls that
+// overlap on one GPU share the array and race.
+constexpr size_t OKMI_NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[OKMI_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int OKMI_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[OKMI_MAX_DEVICES] = {};
+
+int scratch_address(void** out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return okmi_hip_fail(e, "rho_noalloc: current device");
+    if (dev < 0 || dev >= OKMI_MAX_DEVICES) {
+        okmi_set_error("rho_noalloc: device %d (up to %d devices)", dev, OKMI_MAX_DEVICES);
+        return 1;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return okmi_hip_fail(e, "rho_noalloc: scratch address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" size_t okmi_rho_noalloc_scratch_bytes(void) { return OKMI_NOALLOC_SCRATCH_BYTES; }
+
+extern "C" int okmi_rho_noalloc(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np, const double* d_c,
+                                const double* d_occ, int nmo, const int* drv_out, int nout, double* d_rho,
+                                double* d_delta, int64_t ld_delta, double* d_mo_norm, void* stream)
+{
+    g_err.clear();
+    if (!b || !d_c || !d_occ || !d_rho || nmo <= 0 || (nout > 0 && (!d_delta || ld_delta < np))) {
+        okmi_set_error("rho_noalloc: invalid arguments");
+        return 1;
+    }
+    int codes[OKMI_MAX_SETS];
+    int nsets = 0;
+    if (okmi_rho_sets(drv_out, nout, codes, &nsets)) return 1;
+    // the scratch must hold the fixed buffers and one 64-point pass of the
+    // path okmi_rho takes
+    const bool fused = nmo <= 64 && b->max_nrow <= 32;
+    const size_t need = fused ? okmi_fused_ws_bytes(const_cast<okmi_basis*>(b), nmo, OKMI_PREC_F64, 64)
+                              : okmi_workspace_bytes(b, nsets, 64);
+    if (need > OKMI_NOALLOC_SCRATCH_BYTES) {
+        okmi_set_error("rho_noalloc: %d MOs x %d AOs need %zu bytes of scratch for one 64-point pass "
+                       "(the scratch holds %zu); use okmi_rho with a workspace",
+                       nmo, b->nao, need, OKMI_NOALLOC_SCRATCH_BYTES);
+        return 1;
+    }
+    void* ws = nullptr;
+    if (scratch_address(&ws)) return 1;
+    g_no_alloc = true;
+    const int rc = okmi_rho(b, g, p0, np, d_c, d_occ, nmo, drv_out, nout, d_rho, d_delta, ld_delta, d_mo_norm, ws,
+                            OKMI_NOALLOC_SCRATCH_BYTES, stream);
+    g_no_alloc = false;
+    return rc;
 }
 
 extern "C" size_t okmi_rho_fused_workspace_bytes(const okmi_basis* b, int nmo, int precision, int64_t chunk)

@@ -39,6 +39,7 @@ struct OkmiBasis {
     int nkpad = 0;
     std::vector<double> h_frcut2, h_fprcut2;
     int is_normalized = 0;
+    int max_nrow = 0;      // most output rows of one shell (the fused kernels take up to 32)
 };
 
 void okmi_set_error(const char* fmt, ...);
