@@ -209,3 +209,88 @@ def test_g_shells_and_many_tiles(mols):
   r, d = core.rho_compute(qc, drv=['x', 'y', 'z'], precision='bf16x6')
   assert numpy.linalg.norm(r - r64) / numpy.linalg.norm(r64) <= 1e-5
   assert numpy.linalg.norm(d - d64) / numpy.linalg.norm(d64) <= 1e-5
+
+
+# --------------------------------------------------------------------------
+# The batched workload at its real size: 256 molecules of 5-60 atoms
+# ([s,s,p,d] spherical, 8-30 MOs), regular grids with sides 32-80 (5.5e7
+# points), one persistent launch replayed from a HIP graph.  Each molecule is
+# checked against the oracle (orbkit's algorithm, rho_compute once per
+# molecule: orbkit/core.py:314-605) on 1,000 points of its own grid.
+
+C4_SAMPLE = 1000
+
+
+@pytest.fixture(scope='module')
+def c4():
+  import torch
+  if not torch.cuda.is_available():
+    pytest.skip('no HIP device')
+  from orbkit_amd import workloads
+  from oracle import core_oracle as O
+  mols = workloads.make_batch(n_mol=256)
+  samples = []
+  for m, (qc, axes) in enumerate(mols):
+    n = len(axes[0]) * len(axes[1]) * len(axes[2])
+    idx = numpy.sort(numpy.random.default_rng(100 + m).choice(n, size=C4_SAMPLE, replace=False))
+    ix, rest = numpy.divmod(idx, len(axes[1]) * len(axes[2]))   # x slowest, z fastest
+    iy, iz = numpy.divmod(rest, len(axes[2]))
+    x, y, z = axes[0][ix], axes[1][iy], axes[2][iz]
+    r, d, _ = O.rho_slices_c(qc, x, y, z, [1, 2, 3], slice_length=64, nthreads=16)
+    samples.append((idx, (x, y, z), r, d))
+  return mols, samples
+
+
+def _errs(rho, grad, r, d):
+  mask = r > 1e-10 * r.max()
+  return (float(numpy.linalg.norm(rho - r) / numpy.linalg.norm(r)),
+          float(numpy.linalg.norm(grad - d) / numpy.linalg.norm(d)),
+          float(numpy.max(numpy.abs(rho - r)[mask] / r[mask])))
+
+
+# bars just above the measured worst molecule (bf16x6: rho 2.5e-7, grad
+# 2.4e-7, pointwise 7.0e-7; f32: 3.4e-7, 3.2e-7, 7.6e-7): FP32 rounding of
+# the AO values and coefficients and FP32 accumulation over 50-600 AOs, with
+# fewer MOs (8-30) to average over than the 40 of the medium workload
+@pytest.mark.parametrize('precision,bar,bar_max', [('bf16x6', 4e-7, 1.5e-6), ('f32', 5e-7, 1.5e-6)])
+def test_c4_batch_vs_oracle(c4, precision, bar, bar_max):
+  """Every molecule's rho and grad rho from the batched launch (persistent
+  workgroups, 4x4x4 bricks, HIP graph) against the oracle on its sample:
+  norm-wise relative error <= bar for rho and for grad rho, and the
+  pointwise relative rho error (rho > 1e-10 max rho) <= bar_max."""
+  import torch
+  from orbkit_amd.batch import BatchPlan
+  mols, samples = c4
+  print('case C4 batch', precision, len(mols), flush=True)
+  plan = BatchPlan(mols, precision=precision)
+  plan.set_workgroups(2048)
+  s = torch.cuda.Stream()
+  plan.launch(use_graph=True, stream=s.cuda_stream)
+  plan.launch(use_graph=True, stream=s.cuda_stream)
+  s.synchronize()
+  worst = [0.0, 0.0, 0.0]
+  for m in range(len(mols)):
+    idx, _, r, d = samples[m]
+    rho, grad = plan.result(m)
+    e = _errs(rho.reshape(-1)[idx], grad.reshape(3, -1)[:, idx], r, d)
+    worst = [max(a, b) for a, b in zip(worst, e)]
+    assert e[0] <= bar and e[1] <= bar and e[2] <= bar_max, (m, e)
+  print(precision, 'worst rho L2 %.2e, grad L2 %.2e, pointwise rho %.2e' % tuple(worst))
+  del plan
+  torch.cuda.empty_cache()
+
+
+def test_c4_f64_subset_vs_oracle(c4):
+  """The FP64 path (okmi_rho) for every 16th molecule on its sample points
+  against the oracle at the FP64 bar (max |a - b| / max |b| <= 1e-12)."""
+  from conftest import rel_err
+  from orbkit_amd.basis import device_basis
+  from orbkit_amd.core import DeviceGrid, eval_rho
+  mols, samples = c4
+  for m in range(0, len(mols), 16):
+    qc = mols[m][0]
+    idx, (x, y, z), r, d = samples[m]
+    print('case C4 f64 molecule', m, flush=True)
+    rho, grad, _ = eval_rho(device_basis(qc.geo_spec, qc.ao_spec), DeviceGrid(x, y, z, regular=False),
+                            qc.mo_spec.get_coeffs(), qc.mo_spec.get_occ(), [1, 2, 3])
+    assert rel_err(rho, r) <= 1e-12 and rel_err(grad, d) <= 1e-12, (m, rel_err(rho, r), rel_err(grad, d))
