@@ -238,7 +238,9 @@ int okmi_gross_atomic_density(const okmi_basis* b, const okmi_grid* g, int64_t p
  * The same from caller-supplied AO values d_ao [nao][ld] and MO values
  * d_mo [nmo][ld] (orbkit's ao_list / mo_list arguments) for one selection
  * of AO rows d_rows (device array): d_out[p] = sum_i occ_i mo_i(p)
- * sum_{r in rows} C[i][r] ao_r(p).
+ * sum_{r in rows} C[i][r] ao_r(p).  Every row must lie in [0, nao): the
+ * rows are read on the device only, so the caller checks them (the Python
+ * wrapper raises ValueError otherwise).
  */
 int okmi_masked_mo_product(const double* d_c, const double* d_occ, int nmo, int nao, const double* d_ao,
                            const double* d_mo, int64_t np, int64_t ld, const int* d_rows, int nrows,
@@ -252,7 +254,8 @@ int okmi_mo_pair_products(const double* d_bra, int na, const double* d_ket, int 
 /*
  * Transition electronic flux densities (orbkit/extras.py:441-492, calc_jmo):
  *   d_out[d][k][p] = -1/2 (mo[0][i][p] mo[1+d][j][p] - mo[0][j][p] mo[1+d][i][p])
- * for pairs (i, j) = d_pairs[2k], d_pairs[2k+1] (device array); d_mo is
+ * for pairs (i, j) = d_pairs[2k], d_pairs[2k+1] (device array, every index
+ * in [0, nmo): checked by the caller, as the Python wrapper does); d_mo is
  * [1 + nd][nmo][ld_in]: MO values, then nd derivative sets.
  */
 int okmi_transition_flux(const double* d_mo, int nmo, int nd, int64_t np, int64_t ld_in,

@@ -14,6 +14,8 @@ Differences from orbkit, all on the execution side:
     values because every point is independent.
   * integer codes in ``drv`` are accepted in the density branch too (orbkit
     raises TypeError there because it calls ``len()`` on them).
+  * ``drv`` lists of any length are taken, as in orbkit; more than 10 codes
+    (AO / MO values) or 16 density outputs run as several native calls.
   * extra keyword ``precision`` of ``rho_compute``: 'f64' (default) or one of
     the reduced-precision fused paths ('f32', 'bf16x3', 'bf16x6').
 '''
@@ -31,6 +33,10 @@ from .tools import validate_drv, require, convert
 WORKSPACE_BYTES = 1 << 30
 #: device bytes for one chunk of returned values
 OUTPUT_CHUNK_BYTES = 1 << 30
+#: derivative codes per native AO / MO call and density outputs per native
+#: density call; longer ``drv`` lists (orbkit takes any length) are split
+MAX_CODES_PER_CALL = 10
+MAX_OUTPUTS_PER_CALL = 16
 
 
 # --------------------------------------------------------------------------
@@ -89,6 +95,10 @@ def eval_aos(basis, dgrid, codes, p0=0, np_=None, out=None):
   nd = len(codes)
   if out is None:
     out = numpy.empty((nd, basis.nao, np_))
+  if nd > MAX_CODES_PER_CALL:
+    for g in range(0, nd, MAX_CODES_PER_CALL):
+      eval_aos(basis, dgrid, codes[g:g + MAX_CODES_PER_CALL], p0, np_, out=out[g:g + MAX_CODES_PER_CALL])
+    return out
   per = max(1, nd * basis.nao * 8)
   chunk = max(64, min(np_, OUTPUT_CHUNK_BYTES // per))
   buf = torch.empty((nd, basis.nao, min(chunk, max(np_, 1))), dtype=torch.float64,
@@ -114,6 +124,9 @@ def eval_mos(basis, dgrid, codes, coeffs, p0=0, np_=None):
   nmo = C.shape[0]
   if C.ndim != 2 or C.shape[1] != basis.nao:
     raise ValueError('MO coefficients have shape %s but the basis has %d AOs' % (C.shape, basis.nao))
+  if nd > MAX_CODES_PER_CALL:
+    return numpy.concatenate([eval_mos(basis, dgrid, codes[g:g + MAX_CODES_PER_CALL], C, p0, np_)
+                              for g in range(0, nd, MAX_CODES_PER_CALL)])
   dev = dgrid.x.device
   dC = torch.as_tensor(C, device=dev)
   out = numpy.empty((nd, nmo, np_))
@@ -147,6 +160,11 @@ def eval_rho(basis, dgrid, coeffs, occ, codes, p0=0, np_=None, mo_norm=False, st
   nmo = C.shape[0]
   if C.ndim != 2 or C.shape[1] != basis.nao:
     raise ValueError('MO coefficients have shape %s but the basis has %d AOs' % (C.shape, basis.nao))
+  if nd > MAX_OUTPUTS_PER_CALL:
+    k = MAX_OUTPUTS_PER_CALL
+    r, d, nrm = eval_rho(basis, dgrid, C, occ, codes[:k], p0, np_, mo_norm, staged)
+    rest = [eval_rho(basis, dgrid, C, occ, codes[g:g + k], p0, np_, False, staged)[1] for g in range(k, nd, k)]
+    return r, numpy.concatenate([d] + rest), nrm
   dev = dgrid.x.device
   dC = torch.as_tensor(C, device=dev)
   docc = torch.as_tensor(numpy.ascontiguousarray(occ, dtype=numpy.float64), device=dev)
@@ -377,8 +395,11 @@ def rho_compute(qc, calc_ao=False, calc_mo=False, drv=None, laplacian=False,
 
   if precision != 'f64':
     from .fused import rho_fused
-    rho, delta = rho_fused(basis, dg, mo_spec.get_coeffs(), mo_spec.get_occ(),
-                           [] if codes is None else codes, precision=precision)
+    cl = [] if codes is None else list(codes)
+    k = MAX_OUTPUTS_PER_CALL
+    parts = [rho_fused(basis, dg, mo_spec.get_coeffs(), mo_spec.get_occ(), cl[g:g + k], precision=precision)
+             for g in range(0, max(len(cl), 1), k)]
+    rho, delta = parts[0][0], numpy.concatenate([p[1] for p in parts])
     mo_norm = None
   else:
     rho, delta, mo_norm = eval_rho(basis, dg, mo_spec.get_coeffs(), mo_spec.get_occ(),
@@ -508,9 +529,16 @@ def masked_mo_product(coeffs, occ, ao, mo, rows):
   L = _native.lib()
   dev = torch.device('cuda', torch.cuda.current_device())
   t = lambda a: torch.as_tensor(numpy.ascontiguousarray(a, dtype=numpy.float64), device=dev)
+  rows = numpy.ascontiguousarray(rows, dtype=numpy.intc).reshape(-1)
+  nmo, nao = numpy.shape(coeffs)
+  if numpy.shape(ao)[0] != nao or numpy.shape(mo)[0] != nmo or numpy.shape(ao)[1] != numpy.shape(mo)[1]:
+    raise ValueError('masked_mo_product: C %s, AO %s and MO %s do not match'
+                     % (numpy.shape(coeffs), numpy.shape(ao), numpy.shape(mo)))
+  if rows.size and (rows.min() < 0 or rows.max() >= nao):
+    raise ValueError('masked_mo_product: AO rows must lie in [0, %d)' % nao)
   C, docc, A, M = t(coeffs), t(occ), t(ao), t(mo)
   n = A.shape[1]
-  r = torch.as_tensor(numpy.ascontiguousarray(rows, dtype=numpy.intc), device=dev)
+  r = torch.as_tensor(rows, device=dev)
   out = torch.empty(max(n, 1), dtype=torch.float64, device=dev)
   _native.check(L.okmi_masked_mo_product(ctypes.c_void_p(C.data_ptr()), ctypes.c_void_p(docc.data_ptr()),
                                          C.shape[0], C.shape[1], ctypes.c_void_p(A.data_ptr()),
@@ -582,7 +610,12 @@ def transition_flux(qc, indices, drv):
   drv = [drv] if not isinstance(drv, (list, tuple)) else list(drv)
   codes = [0] + [_codes([d])[0] for d in drv]
   mo, npts, N = _mo_values(qc, codes)
-  pairs = torch.as_tensor(numpy.ascontiguousarray(indices, dtype=numpy.intc).reshape(-1), device=mo.device)
+  idx = numpy.ascontiguousarray(indices, dtype=numpy.intc)
+  if idx.size % 2 or (idx.ndim > 1 and idx.shape[-1] != 2):
+    raise ValueError('transition_flux: indices must be pairs (i, j), shape (npair, 2)')
+  if idx.size and (idx.min() < 0 or idx.max() >= mo.shape[1]):
+    raise ValueError('transition_flux: MO indices must lie in [0, %d)' % mo.shape[1])
+  pairs = torch.as_tensor(idx.reshape(-1), device=mo.device)
   npair = pairs.numel() // 2
   out = torch.empty((len(drv), npair, max(npts, 1)), dtype=torch.float64, device=mo.device)
   _native.check(L.okmi_transition_flux(ctypes.c_void_p(mo.data_ptr()), mo.shape[1], len(drv), npts, mo.shape[-1],

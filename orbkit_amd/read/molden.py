@@ -149,6 +149,12 @@ def read_molden(fname, all_mo=False, spin=None, i_md=-1, **kwargs):
       del mo['spin']
     else:
       mo['sym'] += '_%s' % mo['spin'][0]
+  # ORCA writes one label for every MO: number them in file order, keeping
+  # the part after the last '.' (orbkit/read/molden.py:364-368)
+  if mos and len({mo['sym'] for mo in mos}) == 1:
+    suffix = mos[0]['sym'].split('.')[-1]
+    for i, mo in enumerate(mos):
+      mo['sym'] = '%d.%s' % (i + 1, suffix)
   qc.mo_spec = MOClass(mos)
   qc.geo_info, qc.geo_spec = geo_info, geo_spec
   qc.format_geo(is_angstrom=angstrom)

@@ -200,6 +200,14 @@ def test_mo_matrix_and_jmo_gpu(gpu):
   u, ind = numpy.unique(numpy.asarray(ij), return_inverse=True)
   sub = eo.mo_matrix(mo[u][numpy.newaxis], dmo[:, u])
   assert rel_err(j, eo.jmo(sub, ind.reshape(-1, 2))) < 1e-12
+  # indices are checked on the host before they reach the device
+  with pytest.raises(ValueError):
+    core.transition_flux(qc, [[0, 6]], ['x'])
+  with pytest.raises(ValueError):
+    core.transition_flux(qc, [[-1, 2]], ['x'])
+  ao, _ = _aos_mos(qc, x, y, z)
+  with pytest.raises(ValueError):
+    core.masked_mo_product(qc.mo_spec.get_coeffs(), qc.mo_spec.get_occ(), ao, mo, [0, ao.shape[0]])
 
 
 @pytest.mark.gpu

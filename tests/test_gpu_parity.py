@@ -172,6 +172,27 @@ def test_repeated_derivative_codes(ok):
     assert rel_err(d, rd) <= TOL, drv
 
 
+def test_long_derivative_lists(ok):
+  """drv lists longer than one native call takes (10 AO / MO codes, 16
+  density outputs) are split into several calls, as orbkit takes any length:
+  24 labels for the AOs, MOs and the density (FP64 and bf16x6)."""
+  core, grid, O = ok['core'], ok['grid'], ok['O']
+  qc, axes, _ = ok['W'].make('small', npts_side=20)
+  x, y, z = _pts(axes, 700, seed=9)
+  grid.set_grid(x, y, z, is_vector=True)
+  drv = ['x', 'y', 'z', 'xx', 'yy', 'zz', 'xy', 'xz', 'yz', None, 'z', 'y'] * 2
+  print('case long drv', len(drv), flush=True)
+  ao = core.rho_compute(qc, calc_ao=True, drv=drv)
+  assert ao.shape[0] == len(drv) and rel_err(ao, O.rho_compute(qc, x, y, z, True, calc_ao=True, drv=drv)) <= TOL
+  mo = core.rho_compute(qc, calc_mo=True, drv=drv)
+  assert mo.shape[0] == len(drv) and rel_err(mo, O.rho_compute(qc, x, y, z, True, calc_mo=True, drv=drv)) <= TOL
+  rho, d = core.rho_compute(qc, drv=drv)
+  r_rho, rd = O.rho_compute(qc, x, y, z, True, drv=drv)
+  assert d.shape[0] == len(drv) and rel_err(rho, r_rho) <= TOL and rel_err(d, rd) <= TOL
+  frho, fd = core.rho_compute(qc, drv=drv, precision='bf16x6')
+  assert fd.shape == d.shape and numpy.linalg.norm(fd - d) <= 1e-6 * numpy.linalg.norm(d)
+
+
 def test_mo_creator_and_cart2sph(ok):
   core, O = ok['core'], ok['O']
   qc, axes, _ = ok['W'].make('medium', npts_side=20)
@@ -356,32 +377,46 @@ def test_fused_f64_vs_oracle(ok, name, nmo, side):
     assert rel_err(rho1, r_rho) <= TOL
 
 
-@pytest.mark.parametrize('prec,tol', [('f32', 1e-5), ('bf16x6', 1e-5), ('bf16x3', 1e-3)])
-def test_fused_precision(ok, prec, tol):
-  '''Reduced-precision fused path vs the FP64 path: norm-wise relative error
-  of rho, grad rho and the Laplacian (first bar; measured values printed).'''
+# The precision target: <= 1e-7 relative density error against FP64 (norm-wise)
+# for the FP32-operand paths at the medium workload's full size.  Measured:
+# bf16x6 rho 3.2e-8, grad 3.1e-8; f32 rho 5.4e-8, grad 3.9e-8; the largest
+# pointwise rho error (rho > 1e-10 max rho) 4.9e-7 / 5.7e-7, which FP32
+# operands cannot bring below 1e-7 (test_fp32_operand_floor in
+# test_precision_floor.py).  bf16x3 (two bf16 parts, 16 significant bits):
+# rho 1.9e-6, pointwise 2.0e-5.
+@pytest.mark.parametrize('prec,l2_bar,max_bar', [('f32', 1e-7, 1e-6), ('bf16x6', 1e-7, 1e-6), ('bf16x3', 5e-6, 5e-5)])
+def test_fused_precision(ok, prec, l2_bar, max_bar):
+  """Reduced-precision fused path against the FP64 path (which equals the
+  oracle to 1e-12 in the tests above) on the medium workload's full 100^3
+  grid: rho, grad rho and the Laplacian, norm-wise, and rho pointwise."""
   core, grid = ok['core'], ok['grid']
-  qc, axes, _ = ok['W'].make('medium', npts_side=24)
+  qc, axes, _ = ok['W'].make('medium')
   grid.set_grid(*axes, is_vector=False)
-  print('case fused', prec, flush=True)
-  rho, d, lap = core.rho_compute(qc, laplacian=True)
-  frho, fd, flap = core.rho_compute(qc, laplacian=True, precision=prec)
-  rho1, g1 = core.rho_compute(qc, drv=['x', 'y', 'z', 'xy'])
-  frho1, fg1 = core.rho_compute(qc, drv=['x', 'y', 'z', 'xy'], precision=prec)
+  print('case fused precision', prec, flush=True)
+  rho, g = core.rho_compute(qc, drv=['x', 'y', 'z'])
+  frho, fg = core.rho_compute(qc, drv=['x', 'y', 'z'], precision=prec)
+  _, _, lap = core.rho_compute(qc, laplacian=True)
+  _, _, flap = core.rho_compute(qc, laplacian=True, precision=prec)
+  mask = rho > 1e-10 * rho.max()
   errs = {'rho': numpy.linalg.norm(frho - rho) / numpy.linalg.norm(rho),
-          'dd': numpy.linalg.norm(fd - d) / numpy.linalg.norm(d),
+          'grad': numpy.linalg.norm(fg - g) / numpy.linalg.norm(g),
           'lap': numpy.linalg.norm(flap - lap) / numpy.linalg.norm(lap),
-          'grad': numpy.linalg.norm(fg1 - g1) / numpy.linalg.norm(g1),
-          'rho_max': float(numpy.max(numpy.abs(frho - rho)[rho > 1e-10 * rho.max()] /
-                                     rho[rho > 1e-10 * rho.max()]))}
+          'rho_max': float(numpy.max(numpy.abs(frho - rho)[mask] / rho[mask]))}
   print(prec, errs)
-  for k in ('rho', 'dd', 'lap', 'grad'):
-    assert errs[k] <= tol, (k, errs[k])
+  for k in ('rho', 'grad', 'lap'):
+    assert errs[k] <= l2_bar, (k, errs[k])
+  assert errs['rho_max'] <= max_bar, errs['rho_max']
+  frho2, fd2 = core.rho_compute(qc, drv=['x', 'y', 'z', 'xy'], precision=prec)   # mixed-derivative pass
+  rho2, d2 = core.rho_compute(qc, drv=['x', 'y', 'z', 'xy'])
+  assert numpy.linalg.norm(fd2 - d2) / numpy.linalg.norm(d2) <= l2_bar
 
 
+# bars just above the measured errors (bf16x6 rho / grad: 5 MOs 9.8e-8 /
+# 9.9e-8, 20 MOs 5.6e-8 / 8.1e-8, 40 MOs 4.8e-8 / 4.5e-8; bf16x3 up to
+# 4.3e-6 / 1.1e-5): fewer MOs average the rounding over fewer terms
 @pytest.mark.parametrize('nmo', [5, 20, 40])
-@pytest.mark.parametrize('prec,tol', [('bf16x6', 1e-5), ('bf16x3', 1e-3)])
-def test_fused_mo_tiles(ok, prec, tol, nmo):
+@pytest.mark.parametrize('prec', ['bf16x6', 'bf16x3'])
+def test_fused_mo_tiles(ok, prec, nmo):
   '''rho and grad rho (the schedule that shares the C operand through LDS)
   with one, two and three 16-MO tiles and two or three bf16 planes, against
   the FP64 path.'''
@@ -394,10 +429,13 @@ def test_fused_mo_tiles(ok, prec, tol, nmo):
   e_rho = numpy.linalg.norm(frho - rho) / numpy.linalg.norm(rho)
   e_g = numpy.linalg.norm(fg - g) / numpy.linalg.norm(g)
   print(prec, nmo, e_rho, e_g)
+  tol = {'bf16x6': {5: 2e-7, 20: 1.5e-7, 40: 1e-7}[nmo], 'bf16x3': 2e-5}[prec]
   assert e_rho <= tol and e_g <= tol, (e_rho, e_g)
 
 
-@pytest.mark.parametrize('prec,tol', [('f32', 1e-5), ('bf16x6', 1e-5), ('bf16x3', 1e-3)])
+# measured (largest of the seven errors): bf16x6 1.6e-7, f32 1.7e-7 (both the
+# vector grid's xx row), bf16x3 5.0e-6
+@pytest.mark.parametrize('prec,tol', [('f32', 2.5e-7), ('bf16x6', 2.5e-7), ('bf16x3', 2e-5)])
 def test_many_mo_precision(ok, prec, tol):
   '''More MOs than the fused kernel holds (130: two 128-MO tiles, the second
   ragged): FP32 AO chunks + MFMA GEMM path vs the FP64 path, all four AO-set

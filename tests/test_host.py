@@ -42,6 +42,21 @@ def test_cart2sph_matches_orbkit_table(golden):
     assert abs(glob - r['global']) <= 1e-15 * abs(r['global']), (r['l'], r['m'])
 
 
+def test_molden_single_label_renumbered(golden, tmp_path):
+  """Files that give every MO the same label (as ORCA writes them) get the
+  labels '1.a', '2.a', ... in file order (orbkit/read/molden.py:364-368);
+  distinct labels are kept."""
+  import re as _re
+  text = open(os.path.join(golden, 'h2o_rhf_sph.molden')).read()
+  same = _re.sub(r'(?im)^(\s*sym\s*=\s*)\S+', r'\g<1>a', text)
+  p = tmp_path / 'same_label.molden'
+  p.write_text(same)
+  qc = read.main_read(str(p), all_mo=True)
+  assert [mo['sym'] for mo in qc.mo_spec] == ['%d.a' % (i + 1) for i in range(len(qc.mo_spec))]
+  orig = read.main_read(os.path.join(golden, 'h2o_rhf_sph.molden'), all_mo=True)
+  assert len({mo['sym'] for mo in orig.mo_spec}) == len(orig.mo_spec)
+
+
 def test_readers(golden):
   qc = read.main_read(os.path.join(golden, 'h2o_rhf_sph.molden'), all_mo=True)
   assert not qc.ao_spec.spherical and qc.ao_spec.get_ao_num() == 25 and len(qc.mo_spec) == 24

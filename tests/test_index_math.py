@@ -1,0 +1,115 @@
+'''The fused and batch kernels' index arithmetic, replayed on the CPU with
+bounds checks (no GPU).  Restates, line for line in meaning, the device code
+in orbkit_amd/csrc/fused_tile.h and rho_fused.hip:
+
+  * fetch_c / put_c / read_c: the split C operand of one chunk, loaded by the
+    256 threads of a workgroup from the coefficient planes
+    [plane][mpad][kpad] and staged in LDS as 80-byte rows (CStage);
+  * find_item: the batch kernel's binary search from a tile index to its
+    molecule over the tile prefix sums;
+  * brick_of / tile_point: a tile's 64 points in a 2^tlx x 2^tly x 2^tlz
+    brick of a regular grid, clamped at the grid's faces.
+
+Every global read must stay inside the planes, every LDS write inside the
+stage, every contraction read on a written slot, every tile must map to the
+molecule whose range holds it, and the active lanes of all bricks must cover
+each grid point exactly once, for the shapes the GPU tests and the
+batched workload use (one to four MO tiles, two or three bf16 planes, every
+brick shape the tuner tries, grid sides 12..80).
+'''
+import numpy
+import pytest
+
+KC = 32
+
+
+def cstage(nplanes, mt):
+  data = nplanes * mt * 16 * 4
+  return data, nplanes * mt * 16 * 5, (data + 255) // 256
+
+
+@pytest.mark.parametrize('nplanes', [2, 3])
+@pytest.mark.parametrize('mt', [1, 2, 3, 4])
+def test_c_stage_bounds(nplanes, mt):
+  data, slots, rounds = cstage(nplanes, mt)
+  mpad = mt * 16
+  for kpad in (32, 64, 416):
+    nplane_elems = nplanes * mpad * kpad
+    written = numpy.zeros(slots, dtype=bool)
+    for k0 in range(0, kpad, KC):
+      written[:] = False
+      for t in range(rounds):
+        for thread in range(256):
+          # fetch_c: unconditional load, clamped to the last data slot
+          d = min(t * 256 + thread, data - 1)
+          r, c = d >> 2, d & 3
+          q, m = r // (mt * 16), r - (r // (mt * 16)) * (mt * 16)
+          assert 0 <= q < nplanes and 0 <= m < mpad
+          first = (q * mpad + m) * kpad + k0 + 8 * c
+          assert 0 <= first and first + 7 < nplane_elems
+          assert k0 + 8 * c + 7 < kpad             # inside this MO row
+          # put_c: only data slots are written
+          d = t * 256 + thread
+          if d < data:
+            slot = (d >> 2) * 5 + (d & 3)
+            assert slot < slots and not written[slot]
+            written[slot] = True
+      # read_c: every fragment read is a written slot
+      for q in range(nplanes):
+        for m_t in range(mt):
+          for n16 in range(16):
+            for g in range(4):
+              slot = ((q * mt + m_t) * 16 + n16) * 5 + g
+              assert slot < slots and written[slot]
+      assert written.sum() == data
+
+
+def find_item(start, n, t):
+  lo, hi = 0, n - 1
+  while lo < hi:
+    mid = (lo + hi + 1) >> 1
+    if start[mid] <= t:
+      lo = mid
+    else:
+      hi = mid - 1
+  return lo
+
+
+def test_find_item_maps_every_tile():
+  rng = numpy.random.default_rng(3)
+  for n in (1, 2, 10, 256):
+    tiles = rng.integers(1, 1000, size=n)
+    start = numpy.concatenate([[0], numpy.cumsum(tiles)])
+    for t in range(0, int(start[-1]), max(1, int(start[-1]) // 5000)):
+      m = find_item(start, n, t)
+      assert start[m] <= t < start[m + 1]
+    for m in range(n):   # both ends of every molecule's range
+      assert find_item(start, n, int(start[m])) == m
+      assert find_item(start, n, int(start[m + 1]) - 1) == m
+
+
+@pytest.mark.parametrize('shape', [(2, 2, 2), (1, 2, 3), (0, 3, 3), (0, 0, 6)])
+@pytest.mark.parametrize('sides', [(12, 13, 14), (19, 19, 19), (32, 47, 80)])
+def test_bricks_cover_the_grid_once(shape, sides):
+  tlx, tly, tlz = shape
+  nx, ny, nz = sides
+  nbx, nby, nbz = -(-nx >> tlx), -(-ny >> tly), -(-nz >> tlz)
+  ntile = nbx * nby * nbz
+  assert ntile < 2 ** 31
+  count = numpy.zeros(nx * ny * nz, dtype=int)
+  lane = numpy.arange(64)
+  for tile in range(ntile):
+    t = tile                          # brick_of (32-bit arithmetic)
+    bz = t % nbz
+    t //= nbz
+    by = t % nby
+    bx = t // nby
+    ix = (bx << tlx) + (lane >> (tly + tlz))
+    iy = (by << tly) + ((lane >> tlz) & ((1 << tly) - 1))
+    iz = (bz << tlz) + (lane & ((1 << tlz) - 1))
+    act = (ix < nx) & (iy < ny) & (iz < nz)
+    ix, iy, iz = numpy.minimum(ix, nx - 1), numpy.minimum(iy, ny - 1), numpy.minimum(iz, nz - 1)
+    p = (ix * ny + iy) * nz + iz
+    assert p.min() >= 0 and p.max() < nx * ny * nz     # clamped lanes read inside the grid
+    numpy.add.at(count, p[act], 1)
+  assert numpy.all(count == 1)
