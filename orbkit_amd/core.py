@@ -37,6 +37,8 @@ OUTPUT_CHUNK_BYTES = 1 << 30
 #: density call; longer ``drv`` lists (orbkit takes any length) are split
 MAX_CODES_PER_CALL = 10
 MAX_OUTPUTS_PER_CALL = 16
+#: points per block when a result already on the host is written to a file
+OUTPUT_SLICE = 1 << 16
 
 
 # --------------------------------------------------------------------------
@@ -85,22 +87,21 @@ def _codes(drv_list):
   return numpy.array([validate_drv(d) for d in drv_list], dtype=numpy.intc)
 
 
-def eval_aos(basis, dgrid, codes, p0=0, np_=None, out=None):
-  '''AO values for derivative codes on points [p0, p0+np): host array
-  (ncodes, nao, np).'''
+def _fill(out, blocks):
+  """Copies each (first point, host block) into out[..., range] and passes
+  it on (for a file sink that writes the same blocks)."""
+  for c0, b in blocks:
+    out[..., c0:c0 + b.shape[-1]] = b
+    yield c0, b
+
+
+def _ao_blocks(basis, dgrid, codes, p0, np_):
+  """AO values (ncodes, nao, n) of consecutive point chunks, each copied to
+  the host once: (first point relative to p0, block)."""
   torch = _torch()
   L = _native.lib()
-  np_ = dgrid.npts - p0 if np_ is None else np_
-  codes = numpy.ascontiguousarray(codes, dtype=numpy.intc)
   nd = len(codes)
-  if out is None:
-    out = numpy.empty((nd, basis.nao, np_))
-  if nd > MAX_CODES_PER_CALL:
-    for g in range(0, nd, MAX_CODES_PER_CALL):
-      eval_aos(basis, dgrid, codes[g:g + MAX_CODES_PER_CALL], p0, np_, out=out[g:g + MAX_CODES_PER_CALL])
-    return out
-  per = max(1, nd * basis.nao * 8)
-  chunk = max(64, min(np_, OUTPUT_CHUNK_BYTES // per))
+  chunk = max(64, min(np_, OUTPUT_CHUNK_BYTES // max(1, nd * basis.nao * 8)))
   buf = torch.empty((nd, basis.nao, min(chunk, max(np_, 1))), dtype=torch.float64,
                     device=dgrid.x.device)
   for c0 in range(0, np_, chunk):
@@ -109,27 +110,16 @@ def eval_aos(basis, dgrid, codes, p0=0, np_=None, out=None):
                                  codes.ctypes.data_as(_native.P_int), nd, _native.OKMI_F64,
                                  ctypes.c_void_p(buf.data_ptr()), buf.shape[2], _stream()),
                   'okmi_ao_eval')
-    out[:, :, c0:c0 + n] = buf[:, :, :n].cpu().numpy()
-  return out
+    yield c0, buf[:, :, :n].cpu().numpy()
 
 
-def eval_mos(basis, dgrid, codes, coeffs, p0=0, np_=None):
-  '''MO values (ncodes, nmo, np) for derivative codes.'''
+def _mo_blocks(basis, dgrid, codes, C, p0, np_):
+  """MO values (ncodes, nmo, n) of consecutive point chunks, as _ao_blocks."""
   torch = _torch()
   L = _native.lib()
-  np_ = dgrid.npts - p0 if np_ is None else np_
-  codes = numpy.ascontiguousarray(codes, dtype=numpy.intc)
-  nd = len(codes)
-  C = numpy.ascontiguousarray(coeffs, dtype=numpy.float64)
-  nmo = C.shape[0]
-  if C.ndim != 2 or C.shape[1] != basis.nao:
-    raise ValueError('MO coefficients have shape %s but the basis has %d AOs' % (C.shape, basis.nao))
-  if nd > MAX_CODES_PER_CALL:
-    return numpy.concatenate([eval_mos(basis, dgrid, codes[g:g + MAX_CODES_PER_CALL], C, p0, np_)
-                              for g in range(0, nd, MAX_CODES_PER_CALL)])
+  nd, nmo = len(codes), C.shape[0]
   dev = dgrid.x.device
   dC = torch.as_tensor(C, device=dev)
-  out = numpy.empty((nd, nmo, np_))
   chunk = max(64, min(np_, OUTPUT_CHUNK_BYTES // max(1, nd * nmo * 8)))
   buf = torch.empty((nd, nmo, min(chunk, max(np_, 1))), dtype=torch.float64, device=dev)
   ws_chunk = max(64, WORKSPACE_BYTES // max(1, nd * basis.nao * 8))
@@ -142,15 +132,63 @@ def eval_mos(basis, dgrid, codes, coeffs, p0=0, np_=None):
                                  nmo, ctypes.c_void_p(buf.data_ptr()), buf.shape[2],
                                  ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream()),
                   'okmi_mo_eval')
-    out[:, :, c0:c0 + n] = buf[:, :, :n].cpu().numpy()
+    yield c0, buf[:, :, :n].cpu().numpy()
+
+
+def _consume(out, blocks, sink):
+  """Fills out from blocks; sink(shape, blocks) writes them to a file too."""
+  blocks = _fill(out, blocks)
+  if sink is None:
+    for _ in blocks:
+      pass
+  else:
+    sink(out.shape, blocks)
   return out
 
 
-def eval_rho(basis, dgrid, coeffs, occ, codes, p0=0, np_=None, mo_norm=False, staged=False):
+def eval_aos(basis, dgrid, codes, p0=0, np_=None, out=None, sink=None):
+  '''AO values for derivative codes on points [p0, p0+np): host array
+  (ncodes, nao, np).  sink(shape, blocks): optional writer of the same
+  blocks (e.g. a result file), called with a generator of (first point,
+  host block) in point order.'''
+  np_ = dgrid.npts - p0 if np_ is None else np_
+  codes = numpy.ascontiguousarray(codes, dtype=numpy.intc)
+  nd = len(codes)
+  if out is None:
+    out = numpy.empty((nd, basis.nao, np_))
+  if nd > MAX_CODES_PER_CALL:
+    for g in range(0, nd, MAX_CODES_PER_CALL):
+      eval_aos(basis, dgrid, codes[g:g + MAX_CODES_PER_CALL], p0, np_, out=out[g:g + MAX_CODES_PER_CALL])
+    if sink is not None:   # all codes of a point range together: from the host array
+      sink(out.shape, ((c0, out[..., c0:c0 + OUTPUT_SLICE]) for c0 in range(0, np_, OUTPUT_SLICE)))
+    return out
+  return _consume(out, _ao_blocks(basis, dgrid, codes, p0, np_), sink)
+
+
+def eval_mos(basis, dgrid, codes, coeffs, p0=0, np_=None, sink=None):
+  '''MO values (ncodes, nmo, np) for derivative codes (sink as eval_aos).'''
+  np_ = dgrid.npts - p0 if np_ is None else np_
+  codes = numpy.ascontiguousarray(codes, dtype=numpy.intc)
+  nd = len(codes)
+  C = numpy.ascontiguousarray(coeffs, dtype=numpy.float64)
+  nmo = C.shape[0]
+  if C.ndim != 2 or C.shape[1] != basis.nao:
+    raise ValueError('MO coefficients have shape %s but the basis has %d AOs' % (C.shape, basis.nao))
+  if nd > MAX_CODES_PER_CALL:
+    out = numpy.concatenate([eval_mos(basis, dgrid, codes[g:g + MAX_CODES_PER_CALL], C, p0, np_)
+                             for g in range(0, nd, MAX_CODES_PER_CALL)])
+    if sink is not None:
+      sink(out.shape, ((c0, out[..., c0:c0 + OUTPUT_SLICE]) for c0 in range(0, np_, OUTPUT_SLICE)))
+    return out
+  return _consume(numpy.empty((nd, nmo, np_)), _mo_blocks(basis, dgrid, codes, C, p0, np_), sink)
+
+
+def eval_rho(basis, dgrid, coeffs, occ, codes, p0=0, np_=None, mo_norm=False, staged=False, device=False):
   '''Density (np,), derivative outputs (ncodes, np) and optionally the MO
   norms sum_p phi^2 (nmo,), all FP64: okmi_rho (the fused FP64 kernel up to
   64 MOs), or with ``staged`` okmi_rho_staged (AO sets through device memory,
-  then the FP64 GEMM).'''
+  then the FP64 GEMM).  ``device``: density and derivative rows as device
+  tensors instead of host arrays.'''
   torch = _torch()
   L = _native.lib()
   np_ = dgrid.npts - p0 if np_ is None else np_
@@ -162,9 +200,10 @@ def eval_rho(basis, dgrid, coeffs, occ, codes, p0=0, np_=None, mo_norm=False, st
     raise ValueError('MO coefficients have shape %s but the basis has %d AOs' % (C.shape, basis.nao))
   if nd > MAX_OUTPUTS_PER_CALL:
     k = MAX_OUTPUTS_PER_CALL
-    r, d, nrm = eval_rho(basis, dgrid, C, occ, codes[:k], p0, np_, mo_norm, staged)
-    rest = [eval_rho(basis, dgrid, C, occ, codes[g:g + k], p0, np_, False, staged)[1] for g in range(k, nd, k)]
-    return r, numpy.concatenate([d] + rest), nrm
+    r, d, nrm = eval_rho(basis, dgrid, C, occ, codes[:k], p0, np_, mo_norm, staged, device)
+    rest = [eval_rho(basis, dgrid, C, occ, codes[g:g + k], p0, np_, False, staged, device)[1]
+            for g in range(k, nd, k)]
+    return r, (torch.cat([d] + rest) if device else numpy.concatenate([d] + rest)), nrm
   dev = dgrid.x.device
   dC = torch.as_tensor(C, device=dev)
   docc = torch.as_tensor(numpy.ascontiguousarray(occ, dtype=numpy.float64), device=dev)
@@ -181,9 +220,10 @@ def eval_rho(basis, dgrid, coeffs, occ, codes, p0=0, np_=None, mo_norm=False, st
                    ctypes.c_void_p(rho.data_ptr()), ctypes.c_void_p(delta.data_ptr()),
                    delta.shape[1], ctypes.c_void_p(nrm.data_ptr()) if mo_norm else None,
                    ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream()), 'okmi_rho')
-  r = rho[:np_].cpu().numpy()
-  d = delta[:nd, :np_].cpu().numpy()
-  return r, d, (nrm.cpu().numpy() if mo_norm else None)
+  nrm = nrm.cpu().numpy() if mo_norm else None
+  if device:
+    return rho[:np_], delta[:nd, :np_], nrm
+  return rho[:np_].cpu().numpy(), delta[:nd, :np_].cpu().numpy(), nrm
 
 
 def electron_count(rho, d3r):
@@ -375,15 +415,28 @@ def rho_compute(qc, calc_ao=False, calc_mo=False, drv=None, laplacian=False,
   display('\nStarting the calculation of the %s...' % ('molecular orbitals' if calc_mo else 'density'))
   display('The grid has been separated into %d slices each having %.2e grid points.' % (sNum, slice_length))
 
-  if save_hdf5:
-    import h5py  # noqa: F401  (as orbkit: the HDF5 sink needs h5py)
+  # result file: written slice by slice as orbkit's HDF5 sink does (a name
+  # ending in .npz: a NumPy archive, no h5py needed)
+  out_file = _open_result_file(save_hdf5, slice_length, was_vector) if save_hdf5 else None
 
   basis = device_basis(require(geo_spec, dtype='f'), ao_spec)
   dg = DeviceGrid(grid.x, grid.y, grid.z, regular=not was_vector)
 
   if calc_mo:
     c = [0] if codes is None else codes
-    vals = eval_aos(basis, dg, c) if calc_ao else eval_mos(basis, dg, c, mo_spec.get_coeffs())
+    name = 'ao_list' if calc_ao else 'mo_list'
+    sink = None
+    if out_file is not None:
+      # the MO axis alone without drv: (nmo, npts), else (ndrv, nmo, npts)
+      sink = lambda shape, blocks: _write_dataset(out_file, name, shape if is_drv else shape[1:],
+                                                  blocks if is_drv else ((c0, b[0]) for c0, b in blocks),
+                                                  ((len(drv),) if is_drv else ()) + (shape[1],) + N)
+    try:
+      vals = eval_aos(basis, dg, c, sink=sink) if calc_ao else eval_mos(basis, dg, c, mo_spec.get_coeffs(),
+                                                                          sink=sink)
+    finally:
+      if out_file is not None:
+        out_file.close()
     mo_list = vals if is_drv else vals[0]
     if not was_vector and drv is None:
       display('\nNorm of the MOs:')
@@ -393,23 +446,32 @@ def rho_compute(qc, calc_ao=False, calc_mo=False, drv=None, laplacian=False,
     shape = (mo_list.shape[0],) if drv is None else (len(drv), mo_list.shape[1])
     return mo_list.reshape(shape + N)
 
+  torch = _torch()
   if precision != 'f64':
     from .fused import rho_fused
     cl = [] if codes is None else list(codes)
     k = MAX_OUTPUTS_PER_CALL
-    parts = [rho_fused(basis, dg, mo_spec.get_coeffs(), mo_spec.get_occ(), cl[g:g + k], precision=precision)
-             for g in range(0, max(len(cl), 1), k)]
-    rho, delta = parts[0][0], numpy.concatenate([p[1] for p in parts])
+    parts = [rho_fused(basis, dg, mo_spec.get_coeffs(), mo_spec.get_occ(), cl[g:g + k], precision=precision,
+                       device=True) for g in range(0, max(len(cl), 1), k)]
+    rho_d, delta_d = parts[0][0], torch.cat([p[1] for p in parts])
     mo_norm = None
   else:
-    rho, delta, mo_norm = eval_rho(basis, dg, mo_spec.get_coeffs(), mo_spec.get_occ(),
-                                   [] if codes is None else codes, mo_norm=not display_quiet())
+    rho_d, delta_d, mo_norm = eval_rho(basis, dg, mo_spec.get_coeffs(), mo_spec.get_occ(),
+                                       [] if codes is None else codes, mo_norm=not display_quiet(), device=True)
+  # to the host one slice at a time; with a result file each slice is also
+  # written there (rho, then delta_rho)
+  try:
+    rho = _drain(rho_d, npts, slice_length, out_file, 'rho', N)
+    delta = _drain(delta_d, npts, slice_length, out_file, 'delta_rho', (len(drv),) + N) if is_drv else None
+  finally:
+    if out_file is not None:
+      out_file.close()
   if not was_vector and drv is None and mo_norm is not None:
     display('\nNorm of the MOs:')
     for i, v in enumerate(mo_norm):
       display('\t%.6f\tMO %d' % (v * grid.d3r, i))
   if not was_vector:
-    display('We have ' + str(electron_count(rho, grid.d3r)) + ' electrons.')
+    display('We have ' + str(electron_count(rho_d, grid.d3r)) + ' electrons.')
   rho = rho.reshape(N)
   if not is_drv:
     return rho
@@ -422,6 +484,40 @@ def rho_compute(qc, calc_ao=False, calc_mo=False, drv=None, laplacian=False,
 def display_quiet():
   from . import display as _d
   return _d.quiet
+
+
+def _open_result_file(target, slice_length, was_vector):
+  '''Result file of rho_compute with its grid group as orbkit writes it
+  (orbkit/core.py:478-485): the coordinates of the vector form of the grid,
+  is_vector false and is_regular set to whether the input grid was a vector
+  grid (orbkit's values, kept as they are).'''
+  from .output.sink import open_sink, write_grid
+  f = open_sink(target, slice_length)
+  if was_vector:
+    x, y, z = grid.x, grid.y, grid.z
+  else:
+    X, Y, Z = numpy.meshgrid(grid.x, grid.y, grid.z, indexing='ij')
+    x, y, z = X.ravel(), Y.ravel(), Z.ravel()
+  write_grid(f, x, y, z, False, was_vector)
+  return f
+
+
+def _write_dataset(f, name, shape, blocks, grid_shape):
+  f.stream(name, shape, blocks)
+  f.set_attr(name, 'shape', numpy.asarray(grid_shape))
+
+
+def _drain(t, npts, slice_length, f, name, grid_shape):
+  '''Host copy of a device result (..., npts) made one slice of points at a
+  time; with a result file f, each slice also goes to dataset `name`.'''
+  out = numpy.empty(tuple(t.shape[:-1]) + (npts,))
+  step = max(1, int(slice_length))
+  blocks = ((p0, t[..., p0:p0 + step].cpu().numpy()) for p0 in range(0, npts, step))
+  if f is None:
+    _consume(out, blocks, None)
+  else:
+    _consume(out, blocks, lambda shape, b: _write_dataset(f, name, shape, b, grid_shape))
+  return out
 
 
 def rho_compute_no_slice(qc, calc_ao=False, calc_mo=False, drv=None, laplacian=False,
@@ -577,8 +673,6 @@ def calc_mo_matrix(qc_a, qc_b=None, drv=None, numproc=1, slice_length=1e4, save_
   (NDRV = 1 without drv).'''
   torch = _torch()
   L = _native.lib()
-  if save_hdf5:
-    import h5py  # noqa: F401  (as orbkit: the HDF5 sink needs h5py)
   if isinstance(qc_a, dict):
     qc_a = QCinfo(qc_a)
   if drv is None:
@@ -600,7 +694,33 @@ def calc_mo_matrix(qc_a, qc_b=None, drv=None, numproc=1, slice_length=1e4, save_
   _native.check(L.okmi_mo_pair_products(ctypes.c_void_p(bra.data_ptr()), na, ctypes.c_void_p(ket.data_ptr()), nb, nd,
                                         npts, bra.shape[-1], ctypes.c_void_p(out.data_ptr()), out.shape[-1],
                                         _stream()), 'okmi_mo_pair_products')
-  return out[..., :npts].cpu().numpy().reshape((nd, na, nb) + N)
+  if not save_hdf5:
+    return out[..., :npts].cpu().numpy().reshape((nd, na, nb) + N)
+  # result files as orbkit writes them (orbkit/core.py:874-924): the bra
+  # (and, with qc_b, the ket) MO values in their own files, then the grid
+  # and mo_matrix, slice by slice; mo_matrix is stored flat (NDRV, NMO_a,
+  # NMO_b, npts) with its grid shape in the attribute 'shape', as
+  # rho_compute's datasets are (orbkit stores this one in the grid shape)
+  from .output.sink import open_sink, write_grid
+  target = str(save_hdf5)
+  side = (lambda k: target[:-4] + '.%s.npz' % k) if target.endswith('.npz') else (lambda k: target + '.' + k)
+  if qc_b is None or qc_b is qc_a:   # one file with the values and every ket derivative
+    mo_files = [('bra', bra[None] if drv is None else torch.cat([bra[None], ket]))]
+  else:
+    mo_files = [('bra', bra[None]), ('ket', ket)]
+  for kind, mo in mo_files:
+    f = _open_result_file(side(kind), slice_length, grid.is_vector)
+    try:
+      _drain(mo[..., :npts], npts, slice_length, f, 'mo_list', (mo.shape[0], mo.shape[1]) + N)
+    finally:
+      f.close()
+  f = open_sink(target, slice_length)
+  try:
+    write_grid(f, grid.x, grid.y, grid.z, grid.is_vector, None)
+    res = _drain(out[..., :npts], npts, slice_length, f, 'mo_matrix', (nd, na, nb) + N)
+  finally:
+    f.close()
+  return res.reshape((nd, na, nb) + N)
 
 
 def transition_flux(qc, indices, drv):

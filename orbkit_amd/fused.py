@@ -64,8 +64,11 @@ class FusedPlan:
     return self.rho[:self.np].cpu().numpy(), self.delta[:self.nd, :self.np].cpu().numpy()
 
 
-def rho_fused(basis, dgrid, coeffs, occ, codes, precision='bf16x6'):
-  '''Density (np,) and derivative rows (len(codes), np) on the whole grid.'''
+def rho_fused(basis, dgrid, coeffs, occ, codes, precision='bf16x6', device=False):
+  '''Density (np,) and derivative rows (len(codes), np) on the whole grid
+  (device tensors with ``device``, else host arrays).'''
   plan = FusedPlan(basis, dgrid, coeffs, occ, codes, precision)
   plan.launch()
+  if device:
+    return plan.rho[:plan.np], plan.delta[:plan.nd, :plan.np]
   return plan.result()

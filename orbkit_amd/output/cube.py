@@ -1,12 +1,27 @@
-'''Gaussian cube files as orbkit writes them (orbkit/output/cube.py:5-101).
+'''Gaussian cube files of grid data, in the layout orbkit writes
+(orbkit/output/cube.py:5-101), so that files from either program compare
+byte for byte.
 
-``cube_creator(data, filename, geo_info, geo_spec, comments='', labels=None)``
-takes data of shape N or (NDATA,) + N on the grid module's regular grid and
-writes the same text as orbkit: the header (title line, comment line, atom
-count, grid origin, axis vectors, atoms), optional data-set labels and the
-values, x slowest, '%13.5E', six per line plus a break after each z run.
-The value text is produced by the library's native formatter
-(okmi_cube_format); '.gz' file names are gzip-compressed.
+The format, as written here:
+
+  line 1   title ('orbkit calculation')
+  line 2   one space and the caller's comment
+  line 3   number of atoms (%5d; negative when a line of data-set labels
+           follows the atoms), grid origin x y z (%12.6f each) and, for more
+           than one data set, the number of sets (%12d)
+  4 .. 6   per axis: number of points (%5d) and the step vector (%12.6f x 3,
+           the step on the axis's own component, zeros elsewhere)
+  atoms    per atom: the rounded third geo_info column (%5d), the second
+           geo_info column as a float (%12.6f), the position (%12.6f x 3)
+  labels   only with labels: the number of sets and one integer label per
+           set, %5d each, nine fields to a line (the count opens the first),
+           then a line break (also after a full line)
+  values   x slowest, then y, then z, all sets of a point together, each
+           value %13.5E, a line break after every sixth value of an (x, y)
+           run and at the end of the run
+
+The value text comes from the library's native formatter
+(okmi_cube_format); names ending in '.gz' are gzip-compressed.
 '''
 import ctypes
 import gzip
@@ -15,41 +30,28 @@ import numpy
 
 from .. import _native, grid
 
+_EXTENSIONS = ('cube', 'cb', 'cube.gz', 'cb.gz')
+
+
+def _label_block(labels):
+  fields = ['%5d' % len(labels)] + ['%5d' % int(v) for v in labels]
+  return ''.join(f + ('\n' if i % 9 == 8 else '') for i, f in enumerate(fields)) + '\n'
+
 
 def header(ndata, geo_info, geo_spec, comments='', labels=None):
-  '''Header lines (orbkit/output/cube.py:45-82).'''
-  string = 'orbkit calculation\n'
-  string += ' %(f)s\n' % {'f': comments}
-  string += ('%(at)d' % {'at': (-1) ** (labels is not None) * len(geo_info)}).rjust(5)
-  for ii in range(3):
-    string += ('%(min)0.6f' % {'min': grid.min_[ii]}).rjust(12)
+  '''Every line before the values (see the module docstring).'''
+  natoms = len(geo_info) if labels is None else -len(geo_info)
+  first = '%5d' % natoms + ''.join('%12.6f' % grid.min_[a] for a in range(3))
   if ndata > 1:
-    string += ('%d' % ndata).rjust(12)
-  for ii in range(3):
-    string += '\n'
-    string += ('%(N)d' % {'N': grid.N_[ii]}).rjust(5)
-    for jj in range(3):
-      if jj == ii:
-        string += ('%(dr)0.6f' % {'dr': grid.delta_[ii]}).rjust(12)
-      else:
-        string += ('%(dr)0.6f' % {'dr': 0}).rjust(12)
-  string += '\n'
-  for ii in range(len(geo_info)):
-    string += ('%(N)d' % {'N': round(float(geo_info[ii][2]))}).rjust(5)
-    string += ('%(ch)0.6f' % {'ch': float(geo_info[ii][1])}).rjust(12)
-    for jj in range(3):
-      string += ('%(r)0.6f' % {'r': geo_spec[ii][jj]}).rjust(12)
-    string += '\n'
-  if labels is not None:
-    string += ('%(N)d' % {'N': ndata}).rjust(5)
-    c = 0
-    for i, j in enumerate(labels):
-      c += 1
-      string += str(j).rjust(5)
-      if c % 9 == 8:
-        string += '\n'
-    string += '\n'
-  return string
+    first += '%12d' % ndata
+  lines = ['orbkit calculation', ' %s' % comments, first]
+  for a in range(3):
+    step = [grid.delta_[a] if b == a else 0.0 for b in range(3)]
+    lines.append('%5d' % grid.N_[a] + ''.join('%12.6f' % v for v in step))
+  for info, pos in zip(geo_info, geo_spec):
+    lines.append('%5d%12.6f' % (round(float(info[2])), float(info[1])) + ''.join('%12.6f' % v for v in pos[:3]))
+  text = '\n'.join(lines) + '\n'
+  return text + (_label_block(labels) if labels is not None else '')
 
 
 def body(data):
@@ -66,32 +68,42 @@ def body(data):
   return buf.raw[:n].decode('ascii')
 
 
-def cube_creator(data, filename, geo_info, geo_spec, comments='', labels=None, **kwargs):
-  '''Writes a plain text Gaussian cube file (see module docstring).'''
+def _checked(data, labels):
+  '''(data as (NDATA,) + N, integer labels or None); AssertionError on bad
+  input, as orbkit's writer raises.'''
   data = numpy.array(data)
-  dims = 3
-  if data.ndim < dims:
-    raise AssertionError('data.ndim < ndim of grid')
-  elif data.ndim == dims:
+  if data.ndim not in (3, 4):
+    raise AssertionError('cube data must be one grid (3 axes) or a stack of grids (4 axes), not %d axes'
+                         % data.ndim)
+  if data.ndim == 3:
     data = data[numpy.newaxis]
-  elif data.ndim > dims + 1:
-    raise AssertionError('data.ndim > (ndim of grid) +2')
-  if labels is not None:
-    if labels is True or labels == 'auto':
-      labels = list(range(len(data)))
-    assert len(labels) == len(data)
-    try:
-      labels = [int(v) for v in labels]
-    except ValueError:
-      raise AssertionError('labels has to be list of integers.')
-  assert data.shape[1:] == tuple(grid.N_), 'The grid does not fit the data.'
-  if not any(filename.endswith(ext) for ext in ['cube', 'cb', 'cube.gz', 'cb.gz']):
+  if data.shape[1:] != tuple(grid.N_):
+    raise AssertionError('data grid %s differs from the grid module\'s %s' % (data.shape[1:], tuple(grid.N_)))
+  if labels is None:
+    return data, None
+  if labels is True or labels == 'auto':
+    labels = range(len(data))
+  labels = list(labels)
+  if len(labels) != len(data):
+    raise AssertionError('%d labels for %d data sets' % (len(labels), len(data)))
+  try:
+    return data, [int(v) for v in labels]
+  except (TypeError, ValueError):
+    raise AssertionError('cube labels must be integers (or convertible to them)')
+
+
+def cube_creator(data, filename, geo_info, geo_spec, comments='', labels=None, **kwargs):
+  '''Writes grid data (N or (NDATA,) + N on the grid module's regular grid)
+  as a Gaussian cube file and returns the file name ('.cube' appended
+  unless the name already ends in a cube extension).'''
+  data, labels = _checked(data, labels)
+  if not filename.endswith(_EXTENSIONS):
     filename += '.cube'
-  string = header(len(data), geo_info, geo_spec, comments, labels) + body(data)
+  text = header(len(data), geo_info, geo_spec, comments, labels) + body(data)
   if filename.endswith('gz'):
     with gzip.open(filename, 'wb') as f:
-      f.write(string.encode('utf-8'))
+      f.write(text.encode('ascii'))
   else:
     with open(filename, 'w') as f:
-      f.write(string)
+      f.write(text)
   return filename

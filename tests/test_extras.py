@@ -121,6 +121,20 @@ def test_cube_creator_vs_oracle(tmp_path):
     fn = cube.cube_creator(data, str(tmp_path / 'a'), geo_info, geo_spec, comments='test', labels=labels)
     ref = eo.cube_text(data, geo_info, geo_spec, grid.min_, grid.N_, grid.delta_, 'test', labels)
     assert open(fn).read() == ref
+  # label lines: 8 labels fill the first line with the count, 17 the second
+  for nset in (8, 9, 17):
+    many = rng.normal(size=(nset, 4, 5, 7))
+    labels = list(range(3, 3 + nset))
+    fn = cube.cube_creator(many, str(tmp_path / 'm.cb'), geo_info, geo_spec, comments='', labels=labels)
+    assert fn.endswith('m.cb')
+    assert open(fn).read() == eo.cube_text(many, geo_info, geo_spec, grid.min_, grid.N_, grid.delta_, '', labels)
+  for bad in (numpy.zeros((4, 5)), numpy.zeros((2, 4, 5, 6))):
+    with pytest.raises(AssertionError):
+      cube.cube_creator(bad, str(tmp_path / 'x'), geo_info, geo_spec)
+  with pytest.raises(AssertionError):
+    cube.cube_creator(data, str(tmp_path / 'x'), geo_info, geo_spec, labels=['a', 'b', 'c'])
+  with pytest.raises(AssertionError):
+    cube.cube_creator(data, str(tmp_path / 'x'), geo_info, geo_spec, labels=[1, 2])
   fn = cube.cube_creator(data[0], str(tmp_path / 'b.cube.gz'), geo_info, geo_spec)
   with gzip.open(fn, 'rb') as f:
     assert f.read().decode() == eo.cube_text(data[0], geo_info, geo_spec, grid.min_, grid.N_, grid.delta_)
