@@ -113,3 +113,53 @@ def test_bricks_cover_the_grid_once(shape, sides):
     assert p.min() >= 0 and p.max() < nx * ny * nz     # clamped lanes read inside the grid
     numpy.add.at(count, p[act], 1)
   assert numpy.all(count == 1)
+
+
+def _group_sum(alpha, d, rcut2, d2, rr):
+  '''The fused kernels' primitive loop (fused_tile.h eval_shell_chunk):
+  groups of four; the number of primitives of a group that reach the tile
+  (d2 < rcut2) is counted and that many LEADING primitives are summed; a
+  group that is not fully live ends the shell.'''
+  s = 0.0
+  for q in range(0, len(alpha), 4):
+    nl = int(numpy.sum(d2 < rcut2[q:q + 4]))
+    for j in range(nl):
+      s += d[q + j] * numpy.exp(-alpha[q + j] * rr)
+    if nl < 4:
+      break
+  return s
+
+
+def _fused_groups(alpha, d, rcut2):
+  '''prepare_fused_tables: a shell's primitives by screening radius, largest
+  first (stable), in groups of four padded with never-live slots.'''
+  order = sorted(range(len(alpha)), key=lambda k: -rcut2[k])
+  pad = (-len(alpha)) % 4
+  return (numpy.concatenate([alpha[order], numpy.zeros(pad)]), numpy.concatenate([d[order], numpy.zeros(pad)]),
+          numpy.concatenate([rcut2[order], -numpy.ones(pad)]))
+
+
+def test_primitive_groups_need_radius_order():
+  """With the primitives sorted by screening radius the group loop sums
+  exactly the primitives that reach the tile (the per-primitive screened
+  sum, which is what the FP64 AO kernel computes in file order).  In file
+  order, exponents listed tightest first (as the workloads and most basis
+  files list them), the same loop drops live primitives whenever a tight one
+  is screened out: the mechanism of the error a four-at-a-time FP64 AO
+  kernel variant showed on the medium basis (taken out, never committed),
+  whose primitive table is in file order.  The kept kernels build the
+  sorted groups on the host."""
+  rng = numpy.random.default_rng(8)
+  dropped = 0
+  for trial in range(2000):
+    n = int(rng.integers(1, 9))
+    alpha = numpy.sort(numpy.exp(rng.uniform(numpy.log(0.05), numpy.log(1e5), n)))[::-1]   # tightest first
+    d = rng.uniform(0.1, 1.0, n)
+    rcut2 = 92.1 / alpha                      # exp(-alpha r^2) below 1e-40 beyond r^2 = 92.1 / alpha
+    d2 = float(rng.uniform(0, 2 * rcut2.max()))
+    rr = d2 + float(rng.uniform(0, 1))
+    exact = sum(d[k] * numpy.exp(-alpha[k] * rr) for k in range(n) if d2 < rcut2[k])
+    assert _group_sum(*_fused_groups(alpha, d, rcut2), d2, rr) == pytest.approx(exact, rel=1e-15, abs=0)
+    if abs(_group_sum(alpha, d, rcut2, d2, rr) - exact) > 1e-12 * abs(exact):
+      dropped += 1
+  assert dropped > 100   # file order loses live primitives in many configurations
