@@ -28,7 +28,9 @@ from . import _native
 from .basis import DeviceBasis, flatten
 from .core import DeviceGrid, _stream
 
-WORKSPACE_BYTES = 512 << 20
+#: device workspace of the backward pass (per chunk of points: AO values, MO
+#: values, dL/dA, about 7 KB per point at 400 AOs and 40 MOs)
+WORKSPACE_BYTES = 1 << 30
 
 
 def _torch():

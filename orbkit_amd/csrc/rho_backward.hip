@@ -9,7 +9,8 @@
 //   A        AO values (okmi_launch_ao_eval, FP64)
 //   Phi      = C A                                  (FP64 MFMA, k_gemm_f64)
 //   B_ir     = 2 occ_i g_r Phi_ir                   (k_bmat)
-//   dL/dC   += B A^T                                (k_gemm_tn, FP64 MFMA, split over points)
+//   dL/dC   += B A^T                                (k_gemm_tn, FP64 MFMA, split over points,
+//                                                    partials summed in a fixed order)
 //   G        = C^T B   (= dL/dA)                    (k_gemm_f64 with C^T)
 //   per shell s (k_shell_grad, one block per shell and range of points):
 //     w_c(r)  = sum_{rows mu of s} G_mu(r) T_s[mu][c]     (c: Cartesian component)
@@ -49,56 +50,97 @@ __global__ void k_bmat(const double* Phi, const double* occ, const double* g, in
     for (int i = 0; i < nmo; ++i) B[i * ld + p] = p < np ? occ[i] * gp * Phi[i * ld + p] : 0.0;
 }
 
-// dC[m][n] += sum_{k in [k0, k1)} B[m][k] A[n][k].  One wave per 16 MOs x
-// 64 AOs and range of points; v_mfma_f64_16x16x4f64 with the K index of
-// lane (r = l%16, q = l/16) in MFMA j of a 16-point step set to 4q + j, so
-// that each lane reads 4 consecutive points (one 32-byte load) per row.
-// Rows are padded with zeros up to a multiple of 16 points (k1 - k0 is a
-// multiple of 16, pointers 32-byte aligned, ld a multiple of 4).  Item i of
-// lane l of the accumulator holds D[4i + l/16][l%16].
+// dL/dC partials: part[s][m][n] = sum over the points k of range s of
+// B[m][k] A[n][k].  One wave per (point range, tile of 16 NT AOs) computes
+// every MO tile of that AO tile (groups of up to MG tiles), so each AO row is
+// read from HBM once; the waves of one point range run on one XCD (blocks
+// are dealt to the 8 XCDs round-robin: range s takes blocks with index
+// % 8 == s % 8), where they share that range's B rows through the XCD's L2.
+// v_mfma_f64_16x16x4f64 with the K index of lane (r = l%16, q = l/16) in
+// MFMA j of a 16-point step set to 4q + j, so that each lane reads 4
+// consecutive points (one 32-byte load) per row.  Rows are padded with
+// zeros up to a multiple of 16 points (krange a multiple of 16, pointers
+// 32-byte aligned, ld a multiple of 4).  Item i of lane l of the
+// accumulator holds D[4i + l/16][l%16].  The partials are written (no
+// atomics) and summed in a fixed order by k_reduce_dc.
 constexpr int NT = 4;
+template <int MG>
 __global__ __launch_bounds__(64) void k_gemm_tn(const double* __restrict__ B, int nmo, const double* __restrict__ A,
-                                                int nao, int64_t kend, int64_t ld, int64_t krange,
-                                                double* __restrict__ dC)
+                                                int nao, int64_t kend, int64_t ld, int64_t krange, int nsplit,
+                                                double* __restrict__ part)
 {
+    const int ntile = (nao + 16 * NT - 1) / (16 * NT);
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    const int t = local % ntile, split = (local / ntile) * 8 + xcd;
+    if (split >= nsplit) return;
     const int lane = threadIdx.x;
     const int r = lane & 15, q = lane >> 4;
-    const int m0 = blockIdx.y * 16, n0 = blockIdx.x * (16 * NT);
-    const int64_t k0 = (int64_t)blockIdx.z * krange;
+    const int n0 = t * (16 * NT);
+    const int64_t k0 = (int64_t)split * krange;
     const int64_t k1 = min(kend, k0 + krange);
-    const bool mrow = m0 + r < nmo;
-    const d4* bp = (const d4*)(B + (int64_t)(mrow ? m0 + r : 0) * ld + 4 * q);
     const d4* ap[NT];
     bool nrow[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        nrow[t] = n0 + 16 * t + r < nao;
-        ap[t] = (const d4*)(A + (int64_t)(nrow[t] ? n0 + 16 * t + r : 0) * ld + 4 * q);
+    for (int u = 0; u < NT; ++u) {
+        nrow[u] = n0 + 16 * u + r < nao;
+        ap[u] = (const d4*)(A + (int64_t)(nrow[u] ? n0 + 16 * u + r : 0) * ld + 4 * q);
     }
-    d4 acc[NT];
+    double* out = part + (int64_t)split * nmo * nao;
+    const int nmt = (nmo + 15) / 16;
+    for (int g0 = 0; g0 < nmt; g0 += MG) {
+        const d4* bp[MG];
+        bool mrow[MG];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
-    for (int64_t k = k0; k < k1; k += 16) {
-        d4 vb = bp[k >> 2];
-        if (!mrow) vb = d4{0.0, 0.0, 0.0, 0.0};
-        d4 va[NT];
+        for (int mt = 0; mt < MG; ++mt) {
+            const int m = (g0 + mt) * 16 + r;
+            mrow[mt] = m < nmo;
+            bp[mt] = (const d4*)(B + (int64_t)(mrow[mt] ? m : 0) * ld + 4 * q);
+        }
+        d4 acc[MG][NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            va[t] = ap[t][k >> 2];
-            if (!nrow[t]) va[t] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int mt = 0; mt < MG; ++mt)
+#pragma unroll
+            for (int u = 0; u < NT; ++u) acc[mt][u] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int64_t k = k0; k < k1; k += 16) {
+            d4 vb[MG], va[NT];
+#pragma unroll
+            for (int mt = 0; mt < MG; ++mt) {
+                vb[mt] = bp[mt][k >> 2];
+                if (!mrow[mt]) vb[mt] = d4{0.0, 0.0, 0.0, 0.0};
+            }
+#pragma unroll
+            for (int u = 0; u < NT; ++u) {
+                va[u] = ap[u][k >> 2];
+                if (!nrow[u]) va[u] = d4{0.0, 0.0, 0.0, 0.0};
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int mt = 0; mt < MG; ++mt)
+#pragma unroll
+                    for (int u = 0; u < NT; ++u)
+                        acc[mt][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(vb[mt][j], va[u][j], acc[mt][u], 0, 0, 0);
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int mt = 0; mt < MG; ++mt)
 #pragma unroll
-            for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(vb[j], va[t][j], acc[t], 0, 0, 0);
+            for (int u = 0; u < NT; ++u)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int m = (g0 + mt) * 16 + 4 * i + q, n = n0 + 16 * u + r;
+                    if (m < nmo && n < nao) out[(int64_t)m * nao + n] = acc[mt][u][i];
+                }
     }
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int m = m0 + 4 * i + q, n = n0 + 16 * t + r;
-            if (m < nmo && n < nao) atomicAdd(dC + (int64_t)m * nao + n, acc[t][i]);
-        }
+}
+
+// dC[e] += sum_s part[s][e] for e < n, s = 0 .. nsplit - 1 in order
+__global__ void k_reduce_dc(const double* part, int nsplit, int64_t n, double* dC)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double v = 0.0;
+    for (int s = 0; s < nsplit; ++s) v += part[(int64_t)s * n + e];
+    dC[e] += v;
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -288,8 +330,17 @@ struct BwdLayout {
     int64_t chunk = 0;
     int nrange = 0;
     int64_t ncol = 0;
-    size_t ct = 0, a = 0, phi = 0, bm = 0, g = 0, part = 0, cols = 0, total = 0;   // offsets in doubles
+    int nsplit_max = 0;   // point ranges of the dL/dC contraction
+    size_t ct = 0, a = 0, phi = 0, bm = 0, g = 0, part = 0, cols = 0, dcp = 0, total = 0;   // offsets in doubles
 };
+
+// dL/dC partials: up to 512 point ranges, at most 64 MB of them
+constexpr size_t DC_PART_BYTES = (size_t)64 << 20;
+int dc_split_max(const OkmiBasis* b, int nmo)
+{
+    const size_t one = (size_t)nmo * b->nao * sizeof(double);
+    return (int)std::max<size_t>(1, std::min<size_t>(512, DC_PART_BYTES / std::max<size_t>(one, 1)));
+}
 
 BwdLayout bwd_layout(const OkmiBasis* b, int nmo, int64_t chunk)
 {
@@ -305,6 +356,9 @@ BwdLayout bwd_layout(const OkmiBasis* b, int nmo, int64_t chunk)
     L.g = o; o += pad64((size_t)b->nao * L.chunk);
     L.part = o; o += pad64((size_t)L.nrange * L.ncol);
     L.cols = o; o += pad64((size_t)L.ncol);
+    // ranges of at least 256 points: a short chunk needs few partials
+    L.nsplit_max = (int)std::max<int64_t>(1, std::min<int64_t>(dc_split_max(b, nmo), L.chunk / 256));
+    L.dcp = o; o += pad64((size_t)L.nsplit_max * nmo * b->nao);
     L.total = o;
     return L;
 }
@@ -351,6 +405,7 @@ int okmi_launch_rho_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, 
     double* G = w + Lo.g;
     double* part = w + Lo.part;
     double* cols = w + Lo.cols;
+    double* dcp = w + Lo.dcp;
     OkmiProfScope prof("rho_backward", st);
     hipLaunchKernelGGL(k_transpose, dim3((unsigned)(((int64_t)nmo * b->nao + 255) / 256)), dim3(256), 0, st, d_c, nmo,
                        b->nao, CT);
@@ -367,16 +422,31 @@ int okmi_launch_rho_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, 
         if (rc) return rc;
         hipLaunchKernelGGL(k_bmat, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, st, Phi, d_occ, d_grad + c0,
                            nmo, n, n16, chunk, B);
-        // split the point range so that about 2048 waves cover the chip
-        const int64_t tiles = (int64_t)((b->nao + 16 * NT - 1) / (16 * NT)) * ((nmo + 15) / 16);
-        int64_t nsplit = std::max<int64_t>(1, std::min<int64_t>((2048 + tiles - 1) / tiles, n16 / 256));
+        // point ranges: about 2048 waves over the chip, at least 256 points
+        // per range, at most the partial buffer's ranges
+        const int64_t tiles = (b->nao + 16 * NT - 1) / (16 * NT);
+        int64_t nsplit = std::max<int64_t>(1, std::min<int64_t>({(2048 + tiles - 1) / tiles, n16 / 256,
+                                                                 (int64_t)Lo.nsplit_max}));
         int64_t krange = ((n16 + nsplit - 1) / nsplit + 15) & ~(int64_t)15;
         nsplit = (n16 + krange - 1) / krange;
-        dim3 grd((unsigned)((b->nao + 16 * NT - 1) / (16 * NT)), (unsigned)((nmo + 15) / 16), (unsigned)nsplit);
+        // blocks: (range group of 8, AO tile, XCD); range = group * 8 + XCD
+        const unsigned nblk = (unsigned)(((nsplit + 7) / 8) * tiles * 8);
+        const int nmt = (nmo + 15) / 16;
         {
             OkmiProfScope p2("gemm_tn", st);
-            hipLaunchKernelGGL(k_gemm_tn, grd, dim3(64), 0, st, B, nmo, A, b->nao, n16, chunk, krange, d_dc);
+            if (nmt == 1)
+                hipLaunchKernelGGL(k_gemm_tn<1>, dim3(nblk), dim3(64), 0, st, B, nmo, A, b->nao, n16, chunk, krange,
+                                   (int)nsplit, dcp);
+            else if (nmt == 2)
+                hipLaunchKernelGGL(k_gemm_tn<2>, dim3(nblk), dim3(64), 0, st, B, nmo, A, b->nao, n16, chunk, krange,
+                                   (int)nsplit, dcp);
+            else
+                hipLaunchKernelGGL(k_gemm_tn<3>, dim3(nblk), dim3(64), 0, st, B, nmo, A, b->nao, n16, chunk, krange,
+                                   (int)nsplit, dcp);
         }
+        const int64_t ndc = (int64_t)nmo * b->nao;
+        hipLaunchKernelGGL(k_reduce_dc, dim3((unsigned)((ndc + 255) / 256)), dim3(256), 0, st, dcp, (int)nsplit, ndc,
+                           d_dc);
         rc = okmi_launch_gemm_f64(B, chunk, 0, 1, nmo, CT, b->nao, n, G, chunk, st);
         if (rc) return rc;
         ShellGradArgs sa{};
