@@ -292,14 +292,18 @@ def density_step(args, torch, info, qc, axes):
 
 
 def gradient_step(args, torch, info, qc, axes):
-  '''Forward rho and backward of L = sum d3r (rho - rho_t)^2, FP64.'''
-  from orbkit_amd import _native, workloads
+  '''Forward rho and backward of L = sum d3r (rho - rho_t)^2, in FP64
+  (default) or, with --precision f32, in FP32 (FP32 forward contraction and
+  FP32 backward; their gradients are then compared with FP64 ones).'''
+  from orbkit_amd import _native
   from orbkit_amd.autograd import DensityModel
   L = _native.lib()
-  model = DensityModel(qc, axes)
+  prec = 'f32' if args.precision == 'f32' else 'f64'
+  model = DensityModel(qc, axes, precision=prec)
   C, alpha, geo = model.parameters()
   # target from perturbed coefficients C + 0.01 N(0,1) (seed 33)
-  Ct = C.detach() + 0.01 * torch.as_tensor(numpy.random.default_rng(33).normal(size=tuple(C.shape)), device=C.device)
+  Ct = C.detach() + 0.01 * torch.as_tensor(numpy.random.default_rng(33).normal(size=tuple(C.shape)),
+                                           device=C.device, dtype=C.dtype)
   rho_t = model.rho(Ct, alpha, geo)
   d3r = float(numpy.prod([a[1] - a[0] for a in axes]))
 
@@ -309,23 +313,53 @@ def gradient_step(args, torch, info, qc, axes):
     rho = model(C, alpha, geo)
     loss = (d3r * (rho - rho_t) ** 2).sum()
     loss.backward()
-  names = ['rho_f64', 'ao_eval', 'gemm_f64', 'gemm_tn', 'shell_grad', 'rho_backward']
+  sfx = '' if prec == 'f64' else '_f32'
+  fwd = 'rho_fused_f64' if prec == 'f64' else 'rho_fused'
+  names = [fwd, 'ao_eval', 'gemm_nn' + sfx, 'gemm_tn' + sfx, 'shell_grad' + sfx, 'rho_backward' + sfx]
   dt, times = timed(step, args, L, names, torch.cuda.synchronize)
   npts, nao, nmo = model.npts, info['nao'], info['nmo']
-  # dominant backward kernel: the dL/dC contraction over points (FP64 MFMA)
-  cnt, ms = times['gemm_tn']
-  t = ms / 1e3 / args.steps
-  flops = 2.0 * nmo * nao * npts
-  roof = {'bound': 'mfma', 'achieved': flops / t / 1e12, 'peak': PEAK_TFLOPS['f64'], 'unit': 'TFLOP/s',
-          'frac': flops / t / 1e12 / PEAK_TFLOPS['f64'], 'traffic': measured_traffic('k_gemm_tn', 'gradient', 'f64'),
-          'kernel': 'k_gemm_tn', 'kernel_ms_per_step': t * 1e3,
-          'other_kernels_ms_per_step': {k: v[1] / args.steps for k, v in times.items() if k != 'gemm_tn'}}
-  return {'value': npts * args.steps / dt, 'ms_per_step': dt / args.steps * 1e3, 'dtype': 'f64',
-          'metric_note': 'forward rho + backward (dL/dC, dL/dalpha, dL/dR) per grid point',
-          'config': {'workload': 'gradient: %d atoms, %d spherical AOs, %d MOs, %d^3 regular grid, '
-                                 'L = sum d3r (rho - rho_t)^2' % (info['natoms'], nao, nmo, info['side']),
-                     'npts': npts, 'nao': nao, 'nmo': nmo, 'nprim': info['nprim']},
-          'roofline': roof}, numpy.array([], dtype=numpy.intc)
+  es = 8 if prec == 'f64' else 4
+  # the MFMA contractions of the backward: per point, 2 nmo nao flop each
+  # for Phi = C A, dL/dC += B A^T and G = C^T B; algorithmic bytes: A and B
+  # read by the dL/dC contraction; A read and Phi written, B read and G
+  # written by the two C products
+  cost = {'gemm_tn' + sfx: ('k_gemm_tn', 2.0 * nmo * nao * npts, es * (nao + nmo) * npts),
+          'gemm_nn' + sfx: ('k_gemm_nn', 4.0 * nmo * nao * npts, 2 * es * (nao + nmo) * npts)}
+  roofs = {}
+  for k, (kern, flops, nbytes) in cost.items():
+    t = times[k][1] / 1e3 / args.steps
+    t_mfma = flops / (PEAK_TFLOPS[prec] * 1e12)
+    t_hbm = nbytes / (PEAK_HBM_GBS * 1e9)
+    if t_hbm >= t_mfma:
+      r = {'bound': 'hbm', 'achieved': nbytes / t / 1e9, 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
+           'frac': nbytes / t / 1e9 / PEAK_HBM_GBS}
+    else:
+      r = {'bound': 'mfma', 'achieved': flops / t / 1e12, 'peak': PEAK_TFLOPS[prec], 'unit': 'TFLOP/s',
+           'frac': flops / t / 1e12 / PEAK_TFLOPS[prec]}
+    r.update({'traffic': measured_traffic(kern, 'gradient', prec), 'kernel': kern + '<%s>' % prec,
+              'kernel_ms_per_step': t * 1e3, 'algorithmic_flop_per_step': flops,
+              'algorithmic_bytes_per_step': nbytes, 'launches_per_step': times[k][0] / args.steps})
+    roofs[k] = r
+  dom = max(roofs, key=lambda k: roofs[k]['kernel_ms_per_step'])
+  roof = roofs.pop(dom)
+  roof['other_kernels_ms_per_step'] = {k: v[1] / args.steps for k, v in times.items() if k != dom}
+  roof['other_rooflines'] = roofs
+  out = {'value': npts * args.steps / dt, 'ms_per_step': dt / args.steps * 1e3, 'dtype': prec,
+         'metric_note': 'forward rho + backward (dL/dC, dL/dalpha, dL/dR) per grid point',
+         'config': {'workload': 'gradient: %d atoms, %d spherical AOs, %d MOs, %d^3 regular grid, '
+                                'L = sum d3r (rho - rho_t)^2' % (info['natoms'], nao, nmo, info['side']),
+                    'npts': npts, 'nao': nao, 'nmo': nmo, 'nprim': info['nprim']},
+         'roofline': roof}
+  if prec == 'f32':
+    # the same loss and parameter values in FP64: norm-wise relative errors
+    ref = DensityModel(qc, axes, precision='f64')
+    C64, a64, g64 = (t.detach().double().requires_grad_(True) for t in (C, alpha, geo))
+    rt64 = ref.rho(Ct.double(), a64, g64)
+    (d3r * (ref(C64, a64, g64) - rt64) ** 2).sum().backward()
+    err = lambda a, b: float(torch.linalg.norm(a.double() - b) / torch.linalg.norm(b))
+    out['error_vs_f64'] = {'dC_l2_rel': err(C.grad, C64.grad), 'dalpha_l2_rel': err(alpha.grad, a64.grad),
+                           'dgeo_l2_rel': err(geo.grad, g64.grad)}
+  return out, numpy.array([], dtype=numpy.intc)
 
 
 def batch_step(args, torch):

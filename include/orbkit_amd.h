@@ -211,6 +211,27 @@ int okmi_rho_backward(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64
                       void* d_ws, size_t ws_bytes, void* stream);
 
 /*
+ * Workspace bytes okmi_rho_backward_f32 needs to process `chunk` points per
+ * pass (about half of the FP64 figure).
+ */
+size_t okmi_rho_backward_f32_workspace_bytes(const okmi_basis* b, int nmo, int64_t chunk);
+
+/*
+ * okmi_rho_backward in FP32 arithmetic: C, occupations, dL/drho and the
+ * accumulated outputs are FP32; AO values, MO values, dL/dA and the MFMA
+ * products (v_mfma_f32_16x16x4f32) are FP32.  Kept in FP64: the grid
+ * points, r - R and each exp argument alpha |r - R|^2, the basis constants,
+ * and every sum across lanes, point ranges and chunks (FP32 partials added
+ * in FP64 in a fixed order, then rounded once into the FP32 outputs).
+ * The same semantics as okmi_rho_backward (accumulating outputs, fixed row
+ * factors); orbkit has no derivative code (see okmi_rho_backward).
+ */
+int okmi_rho_backward_f32(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np,
+                          const float* d_c, const float* d_occ, int nmo, const float* d_grad,
+                          float* d_dc, float* d_dalpha, float* d_dgeo,
+                          void* d_ws, size_t ws_bytes, void* stream);
+
+/*
  * Integrated electron count d_out[0] = d3r * sum_p d_rho[p] in FP64 with
  * compensated summation (Neumaier per thread, TwoSum merges in a fixed
  * order; deterministic).  orbkit prints numpy.sum(rho) * d3r
@@ -318,7 +339,10 @@ int okmi_batch_destroy(okmi_batch* b);
  * Kernel timing for measurement tools: while enabled, every kernel launch
  * records a pair of HIP events on its stream.  okmi_profile_query sums the
  * elapsed times of the launches of one kernel family ("ao_eval", "gemm_f64",
- * "rho_f64", "rho_fused", "rho_big", "rho_backward", "gemm_tn", "shell_grad", "rho_batch") since the last okmi_profile_enable call.
+ * "rho_f64", "rho_fused", "rho_fused_f64", "rho_big", "rho_batch"; the
+ * backward's "rho_backward", "gemm_nn", "gemm_tn", "shell_grad" and their
+ * FP32 forms "rho_backward_f32", "gemm_nn_f32", "gemm_tn_f32",
+ * "shell_grad_f32") since the last okmi_profile_enable call.
  * No counterpart in the reference (orbkit has no kernel timers).
  */
 int okmi_profile_enable(int on);

@@ -49,6 +49,9 @@ SIGNATURES = {
     'okmi_rho_backward_workspace_bytes': (c_sz, [c_vp, c_int, c_i64]),
     'okmi_rho_backward': (c_int, [c_vp, ctypes.POINTER(OkmiGrid), c_i64, c_i64, c_vp, c_vp, c_int, c_vp, c_vp, c_vp,
                                   c_vp, c_vp, c_sz, c_vp]),
+    'okmi_rho_backward_f32_workspace_bytes': (c_sz, [c_vp, c_int, c_i64]),
+    'okmi_rho_backward_f32': (c_int, [c_vp, ctypes.POINTER(OkmiGrid), c_i64, c_i64, c_vp, c_vp, c_int, c_vp, c_vp,
+                                      c_vp, c_vp, c_vp, c_sz, c_vp]),
     'okmi_electron_count_workspace_bytes': (c_sz, []),
     'okmi_electron_count': (c_int, [c_vp, c_i64, c_dbl, c_vp, c_sz, c_vp, c_vp]),
     'okmi_gross_atomic_workspace_bytes': (c_sz, [c_vp, c_i64, c_int, c_int]),

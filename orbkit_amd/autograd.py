@@ -1,22 +1,30 @@
 '''Differentiable electron density on the device.
 
 ``DensityModel`` evaluates rho(r) = sum_i occ_i (sum_mu C_i,mu chi_mu(r))^2 on
-a fixed grid through the FP64 HIP path (the forward of
-orbkit/core.py:rho_compute, orbkit/core.py:529-607) and differentiates it
-with respect to the MO coefficients C, the primitive exponents alpha and the
-nuclear positions R.  The backward pass is native (okmi_rho_backward): AO
-values and their gradients are recomputed per chunk of points, dL/dC is an
-FP64 MFMA contraction over points and the exponent / position gradients are
-reduced per shell in a fixed order.
+a fixed grid (the forward of orbkit/core.py:rho_compute,
+orbkit/core.py:529-607) and differentiates it with respect to the MO
+coefficients C, the primitive exponents alpha and the nuclear positions R.
+The backward pass is native: AO values and their gradients are recomputed
+per chunk of points, dL/dC is an MFMA contraction over points and the
+exponent / position gradients are reduced per shell in a fixed order.
+
+``precision``:
+  'f64'  FP64 throughout: forward okmi_rho, backward okmi_rho_backward;
+         parameters and gradients are float64 tensors.
+  'f32'  FP32: forward okmi_rho_fused with the FP32 MFMA contraction (FP64
+         exp arguments, FP64 density assembly), backward
+         okmi_rho_backward_f32 (FP32 values and MFMA products; FP64 grid
+         geometry, exp arguments and reductions); parameters, rho and
+         gradients are float32 tensors.
 
     model = DensityModel(qc, axes)                  # regular grid axes (or x, y, z with is_vector)
     C, alpha, geo = model.parameters()              # leaf tensors on the device
-    rho = model(C, alpha, geo)                      # (npts,) FP64
+    rho = model(C, alpha, geo)                      # (npts,) in the model's precision
     loss = ((rho - target) ** 2 * w).sum()
     loss.backward()                                 # C.grad, alpha.grad, geo.grad
 
-orbkit has no derivative code; the forward values equal okmi_rho's (and so
-orbkit's to 1e-12).  Row factors of the basis (orbkit's 'N' row
+orbkit has no derivative code; the FP64 forward values equal okmi_rho's (and
+so orbkit's to 1e-12).  Row factors of the basis (orbkit's 'N' row
 renormalisation, the Cartesian-to-spherical map, contraction coefficients)
 are constants of the model.
 '''
@@ -29,8 +37,9 @@ from .basis import DeviceBasis, flatten
 from .core import DeviceGrid, _stream
 
 #: device workspace of the backward pass (per chunk of points: AO values, MO
-#: values, dL/dA, about 7 KB per point at 400 AOs and 40 MOs)
+#: values, dL/dA, about 7 KB per point at 400 AOs and 40 MOs in FP64)
 WORKSPACE_BYTES = 1 << 30
+PRECISIONS = ('f64', 'f32')
 
 
 def _torch():
@@ -44,8 +53,12 @@ def _torch():
 class DensityModel:
   '''rho on a fixed grid as a differentiable function of (C, alpha, geo).'''
 
-  def __init__(self, qc, axes=None, is_vector=False, occ=None, workspace_bytes=WORKSPACE_BYTES):
+  def __init__(self, qc, axes=None, is_vector=False, occ=None, workspace_bytes=WORKSPACE_BYTES, precision='f64'):
     torch = _torch()
+    if precision not in PRECISIONS:
+      raise ValueError('precision must be one of %s, not %r' % (PRECISIONS, precision))
+    self.precision = precision
+    self.dtype = torch.float64 if precision == 'f64' else torch.float32
     self.flat = flatten(qc.geo_spec, qc.ao_spec)
     if axes is None:
       from . import grid
@@ -57,6 +70,7 @@ class DensityModel:
     self.device = dev
     occ = qc.mo_spec.get_occ() if occ is None else occ
     self.occ = torch.as_tensor(numpy.ascontiguousarray(occ, dtype=numpy.float64), device=dev)
+    self.occ_r = self.occ.to(self.dtype)
     self.C0 = numpy.ascontiguousarray(qc.mo_spec.get_coeffs(), dtype=numpy.float64)
     self.nprim = len(self.flat['prims'])
     self.natoms = len(self.flat['geo'])
@@ -68,7 +82,7 @@ class DensityModel:
   def parameters(self, requires_grad=True):
     '''Leaf tensors (C, alpha, geo) holding the model's initial values.'''
     torch = _torch()
-    t = lambda a: torch.tensor(numpy.array(a, dtype=numpy.float64), device=self.device,
+    t = lambda a: torch.tensor(numpy.array(a, dtype=numpy.float64), device=self.device, dtype=self.dtype,
                                requires_grad=requires_grad)
     return t(self.C0), t(self.flat['prims'][:, 0]), t(self.flat['geo'])
 
@@ -100,42 +114,52 @@ class DensityModel:
     return self._ws
 
   def rho(self, C, alpha, geo):
-    '''Forward only (no graph): FP64 density (npts,).'''
+    '''Forward only (no graph): density (npts,) in the model's precision.'''
     torch = _torch()
     L = _native.lib()
     b = self.basis(alpha, geo)
+    # C enters the native call in FP64 (the fused FP32 path rounds it to FP32
+    # itself, so an FP32 C passes through exactly)
     C = C.detach().to(device=self.device, dtype=torch.float64).contiguous()
     nmo = C.shape[0]
     if C.shape != (nmo, b.nao) or nmo != self.occ.numel():
       raise ValueError('C has shape %s; expected (%d, %d)' % (tuple(C.shape), self.occ.numel(), b.nao))
     rho = torch.empty(self.npts, dtype=torch.float64, device=self.device)
     ws = self.workspace()
-    _native.check(L.okmi_rho(b.handle, ctypes.byref(self.grid.desc), 0, self.npts, ctypes.c_void_p(C.data_ptr()),
-                             ctypes.c_void_p(self.occ.data_ptr()), nmo, None, 0, ctypes.c_void_p(rho.data_ptr()),
-                             None, self.npts, None, ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream()),
-                  'okmi_rho')
-    return rho
+    args = (b.handle, ctypes.byref(self.grid.desc), 0, self.npts, ctypes.c_void_p(C.data_ptr()),
+            ctypes.c_void_p(self.occ.data_ptr()), nmo)
+    if self.precision == 'f64':
+      _native.check(L.okmi_rho(*args, None, 0, ctypes.c_void_p(rho.data_ptr()), None, self.npts, None,
+                               ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream()), 'okmi_rho')
+      return rho
+    _native.check(L.okmi_rho_fused(*args, None, 0, _native.PREC['f32'], ctypes.c_void_p(rho.data_ptr()), None,
+                                   self.npts, ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream()),
+                  'okmi_rho_fused')
+    return rho.to(torch.float32)
 
   def vjp(self, C, alpha, geo, grad_rho):
     '''(dL/dC, dL/dalpha, dL/dgeo) for dL/drho = grad_rho.'''
     torch = _torch()
     L = _native.lib()
     b = self.basis(alpha, geo)
-    C = C.detach().to(device=self.device, dtype=torch.float64).contiguous()
-    g = grad_rho.detach().to(device=self.device, dtype=torch.float64).contiguous().reshape(-1)
+    C = C.detach().to(device=self.device, dtype=self.dtype).contiguous()
+    g = grad_rho.detach().to(device=self.device, dtype=self.dtype).contiguous().reshape(-1)
     if g.numel() != self.npts:
       raise ValueError('grad_rho has %d entries for %d grid points' % (g.numel(), self.npts))
     nmo = C.shape[0]
+    if C.shape != (nmo, b.nao) or nmo != self.occ.numel():
+      raise ValueError('C has shape %s; expected (%d, %d)' % (tuple(C.shape), self.occ.numel(), b.nao))
     dC = torch.zeros_like(C)
-    da = torch.zeros(self.nprim, dtype=torch.float64, device=self.device)
-    dg = torch.zeros((self.natoms, 3), dtype=torch.float64, device=self.device)
+    da = torch.zeros(self.nprim, dtype=self.dtype, device=self.device)
+    dg = torch.zeros((self.natoms, 3), dtype=self.dtype, device=self.device)
     ws = self.workspace()
-    _native.check(L.okmi_rho_backward(b.handle, ctypes.byref(self.grid.desc), 0, self.npts,
-                                      ctypes.c_void_p(C.data_ptr()), ctypes.c_void_p(self.occ.data_ptr()), nmo,
-                                      ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(dC.data_ptr()),
-                                      ctypes.c_void_p(da.data_ptr()), ctypes.c_void_p(dg.data_ptr()),
-                                      ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream()),
-                  'okmi_rho_backward')
+    fn, name = ((L.okmi_rho_backward, 'okmi_rho_backward') if self.precision == 'f64'
+                else (L.okmi_rho_backward_f32, 'okmi_rho_backward_f32'))
+    _native.check(fn(b.handle, ctypes.byref(self.grid.desc), 0, self.npts,
+                     ctypes.c_void_p(C.data_ptr()), ctypes.c_void_p(self.occ_r.data_ptr()), nmo,
+                     ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(dC.data_ptr()),
+                     ctypes.c_void_p(da.data_ptr()), ctypes.c_void_p(dg.data_ptr()),
+                     ctypes.c_void_p(ws.data_ptr()), ws.numel() * 8, _stream()), name)
     return dC, da, dg
 
   def __call__(self, C, alpha, geo):

@@ -651,6 +651,27 @@ extern "C" int okmi_rho_backward(const okmi_basis* b, const okmi_grid* g, int64_
                                     ws_bytes, (hipStream_t)stream);
 }
 
+extern "C" size_t okmi_rho_backward_f32_workspace_bytes(const okmi_basis* b, int nmo, int64_t chunk)
+{
+    if (!b || nmo <= 0 || chunk <= 0) return 0;
+    return okmi_backward_f32_ws_bytes(b, nmo, chunk);
+}
+
+extern "C" int okmi_rho_backward_f32(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np,
+                                     const float* d_c, const float* d_occ, int nmo, const float* d_grad,
+                                     float* d_dc, float* d_dalpha, float* d_dgeo, void* d_ws, size_t ws_bytes,
+                                     void* stream)
+{
+    g_err.clear();
+    if (!b || !d_c || !d_occ || !d_grad || !d_dc || !d_dalpha || !d_dgeo || nmo <= 0) {
+        okmi_set_error("rho_backward_f32: invalid arguments");
+        return 1;
+    }
+    if (check_grid(g, p0, np)) return 1;
+    return okmi_launch_rho_backward_f32(b, to_desc(g), p0, np, d_c, d_occ, nmo, d_grad, d_dc, d_dalpha, d_dgeo,
+                                        d_ws, ws_bytes, (hipStream_t)stream);
+}
+
 extern "C" int okmi_batch_create(int nmol, okmi_basis* const* bases, const okmi_grid* grids, const double* const* d_c,
                                  const double* const* d_occ, const int* nmo, double* const* d_rho,
                                  double* const* d_grad, int precision, int order_by_cost, okmi_batch** out)

@@ -98,6 +98,11 @@ size_t okmi_backward_ws_bytes(const OkmiBasis* b, int nmo, int64_t chunk);
 int okmi_launch_rho_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const double* d_c,
                              const double* d_occ, int nmo, const double* d_grad, double* d_dc, double* d_dalpha,
                              double* d_dr, void* ws, size_t ws_bytes, hipStream_t st);
+// the same in FP32 arithmetic (FP64 grid geometry, exp arguments and reductions)
+size_t okmi_backward_f32_ws_bytes(const OkmiBasis* b, int nmo, int64_t chunk);
+int okmi_launch_rho_backward_f32(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const float* d_c,
+                                 const float* d_occ, int nmo, const float* d_grad, float* d_dc, float* d_dalpha,
+                                 float* d_dr, void* ws, size_t ws_bytes, hipStream_t st);
 
 // grid analysis helpers (extras.hip)
 size_t okmi_gross_ws_bytes(const OkmiBasis* b, int64_t chunk, int nrows_total, int nsel);

@@ -6,12 +6,12 @@
 // nuclear positions.  Per chunk of points (AO values are recomputed, never
 // kept across chunks):
 //
-//   A        AO values (okmi_launch_ao_eval, FP64)
-//   Phi      = C A                                  (FP64 MFMA, k_gemm_f64)
+//   A        AO values (okmi_launch_ao_eval)
+//   Phi      = C A                                  (k_gemm_nn, MFMA)
 //   B_ir     = 2 occ_i g_r Phi_ir                   (k_bmat)
-//   dL/dC   += B A^T                                (k_gemm_tn, FP64 MFMA, split over points,
+//   dL/dC   += B A^T                                (k_gemm_tn, MFMA, split over points,
 //                                                    partials summed in a fixed order)
-//   G        = C^T B   (= dL/dA)                    (k_gemm_f64 with C^T)
+//   G        = C^T B   (= dL/dA)                    (k_gemm_nn with C^T)
 //   per shell s (k_shell_grad, one block per shell and range of points):
 //     w_c(r)  = sum_{rows mu of s} G_mu(r) T_s[mu][c]     (c: Cartesian component)
 //     dL/dR_atom(s) -= sum_r sum_c w_c(r) grad_r chi_c(r)   (chi_c depends on r - R)
@@ -22,17 +22,48 @@
 //        pre-normalised primitives.
 //   Row factors of the basis (the 1/sqrt(df) and cart_scale folded into T)
 //   are held fixed.
+//
+// Two arithmetics share these kernels (template argument R):
+//   double  FP64 throughout (v_mfma_f64_16x16x4f64).
+//   float   FP32 values and MFMA products (v_mfma_f32_16x16x4f32); kept in
+//           FP64: the grid points, r - R and the exp argument alpha |r-R|^2
+//           (tight exponents near their centre), the basis constants, and
+//           every reduction across lanes, ranges and chunks (partial sums of
+//           FP32 terms are added in FP64 in a fixed order).
 #include "okmi_internal.h"
 #include "ao_device.h"
 
 namespace {
 
-typedef double d4 __attribute__((ext_vector_type(4)));
-
 constexpr int MAXP = 16;     // primitives per shell handled by one block (more: extra windows)
 constexpr int RANGE = 16384; // points per shell-gradient block
 
-__global__ void k_transpose(const double* C, int nmo, int nao, double* CT)
+// 16x16x4 MFMA of each arithmetic.  Item i of lane l of the accumulator
+// holds D[row(i, l/16)][l%16]: FP64 row 4i + l/16, FP32 row 4(l/16) + i.
+template <typename R> struct Mfma;
+template <> struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int i, int q) { return 4 * i + q; }
+};
+template <> struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int i, int q) { return 4 * q + i; }
+};
+
+template <typename R> __device__ __forceinline__ R gauss(double t);
+template <> __device__ __forceinline__ double gauss<double>(double t) { return gauss_f64(t); }
+template <> __device__ __forceinline__ float gauss<float>(double t) { return gauss_f32(t); }
+
+template <typename R>
+__global__ void k_transpose(const R* C, int nmo, int nao, R* CT)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)nmo * nao) return;
@@ -40,14 +71,100 @@ __global__ void k_transpose(const double* C, int nmo, int nao, double* CT)
     CT[k * nmo + m] = C[i];
 }
 
-// B[i][p] = 2 occ_i g(p) Phi[i][p] for p < np, 0 for np <= p < ld_used
-__global__ void k_bmat(const double* Phi, const double* occ, const double* g, int nmo, int64_t np, int64_t nfill,
-                       int64_t ld, double* B)
+// B[i][p] = 2 occ_i g(p) Phi[i][p] for p < np, 0 for np <= p < nfill
+template <typename R>
+__global__ void k_bmat(const R* Phi, const R* occ, const R* g, int nmo, int64_t np, int64_t nfill, int64_t ld, R* B)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nfill) return;
-    const double gp = p < np ? 2.0 * g[p] : 0.0;
-    for (int i = 0; i < nmo; ++i) B[i * ld + p] = p < np ? occ[i] * gp * Phi[i * ld + p] : 0.0;
+    const R gp = p < np ? R(2) * g[p] : R(0);
+    for (int i = 0; i < nmo; ++i) B[i * ld + p] = p < np ? occ[i] * gp * Phi[i * ld + p] : R(0);
+}
+
+// W (M x K, row-major) packed as MFMA A operands: Wp[t][s][l] = W[16t +
+// l%16][4s + l/16] for M tile t and K step s, zero outside W, so that a
+// wave reads a tile's operand for one K step as one coalesced 64-lane load.
+template <typename R>
+__global__ void k_pack_w(const R* W, int M, int K, R* Wp)
+{
+    const int nmt = (M + 15) / 16, nks = (K + 3) / 4;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nmt * nks * 64) return;
+    const int l = (int)(i & 63);
+    const int64_t ts = i >> 6;
+    const int t = (int)(ts / nks), st = (int)(ts % nks);
+    const int m = 16 * t + (l & 15), k = 4 * st + (l >> 4);
+    Wp[i] = (m < M && k < K) ? W[(int64_t)m * K + k] : R(0);
+}
+
+// out[m][p] = sum_k W[m][k] X[k][p] for m < M and the points of [0, npad)
+// (npad a multiple of 64, <= the row strides; X rows 16-byte aligned for
+// FP32, 32-byte for FP64), W given packed by k_pack_w.  One wave per 64
+// points computes every M tile, in groups of MG tiles: lane (r = l%16,
+// q = l/16) reads X[k + q][p0 + 4r .. p0 + 4r + 3] (one vector load per row
+// of a K step) and MFMA j of the step takes element j, so column r of point
+// tile j is point p0 + 4r + j; the four point tiles of one output row then
+// leave this lane as one vector store of four consecutive points.  The
+// operands of K step s + D are loaded while step s is multiplied (D-deep
+// register ring), which hides the HBM latency of X behind the MFMAs.  Rows
+// k >= K of the last step read row K - 1 (finite), whose W entries are 0.
+template <typename R, int MG>
+__global__ __launch_bounds__(256, 2) void k_gemm_nn(const R* __restrict__ Wp, int M, int K,
+                                                    const R* __restrict__ X, int64_t ldx, int64_t npad,
+                                                    R* __restrict__ out, int64_t ldo)
+{
+    typedef typename Mfma<R>::v4 V;
+    constexpr int D = 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * 64;
+    if (p0 >= npad) return;
+    const R* xp = X + p0 + 4 * r;
+    const int nmt = (M + 15) / 16, nks = (K + 3) / 4;
+    for (int g0 = 0; g0 < nmt; g0 += MG) {
+        const R* wp = Wp + (int64_t)g0 * nks * 64 + lane;
+        V xr[D];
+        R wr[D][MG];
+        auto load = [&](int st, V& xv, R (&wv)[MG]) {
+            xv = *(const V*)(xp + (int64_t)min(4 * st + q, K - 1) * ldx);
+#pragma unroll
+            for (int mt = 0; mt < MG; ++mt) wv[mt] = g0 + mt < nmt ? wp[((int64_t)mt * nks + st) * 64] : R(0);
+        };
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            if (d < nks) load(d, xr[d], wr[d]);
+        V acc[MG][4];
+#pragma unroll
+        for (int mt = 0; mt < MG; ++mt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[mt][j] = V{0, 0, 0, 0};
+        for (int s0 = 0; s0 < nks; s0 += D) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int st = s0 + d;
+                if (st < nks) {
+                    const V xv = xr[d];
+                    R wv[MG];
+#pragma unroll
+                    for (int mt = 0; mt < MG; ++mt) wv[mt] = wr[d][mt];
+                    if (st + D < nks) load(st + D, xr[d], wr[d]);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+#pragma unroll
+                        for (int mt = 0; mt < MG; ++mt) acc[mt][j] = Mfma<R>::mac(wv[mt], xv[j], acc[mt][j]);
+                }
+            }
+        }
+#pragma unroll
+        for (int mt = 0; mt < MG; ++mt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = (g0 + mt) * 16 + Mfma<R>::row(i, q);
+                if (m < M)
+                    *(V*)(out + (int64_t)m * ldo + p0 + 4 * r) =
+                        V{acc[mt][0][i], acc[mt][1][i], acc[mt][2][i], acc[mt][3][i]};
+            }
+    }
 }
 
 // dL/dC partials: part[s][m][n] = sum over the points k of range s of
@@ -56,19 +173,19 @@ __global__ void k_bmat(const double* Phi, const double* occ, const double* g, in
 // read from HBM once; the waves of one point range run on one XCD (blocks
 // are dealt to the 8 XCDs round-robin: range s takes blocks with index
 // % 8 == s % 8), where they share that range's B rows through the XCD's L2.
-// v_mfma_f64_16x16x4f64 with the K index of lane (r = l%16, q = l/16) in
-// MFMA j of a 16-point step set to 4q + j, so that each lane reads 4
-// consecutive points (one 32-byte load) per row.  Rows are padded with
-// zeros up to a multiple of 16 points (krange a multiple of 16, pointers
-// 32-byte aligned, ld a multiple of 4).  Item i of lane l of the
-// accumulator holds D[4i + l/16][l%16].  The partials are written (no
-// atomics) and summed in a fixed order by k_reduce_dc.
+// The K index of lane (r = l%16, q = l/16) in MFMA j of a 16-point step is
+// 4q + j, so that each lane reads 4 consecutive points (one vector load) per
+// row.  Rows are padded with zeros up to a multiple of 16 points (krange a
+// multiple of 16, rows aligned to 4 elements); the rows of the next step are
+// loaded while a step is multiplied.  The partials are written (no atomics)
+// and summed in FP64 in a fixed order by k_reduce_dc.
 constexpr int NT = 4;
-template <int MG>
-__global__ __launch_bounds__(64) void k_gemm_tn(const double* __restrict__ B, int nmo, const double* __restrict__ A,
-                                                int nao, int64_t kend, int64_t ld, int64_t krange, int nsplit,
-                                                double* __restrict__ part)
+template <typename R, int MG>
+__global__ __launch_bounds__(64, 2) void k_gemm_tn(const R* __restrict__ B, int nmo, const R* __restrict__ A, int nao,
+                                                int64_t kend, int64_t ld, int64_t krange, int nsplit,
+                                                R* __restrict__ part)
 {
+    typedef typename Mfma<R>::v4 V;
     const int ntile = (nao + 16 * NT - 1) / (16 * NT);
     const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
     const int t = local % ntile, split = (local / ntile) * 8 + xcd;
@@ -78,48 +195,53 @@ __global__ __launch_bounds__(64) void k_gemm_tn(const double* __restrict__ B, in
     const int n0 = t * (16 * NT);
     const int64_t k0 = (int64_t)split * krange;
     const int64_t k1 = min(kend, k0 + krange);
-    const d4* ap[NT];
+    const V* ap[NT];
     bool nrow[NT];
 #pragma unroll
     for (int u = 0; u < NT; ++u) {
         nrow[u] = n0 + 16 * u + r < nao;
-        ap[u] = (const d4*)(A + (int64_t)(nrow[u] ? n0 + 16 * u + r : 0) * ld + 4 * q);
+        ap[u] = (const V*)(A + (int64_t)(nrow[u] ? n0 + 16 * u + r : 0) * ld + 4 * q);
     }
-    double* out = part + (int64_t)split * nmo * nao;
+    R* out = part + (int64_t)split * nmo * nao;
     const int nmt = (nmo + 15) / 16;
     for (int g0 = 0; g0 < nmt; g0 += MG) {
-        const d4* bp[MG];
+        const V* bp[MG];
         bool mrow[MG];
 #pragma unroll
         for (int mt = 0; mt < MG; ++mt) {
             const int m = (g0 + mt) * 16 + r;
             mrow[mt] = m < nmo;
-            bp[mt] = (const d4*)(B + (int64_t)(mrow[mt] ? m : 0) * ld + 4 * q);
+            bp[mt] = (const V*)(B + (int64_t)(mrow[mt] ? m : 0) * ld + 4 * q);
         }
-        d4 acc[MG][NT];
+        V acc[MG][NT];
 #pragma unroll
         for (int mt = 0; mt < MG; ++mt)
 #pragma unroll
-            for (int u = 0; u < NT; ++u) acc[mt][u] = d4{0.0, 0.0, 0.0, 0.0};
+            for (int u = 0; u < NT; ++u) acc[mt][u] = V{0, 0, 0, 0};
+        // the next step's rows are loaded while this step is multiplied
+        V nb[MG], na[NT];
+        auto load = [&](int64_t k) {
+#pragma unroll
+            for (int mt = 0; mt < MG; ++mt) nb[mt] = bp[mt][k >> 2];
+#pragma unroll
+            for (int u = 0; u < NT; ++u) na[u] = ap[u][k >> 2];
+        };
+        if (k0 < k1) load(k0);
         for (int64_t k = k0; k < k1; k += 16) {
-            d4 vb[MG], va[NT];
+            // rows past nmo / nao read row 0: they only reach accumulator
+            // rows / columns that are not stored
+            V vb[MG], va[NT];
 #pragma unroll
-            for (int mt = 0; mt < MG; ++mt) {
-                vb[mt] = bp[mt][k >> 2];
-                if (!mrow[mt]) vb[mt] = d4{0.0, 0.0, 0.0, 0.0};
-            }
+            for (int mt = 0; mt < MG; ++mt) vb[mt] = nb[mt];
 #pragma unroll
-            for (int u = 0; u < NT; ++u) {
-                va[u] = ap[u][k >> 2];
-                if (!nrow[u]) va[u] = d4{0.0, 0.0, 0.0, 0.0};
-            }
+            for (int u = 0; u < NT; ++u) va[u] = na[u];
+            if (k + 16 < k1) load(k + 16);
 #pragma unroll
             for (int j = 0; j < 4; ++j)
 #pragma unroll
                 for (int mt = 0; mt < MG; ++mt)
 #pragma unroll
-                    for (int u = 0; u < NT; ++u)
-                        acc[mt][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(vb[mt][j], va[u][j], acc[mt][u], 0, 0, 0);
+                    for (int u = 0; u < NT; ++u) acc[mt][u] = Mfma<R>::mac(vb[mt][j], va[u][j], acc[mt][u]);
         }
 #pragma unroll
         for (int mt = 0; mt < MG; ++mt)
@@ -127,20 +249,21 @@ __global__ __launch_bounds__(64) void k_gemm_tn(const double* __restrict__ B, in
             for (int u = 0; u < NT; ++u)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const int m = (g0 + mt) * 16 + 4 * i + q, n = n0 + 16 * u + r;
+                    const int m = (g0 + mt) * 16 + Mfma<R>::row(i, q), n = n0 + 16 * u + r;
                     if (m < nmo && n < nao) out[(int64_t)m * nao + n] = acc[mt][u][i];
                 }
     }
 }
 
-// dC[e] += sum_s part[s][e] for e < n, s = 0 .. nsplit - 1 in order
-__global__ void k_reduce_dc(const double* part, int nsplit, int64_t n, double* dC)
+// dC[e] += sum_s part[s][e] for e < n, s = 0 .. nsplit - 1 in order (FP64 sum)
+template <typename R>
+__global__ void k_reduce_dc(const R* part, int nsplit, int64_t n, R* dC)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     double v = 0.0;
-    for (int s = 0; s < nsplit; ++s) v += part[(int64_t)s * n + e];
-    dC[e] += v;
+    for (int s = 0; s < nsplit; ++s) v += (double)part[(int64_t)s * n + e];
+    dC[e] = (R)((double)dC[e] + v);
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -150,6 +273,7 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+template <typename R>
 struct ShellGradArgs {
     const ShellRec* shells;
     const PrimRec* prims;
@@ -158,7 +282,7 @@ struct ShellGradArgs {
     int64_t p0, np, ld;
     int nprim_total;
     int is_normalized;
-    const double* G;     // [nao][ld]
+    const R* G;          // [nao][ld]
     double* partial;     // [nrange][nprim_total + 3 nshell]
     int64_t ncol;
 };
@@ -167,74 +291,75 @@ struct ShellGradArgs {
 // to the exponent gradients of primitives [pw0, pw0 + MP), accumulated into
 // gr / ga when `near` (the lane's point is within the shell's radius).
 // Every primitive's d e is computed once (kept for those of the window).
-template <int L, int MP>
-__device__ __forceinline__ void shell_point(const ShellGradArgs& a, const ShellRec& sh, int pw0, int pend,
+template <int L, int MP, typename R>
+__device__ __forceinline__ void shell_point(const ShellGradArgs<R>& a, const ShellRec& sh, int pw0, int pend,
                                             double px, double py, double pz, int64_t i, double nfac, bool near,
-                                            double (&gr)[3], double (&ga)[MP])
+                                            R (&gr)[3], R (&ga)[MP])
 {
     constexpr int NC = ncart(L);
     const double X = px - sh.cx, Y = py - sh.cy, Z = pz - sh.cz;
     const double rr = X * X + Y * Y + Z * Z;
-    double S0 = 0.0, S1 = 0.0;
-    double dw[MP];
+    const R Xr = (R)X, Yr = (R)Y, Zr = (R)Z;
+    R S0 = 0, S1 = 0;
+    R dw[MP];
 #pragma unroll
-    for (int j = 0; j < MP; ++j) dw[j] = 0.0;
+    for (int j = 0; j < MP; ++j) dw[j] = 0;
     for (int p = 0; p < sh.nprim; ++p) {
         const PrimRec pr = load_prim(a.prims, sh.prim0 + p);
-        const double de = pr.d * gauss_f64(pr.alpha * rr);
+        const R de = (R)pr.d * gauss<R>(pr.alpha * rr);
         S0 += de;
-        S1 = fma(-2.0 * pr.alpha, de, S1);
+        S1 = fma((R)(-2.0 * pr.alpha), de, S1);
 #pragma unroll
         for (int j = 0; j < MP; ++j)
             if (p == pw0 + j) dw[j] = de;
     }
     // w_c = sum_rows G_row T[row][c]
     const OKMI_CONST double* T = cptr(a.T) + sh.t0;
-    double w[NC];
+    R w[NC];
 #pragma unroll
-    for (int k = 0; k < NC; ++k) w[k] = 0.0;
+    for (int k = 0; k < NC; ++k) w[k] = 0;
     for (int rw = 0; rw < sh.nrow; ++rw) {
-        const double g = a.G[(int64_t)(sh.row0 + rw) * a.ld + i];
+        const R g = a.G[(int64_t)(sh.row0 + rw) * a.ld + i];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) w[k] = fma(g, T[rw * NC + k], w[k]);
+        for (int k = 0; k < NC; ++k) w[k] = fma(g, (R)T[rw * NC + k], w[k]);
     }
     // q = sum_c w_c m_c and, with the lowered monomials u (times S0),
     // sum_c w_c d_a chi_c = R_a S1 q + sum_c w_c e_a(c) u_{c - 1_a}
-    ShellPows<L, double> P;
-    P.init(X, Y, Z);
-    double m[NC];
-    first_mono<L, 0, double>(P, m);
-    double q = 0.0;
+    ShellPows<L, R> P;
+    P.init(Xr, Yr, Zr);
+    R m[NC];
+    first_mono<L, 0, R>(P, m);
+    R q = 0;
 #pragma unroll
     for (int k = 0; k < NC; ++k) q = fma(w[k], m[k], q);
     if (pw0 == 0 && near) {
-        double u[L > 0 ? ncart(L - 1) : 1];
-        lower_mono<L, 0, double>(P, S0, u);
-        gr[0] -= fma(X * S1, q, lowered_dot<L, 0, 0, double>(w, u, 0.0));
-        gr[1] -= fma(Y * S1, q, lowered_dot<L, 1, 0, double>(w, u, 0.0));
-        gr[2] -= fma(Z * S1, q, lowered_dot<L, 2, 0, double>(w, u, 0.0));
+        R u[L > 0 ? ncart(L - 1) : 1];
+        lower_mono<L, 0, R>(P, S0, u);
+        gr[0] -= fma(Xr * S1, q, lowered_dot<L, 0, 0, R>(w, u, R(0)));
+        gr[1] -= fma(Yr * S1, q, lowered_dot<L, 1, 0, R>(w, u, R(0)));
+        gr[2] -= fma(Zr * S1, q, lowered_dot<L, 2, 0, R>(w, u, R(0)));
     }
-    if (!near) q = 0.0;
+    if (!near) q = 0;
 #pragma unroll
     for (int j = 0; j < MP; ++j) {
         if (pw0 + j < pend) {
             const double alpha = load_prim(a.prims, sh.prim0 + pw0 + j).alpha;
-            ga[j] = fma(q * dw[j], nfac / alpha - rr, ga[j]);
+            ga[j] = fma(q * dw[j], (R)(nfac / alpha - rr), ga[j]);
         }
     }
 }
 
-template <int L, int MP>
-__device__ void shell_grad(const ShellGradArgs& a, const ShellRec& sh, int pw0)
+template <int L, int MP, typename R>
+__device__ void shell_grad(const ShellGradArgs<R>& a, const ShellRec& sh, int pw0)
 {
     __shared__ double red[4][3 + MP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int pend = min(sh.nprim, pw0 + MP);
     const double nfac = a.is_normalized ? 0.0 : (2.0 * L + 3.0) / 4.0;
-    double gr[3] = {0.0, 0.0, 0.0};
-    double ga[MP];
+    R gr[3] = {0, 0, 0};
+    R ga[MP];
 #pragma unroll
-    for (int j = 0; j < MP; ++j) ga[j] = 0.0;
+    for (int j = 0; j < MP; ++j) ga[j] = 0;
     const int64_t r0 = (int64_t)blockIdx.x * RANGE;
     const int64_t r1 = min(a.np, r0 + RANGE);
     for (int64_t b = r0; b < r1; b += 256) {
@@ -245,18 +370,18 @@ __device__ void shell_grad(const ShellGradArgs& a, const ShellRec& sh, int pw0)
         const double X = px - sh.cx, Y = py - sh.cy, Z = pz - sh.cz;
         const bool near = act && X * X + Y * Y + Z * Z < sh.rcut2;
         if (!__builtin_amdgcn_ballot_w64(near)) continue;   // no point of this wave reaches the shell
-        shell_point<L, MP>(a, sh, pw0, pend, px, py, pz, act ? i : r0, nfac, near, gr, ga);
+        shell_point<L, MP, R>(a, sh, pw0, pend, px, py, pz, act ? i : r0, nfac, near, gr, ga);
     }
-    // block reduction in a fixed order (deterministic)
+    // block reduction in FP64, in a fixed order (deterministic)
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double v = wave_sum(gr[d]);
+        const double v = wave_sum((double)gr[d]);
         if (lane == 0) red[wave][d] = v;
     }
 #pragma unroll
     for (int j = 0; j < MP; ++j) {
         if (pw0 + j < pend) {
-            const double v = wave_sum(ga[j]);
+            const double v = wave_sum((double)ga[j]);
             if (lane == 0) red[wave][3 + j] = v;
         }
     }
@@ -272,18 +397,18 @@ __device__ void shell_grad(const ShellGradArgs& a, const ShellRec& sh, int pw0)
 
 // MP primitives per block (4 when no shell has more, else MAXP; more
 // primitives than MP: extra windows along grid z)
-template <int MP>
-__global__ __launch_bounds__(256) void k_shell_grad(ShellGradArgs a)
+template <int MP, typename R>
+__global__ __launch_bounds__(256) void k_shell_grad(ShellGradArgs<R> a)
 {
     const ShellRec sh = load_shell(a.shells, blockIdx.y);
     const int pw0 = blockIdx.z * MP;
     if (pw0 >= sh.nprim) return;
     switch (sh.l) {
-    case 0: shell_grad<0, MP>(a, sh, pw0); break;
-    case 1: shell_grad<1, MP>(a, sh, pw0); break;
-    case 2: shell_grad<2, MP>(a, sh, pw0); break;
-    case 3: shell_grad<3, MP>(a, sh, pw0); break;
-    default: shell_grad<4, MP>(a, sh, pw0); break;
+    case 0: shell_grad<0, MP, R>(a, sh, pw0); break;
+    case 1: shell_grad<1, MP, R>(a, sh, pw0); break;
+    case 2: shell_grad<2, MP, R>(a, sh, pw0); break;
+    case 3: shell_grad<3, MP, R>(a, sh, pw0); break;
+    default: shell_grad<4, MP, R>(a, sh, pw0); break;
     }
 }
 
@@ -299,128 +424,174 @@ __global__ void k_reduce_cols(const double* partial, int nrange, int64_t ncol, d
     colsum[c] = s;
 }
 
+template <typename R>
 __global__ void k_scatter_grad(const double* colsum, const ShellRec* shells, int nshell, int nprim_total,
-                               int natoms, double* dalpha, double* dR)
+                               int natoms, R* dalpha, R* dR)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nprim_total) {
-        dalpha[t] += colsum[t];
+        dalpha[t] = (R)((double)dalpha[t] + colsum[t]);
     } else if (t < nprim_total + 3 * natoms) {
         const int a = (t - nprim_total) / 3, d = (t - nprim_total) % 3;
         double s = 0.0;
         for (int sh = 0; sh < nshell; ++sh)
             if (load_shell(shells, sh).atom == a) s += colsum[nprim_total + 3 * sh + d];
-        dR[3 * a + d] += s;
+        dR[3 * a + d] = (R)((double)dR[3 * a + d] + s);
     }
 }
 
 // zero-filled padding of a [rows][ld] buffer's columns [c0, c1) (fill kernel,
 // no memset, so the sequence can sit in a captured graph)
-__global__ void k_zero_cols(double* X, int rows, int64_t ld, int64_t c0, int64_t c1)
+template <typename R>
+__global__ void k_zero_cols(R* X, int rows, int64_t ld, int64_t c0, int64_t c1)
 {
     const int64_t n = c1 - c0;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * rows) return;
-    X[(i / n) * ld + c0 + i % n] = 0.0;
+    X[(i / n) * ld + c0 + i % n] = R(0);
 }
 
-inline size_t pad64(size_t n) { return (n + 63) & ~(size_t)63; }
+inline size_t pad256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 struct BwdLayout {
     int64_t chunk = 0;
     int nrange = 0;
     int64_t ncol = 0;
     int nsplit_max = 0;   // point ranges of the dL/dC contraction
-    size_t ct = 0, a = 0, phi = 0, bm = 0, g = 0, part = 0, cols = 0, dcp = 0, total = 0;   // offsets in doubles
+    size_t ct = 0, wc = 0, wct = 0, a = 0, phi = 0, bm = 0, g = 0, part = 0, cols = 0, dcp = 0, total = 0;   // bytes
 };
 
 // dL/dC partials: up to 512 point ranges, at most 64 MB of them
 constexpr size_t DC_PART_BYTES = (size_t)64 << 20;
-int dc_split_max(const OkmiBasis* b, int nmo)
+int dc_split_max(const OkmiBasis* b, int nmo, size_t es)
 {
-    const size_t one = (size_t)nmo * b->nao * sizeof(double);
+    const size_t one = (size_t)nmo * b->nao * es;
     return (int)std::max<size_t>(1, std::min<size_t>(512, DC_PART_BYTES / std::max<size_t>(one, 1)));
 }
 
-BwdLayout bwd_layout(const OkmiBasis* b, int nmo, int64_t chunk)
+// elements of W (M x K) packed by k_pack_w
+inline size_t packed_w_elems(int M, int K) { return (size_t)((M + 15) / 16) * ((K + 3) / 4) * 64; }
+
+// es: bytes per value of the arithmetic (8 FP64, 4 FP32)
+BwdLayout bwd_layout(const OkmiBasis* b, int nmo, int64_t chunk, size_t es)
 {
     BwdLayout L;
     L.chunk = (chunk + 63) & ~(int64_t)63;
     L.nrange = (int)((L.chunk + RANGE - 1) / RANGE);
     L.ncol = b->nprim + 3 * (int64_t)b->nshell;
     size_t o = 0;
-    L.ct = o; o += pad64((size_t)b->nao * nmo);
-    L.a = o; o += pad64((size_t)b->nao * L.chunk);
-    L.phi = o; o += pad64((size_t)nmo * L.chunk);
-    L.bm = o; o += pad64((size_t)nmo * L.chunk);
-    L.g = o; o += pad64((size_t)b->nao * L.chunk);
-    L.part = o; o += pad64((size_t)L.nrange * L.ncol);
-    L.cols = o; o += pad64((size_t)L.ncol);
+    L.ct = o; o += pad256((size_t)b->nao * nmo * es);
+    L.wc = o; o += pad256(packed_w_elems(nmo, b->nao) * es);     // C packed (Phi = C A)
+    L.wct = o; o += pad256(packed_w_elems(b->nao, nmo) * es);    // C^T packed (G = C^T B)
+    L.a = o; o += pad256((size_t)b->nao * L.chunk * es);
+    L.phi = o; o += pad256((size_t)nmo * L.chunk * es);
+    L.bm = o; o += pad256((size_t)nmo * L.chunk * es);
+    L.g = o; o += pad256((size_t)b->nao * L.chunk * es);
+    L.part = o; o += pad256((size_t)L.nrange * L.ncol * sizeof(double));
+    L.cols = o; o += pad256((size_t)L.ncol * sizeof(double));
     // ranges of at least 256 points: a short chunk needs few partials
-    L.nsplit_max = (int)std::max<int64_t>(1, std::min<int64_t>(dc_split_max(b, nmo), L.chunk / 256));
-    L.dcp = o; o += pad64((size_t)L.nsplit_max * nmo * b->nao);
+    L.nsplit_max = (int)std::max<int64_t>(1, std::min<int64_t>(dc_split_max(b, nmo, es), L.chunk / 256));
+    L.dcp = o; o += pad256((size_t)L.nsplit_max * nmo * b->nao * es);
     L.total = o;
     return L;
 }
 
-}  // namespace
-
-size_t okmi_backward_ws_bytes(const OkmiBasis* b, int nmo, int64_t chunk)
+size_t ws_bytes_of(const OkmiBasis* b, int nmo, int64_t chunk, size_t es)
 {
-    return sizeof(double) * bwd_layout(b, nmo, chunk).total + 256;
+    return bwd_layout(b, nmo, chunk, es).total + 256;
 }
 
-int okmi_launch_rho_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const double* d_c,
-                             const double* d_occ, int nmo, const double* d_grad, double* d_dc, double* d_dalpha,
-                             double* d_dr, void* ws, size_t ws_bytes, hipStream_t st)
+// out[M][p] = W[M][K] X[K][p] for the points of [0, npad), W packed by
+// k_pack_w: M tiles in groups of at most 3 (FP64) or 5 (FP32) tiles, so that
+// the accumulators leave room for two waves per SIMD, the group size chosen
+// so that the groups divide the tiles as evenly as possible
+template <typename R>
+void launch_gemm_nn(const R* Wp, int M, int K, const R* X, int64_t ldx, int64_t npad, R* out, int64_t ldo,
+                    hipStream_t st)
 {
+    const int nmt = (M + 15) / 16;
+    const int gmax = sizeof(R) == 8 ? 3 : 5;
+    const int ngroup = (nmt + gmax - 1) / gmax;
+    const int mg = (nmt + ngroup - 1) / ngroup;
+    const dim3 grd((unsigned)((npad + 255) / 256)), blk(256);
+    switch (mg) {
+    case 1: hipLaunchKernelGGL((k_gemm_nn<R, 1>), grd, blk, 0, st, Wp, M, K, X, ldx, npad, out, ldo); break;
+    case 2: hipLaunchKernelGGL((k_gemm_nn<R, 2>), grd, blk, 0, st, Wp, M, K, X, ldx, npad, out, ldo); break;
+    case 3: hipLaunchKernelGGL((k_gemm_nn<R, 3>), grd, blk, 0, st, Wp, M, K, X, ldx, npad, out, ldo); break;
+    case 4:
+        if constexpr (sizeof(R) == 4)
+            hipLaunchKernelGGL((k_gemm_nn<R, 4>), grd, blk, 0, st, Wp, M, K, X, ldx, npad, out, ldo);
+        break;
+    default:
+        if constexpr (sizeof(R) == 4)
+            hipLaunchKernelGGL((k_gemm_nn<R, 5>), grd, blk, 0, st, Wp, M, K, X, ldx, npad, out, ldo);
+        break;
+    }
+}
+
+template <typename R>
+int launch_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const R* d_c, const R* d_occ,
+                    int nmo, const R* d_grad, R* d_dc, R* d_dalpha, R* d_dr, void* ws, size_t ws_bytes,
+                    hipStream_t st)
+{
+    constexpr size_t es = sizeof(R);
+    const char* name = es == 8 ? "rho_backward" : "rho_backward_f32";
     if (np <= 0 || nmo <= 0) return 0;
     // largest chunk (a multiple of 64 points, at most np rounded up) whose layout fits
     int64_t lo = 64, hi = (np + 63) & ~(int64_t)63;
-    if (!ws || okmi_backward_ws_bytes(b, nmo, lo) > ws_bytes) {
-        okmi_set_error("rho_backward: workspace of %zu bytes is too small (need at least %zu)", ws_bytes,
-                       okmi_backward_ws_bytes(b, nmo, lo));
+    if (!ws || ws_bytes_of(b, nmo, lo, es) > ws_bytes) {
+        okmi_set_error("%s: workspace of %zu bytes is too small (need at least %zu)", name, ws_bytes,
+                       ws_bytes_of(b, nmo, lo, es));
         return 1;
     }
     while (lo < hi) {
         const int64_t mid = ((lo + hi) / 2 + 64) & ~(int64_t)63;
         if (mid > hi) break;
-        if (okmi_backward_ws_bytes(b, nmo, mid) <= ws_bytes) lo = mid;
+        if (ws_bytes_of(b, nmo, mid, es) <= ws_bytes) lo = mid;
         else hi = mid - 64;
     }
-    const BwdLayout Lo = bwd_layout(b, nmo, lo);
+    const BwdLayout Lo = bwd_layout(b, nmo, lo, es);
     const int64_t chunk = Lo.chunk;
     for (int s = 0; s < b->nshell; ++s)
         if (b->h_shells[s].nprim > 256 * MAXP) {
-            okmi_set_error("rho_backward: shell %d has %d primitives (up to %d)", s, b->h_shells[s].nprim, 256 * MAXP);
+            okmi_set_error("%s: shell %d has %d primitives (up to %d)", name, s, b->h_shells[s].nprim, 256 * MAXP);
             return 1;
         }
     int maxp = 1;
     for (int s = 0; s < b->nshell; ++s) maxp = std::max(maxp, b->h_shells[s].nprim);
-    double* w = (double*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
-    double* CT = w + Lo.ct;
-    double* A = w + Lo.a;
-    double* Phi = w + Lo.phi;
-    double* B = w + Lo.bm;
-    double* G = w + Lo.g;
-    double* part = w + Lo.part;
-    double* cols = w + Lo.cols;
-    double* dcp = w + Lo.dcp;
-    OkmiProfScope prof("rho_backward", st);
-    hipLaunchKernelGGL(k_transpose, dim3((unsigned)(((int64_t)nmo * b->nao + 255) / 256)), dim3(256), 0, st, d_c, nmo,
-                       b->nao, CT);
+    char* w = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+    R* CT = (R*)(w + Lo.ct);
+    R* WC = (R*)(w + Lo.wc);
+    R* WCT = (R*)(w + Lo.wct);
+    R* A = (R*)(w + Lo.a);
+    R* Phi = (R*)(w + Lo.phi);
+    R* B = (R*)(w + Lo.bm);
+    R* G = (R*)(w + Lo.g);
+    double* part = (double*)(w + Lo.part);
+    double* cols = (double*)(w + Lo.cols);
+    R* dcp = (R*)(w + Lo.dcp);
+    OkmiProfScope prof(name, st);
+    hipLaunchKernelGGL(k_transpose<R>, dim3((unsigned)(((int64_t)nmo * b->nao + 255) / 256)), dim3(256), 0, st, d_c,
+                       nmo, b->nao, CT);
+    hipLaunchKernelGGL(k_pack_w<R>, dim3((unsigned)((packed_w_elems(nmo, b->nao) + 255) / 256)), dim3(256), 0, st,
+                       d_c, nmo, b->nao, WC);
+    hipLaunchKernelGGL(k_pack_w<R>, dim3((unsigned)((packed_w_elems(b->nao, nmo) + 255) / 256)), dim3(256), 0, st,
+                       CT, b->nao, nmo, WCT);
     const int set0 = 0;
     for (int64_t c0 = 0; c0 < np; c0 += chunk) {
         const int64_t n = np - c0 < chunk ? np - c0 : chunk;
         const int64_t n16 = (n + 15) & ~(int64_t)15;
-        int rc = okmi_launch_ao_eval(b, g, p0 + c0, n, &set0, 1, OKMI_F64, A, chunk, 0, st);
+        const int64_t n64 = (n + 63) & ~(int64_t)63;
+        int rc = okmi_launch_ao_eval(b, g, p0 + c0, n, &set0, 1, es == 8 ? OKMI_F64 : OKMI_F32, A, chunk, 0, st);
         if (rc) return rc;
         if (n16 > n)
-            hipLaunchKernelGGL(k_zero_cols, dim3((unsigned)((b->nao * (n16 - n) + 255) / 256)), dim3(256), 0, st, A,
-                               b->nao, chunk, n, n16);
-        rc = okmi_launch_gemm_f64(A, chunk, 0, 1, b->nao, d_c, nmo, n, Phi, chunk, st);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_bmat, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, st, Phi, d_occ, d_grad + c0,
+            hipLaunchKernelGGL(k_zero_cols<R>, dim3((unsigned)((b->nao * (n16 - n) + 255) / 256)), dim3(256), 0, st,
+                               A, b->nao, chunk, n, n16);
+        {
+            OkmiProfScope p1(es == 8 ? "gemm_nn" : "gemm_nn_f32", st);
+            launch_gemm_nn<R>(WC, nmo, b->nao, A, chunk, n64, Phi, chunk, st);
+        }
+        hipLaunchKernelGGL(k_bmat<R>, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, st, Phi, d_occ, d_grad + c0,
                            nmo, n, n16, chunk, B);
         // point ranges: about 2048 waves over the chip, at least 256 points
         // per range, at most the partial buffer's ranges
@@ -433,41 +604,69 @@ int okmi_launch_rho_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, 
         const unsigned nblk = (unsigned)(((nsplit + 7) / 8) * tiles * 8);
         const int nmt = (nmo + 15) / 16;
         {
-            OkmiProfScope p2("gemm_tn", st);
+            OkmiProfScope p2(es == 8 ? "gemm_tn" : "gemm_tn_f32", st);
             if (nmt == 1)
-                hipLaunchKernelGGL(k_gemm_tn<1>, dim3(nblk), dim3(64), 0, st, B, nmo, A, b->nao, n16, chunk, krange,
-                                   (int)nsplit, dcp);
+                hipLaunchKernelGGL((k_gemm_tn<R, 1>), dim3(nblk), dim3(64), 0, st, B, nmo, A, b->nao, n16, chunk,
+                                   krange, (int)nsplit, dcp);
             else if (nmt == 2)
-                hipLaunchKernelGGL(k_gemm_tn<2>, dim3(nblk), dim3(64), 0, st, B, nmo, A, b->nao, n16, chunk, krange,
-                                   (int)nsplit, dcp);
+                hipLaunchKernelGGL((k_gemm_tn<R, 2>), dim3(nblk), dim3(64), 0, st, B, nmo, A, b->nao, n16, chunk,
+                                   krange, (int)nsplit, dcp);
             else
-                hipLaunchKernelGGL(k_gemm_tn<3>, dim3(nblk), dim3(64), 0, st, B, nmo, A, b->nao, n16, chunk, krange,
-                                   (int)nsplit, dcp);
+                hipLaunchKernelGGL((k_gemm_tn<R, 3>), dim3(nblk), dim3(64), 0, st, B, nmo, A, b->nao, n16, chunk,
+                                   krange, (int)nsplit, dcp);
         }
         const int64_t ndc = (int64_t)nmo * b->nao;
-        hipLaunchKernelGGL(k_reduce_dc, dim3((unsigned)((ndc + 255) / 256)), dim3(256), 0, st, dcp, (int)nsplit, ndc,
-                           d_dc);
-        rc = okmi_launch_gemm_f64(B, chunk, 0, 1, nmo, CT, b->nao, n, G, chunk, st);
-        if (rc) return rc;
-        ShellGradArgs sa{};
+        hipLaunchKernelGGL(k_reduce_dc<R>, dim3((unsigned)((ndc + 255) / 256)), dim3(256), 0, st, dcp, (int)nsplit,
+                           ndc, d_dc);
+        {
+            OkmiProfScope p1(es == 8 ? "gemm_nn" : "gemm_nn_f32", st);
+            launch_gemm_nn<R>(WCT, b->nao, nmo, B, chunk, n64, G, chunk, st);
+        }
+        ShellGradArgs<R> sa{};
         sa.shells = b->d_shells; sa.prims = b->d_prims; sa.T = b->d_T; sa.grid = g;
         sa.p0 = p0 + c0; sa.np = n; sa.ld = chunk; sa.nprim_total = b->nprim; sa.is_normalized = b->is_normalized;
         sa.G = G; sa.partial = part; sa.ncol = Lo.ncol;
         const int nrange = (int)((n + RANGE - 1) / RANGE);
         {
-            OkmiProfScope p3("shell_grad", st);
+            OkmiProfScope p3(es == 8 ? "shell_grad" : "shell_grad_f32", st);
             if (maxp <= 4)
-                hipLaunchKernelGGL(k_shell_grad<4>, dim3((unsigned)nrange, (unsigned)b->nshell, 1u), dim3(256), 0, st,
-                                   sa);
+                hipLaunchKernelGGL((k_shell_grad<4, R>), dim3((unsigned)nrange, (unsigned)b->nshell, 1u), dim3(256),
+                                   0, st, sa);
             else
-                hipLaunchKernelGGL(k_shell_grad<MAXP>,
+                hipLaunchKernelGGL((k_shell_grad<MAXP, R>),
                                    dim3((unsigned)nrange, (unsigned)b->nshell, (unsigned)((maxp + MAXP - 1) / MAXP)),
                                    dim3(256), 0, st, sa);
         }
         hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)((Lo.ncol + 255) / 256)), dim3(256), 0, st, part, nrange,
                            Lo.ncol, cols);
-        hipLaunchKernelGGL(k_scatter_grad, dim3((unsigned)((b->nprim + 3 * b->natoms + 255) / 256)), dim3(256), 0, st,
-                           cols, b->d_shells, b->nshell, b->nprim, b->natoms, d_dalpha, d_dr);
+        hipLaunchKernelGGL(k_scatter_grad<R>, dim3((unsigned)((b->nprim + 3 * b->natoms + 255) / 256)), dim3(256), 0,
+                           st, cols, b->d_shells, b->nshell, b->nprim, b->natoms, d_dalpha, d_dr);
     }
-    return okmi_check_launch("rho_backward");
+    return okmi_check_launch(name);
+}
+
+}  // namespace
+
+size_t okmi_backward_ws_bytes(const OkmiBasis* b, int nmo, int64_t chunk)
+{
+    return ws_bytes_of(b, nmo, chunk, sizeof(double));
+}
+
+size_t okmi_backward_f32_ws_bytes(const OkmiBasis* b, int nmo, int64_t chunk)
+{
+    return ws_bytes_of(b, nmo, chunk, sizeof(float));
+}
+
+int okmi_launch_rho_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const double* d_c,
+                             const double* d_occ, int nmo, const double* d_grad, double* d_dc, double* d_dalpha,
+                             double* d_dr, void* ws, size_t ws_bytes, hipStream_t st)
+{
+    return launch_backward<double>(b, g, p0, np, d_c, d_occ, nmo, d_grad, d_dc, d_dalpha, d_dr, ws, ws_bytes, st);
+}
+
+int okmi_launch_rho_backward_f32(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np, const float* d_c,
+                                 const float* d_occ, int nmo, const float* d_grad, float* d_dc, float* d_dalpha,
+                                 float* d_dr, void* ws, size_t ws_bytes, hipStream_t st)
+{
+    return launch_backward<float>(b, g, p0, np, d_c, d_occ, nmo, d_grad, d_dc, d_dalpha, d_dr, ws, ws_bytes, st);
 }

@@ -193,3 +193,85 @@ def test_backward_identities_and_chunking():
   assert rel_err(dC2.cpu().numpy(), C.grad.cpu().numpy()) < 1e-12
   assert rel_err(da2.cpu().numpy(), alpha.grad.cpu().numpy()) < 1e-12
   assert rel_err(dg2.cpu().numpy(), geo.grad.cpu().numpy()) < 1e-12
+
+
+# ------------------------------------------------------------ FP32 backward
+#
+# The FP32 model rounds its parameters (C, exponents, positions) to FP32; the
+# FP64 checks below evaluate the oracle at those same rounded values, so the
+# differences measure the FP32 arithmetic alone.  Errors are norm-wise
+# (max |a - b| / max |b|).
+
+def _f32_params(torch, model):
+  C, alpha, geo = model.parameters()
+  assert C.dtype == alpha.dtype == geo.dtype == torch.float32
+  return C, alpha, geo, tuple(t.detach().double().cpu().numpy() for t in (C, alpha, geo))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name,prenorm', CASES)
+def test_backward_f32_vs_oracle(name, prenorm):
+  '''FP32 forward and FP32 backward (okmi_rho_backward_f32) against the FP64
+  oracle at the same FP32-rounded parameters and the same upstream gradient.'''
+  torch = _gpu()
+  from orbkit_amd.autograd import DensityModel
+  from oracle import grad_oracle as go
+  qc, x, y, z = _case(name, prenorm, npts=3001, seed=7)
+  print('case', name, 'prenormalized', prenorm, 'npts', len(x), 'f32', flush=True)
+  model = DensityModel(qc, (x, y, z), is_vector=True, precision='f32')
+  C, alpha, geo, (Cd, ad, gd) = _f32_params(torch, model)
+  rho = model(C, alpha, geo)
+  assert rho.dtype == torch.float32
+  r_o = go.density(qc, x, y, z, C=Cd, alpha=ad, geo=gd)
+  w, t = _loss_parts(r_o, 1)
+  g = 2.0 * w * (r_o - t)
+  _, dC_o, da_o, dg_o = go.vjp(qc, x, y, z, g, C=Cd, alpha=ad, geo=gd)
+  dC, da, dg = model.vjp(C, alpha, geo, torch.as_tensor(g, device=C.device))
+  assert dC.dtype == da.dtype == dg.dtype == torch.float32
+  errs = [rel_err(rho.detach().cpu().numpy(), r_o), rel_err(dC.cpu().numpy(), dC_o), rel_err(da.cpu().numpy(), da_o),
+          rel_err(dg.cpu().numpy(), dg_o)]
+  print('f32 errors rho %.2e dC %.2e dalpha %.2e dgeo %.2e' % tuple(errs), flush=True)
+  assert errs[0] < 1e-6
+  assert max(errs[1:]) < 3e-5
+  # through torch.autograd: the FP32 loss of the FP32 forward
+  wt = torch.as_tensor(w, device=rho.device, dtype=torch.float32)
+  tt = torch.as_tensor(t, device=rho.device, dtype=torch.float32)
+  (wt * (rho - tt) ** 2).sum().backward()
+  assert C.grad.dtype == alpha.grad.dtype == geo.grad.dtype == torch.float32
+  assert rel_err(C.grad.cpu().numpy(), dC_o) < 1e-4
+  assert rel_err(alpha.grad.cpu().numpy(), da_o) < 1e-4
+  assert rel_err(geo.grad.cpu().numpy(), dg_o) < 1e-4
+
+
+@pytest.mark.gpu
+def test_backward_f32_full_grid():
+  '''Full 80^3 grid of the gradient workload: FP32 gradients against the FP64
+  backward for the same upstream gradient and FP32-rounded parameters, the
+  homogeneity identity in FP32, and the same gradients from a workspace
+  small enough to force many chunks.'''
+  torch = _gpu()
+  from orbkit_amd import workloads
+  from orbkit_amd.autograd import DensityModel
+  qc, axes, _ = workloads.make('gradient')
+  print('case gradient 80^3 f32', flush=True)
+  m32 = DensityModel(qc, axes, precision='f32')
+  m64 = DensityModel(qc, axes)
+  C, alpha, geo, _ = _f32_params(torch, m32)
+  C64, a64, g64 = (t.detach().double() for t in (C, alpha, geo))
+  rho64 = m64.rho(C64, a64, g64)
+  g = torch.randn(rho64.shape, dtype=torch.float64, device=rho64.device,
+                  generator=torch.Generator(device=rho64.device).manual_seed(1))
+  ref = m64.vjp(C64, a64, g64, g)
+  got = m32.vjp(C, alpha, geo, g.float())
+  errs = [rel_err(a.cpu().numpy(), b.cpu().numpy()) for a, b in zip(got, ref)]
+  print('f32 vs f64 full grid: dC %.2e dalpha %.2e dgeo %.2e' % tuple(errs), flush=True)
+  assert max(errs) < 3e-5
+  rho32 = m32.rho(C, alpha, geo)
+  assert rel_err(rho32.cpu().numpy(), rho64.cpu().numpy()) < 1e-6
+  lhs = float((C64 * got[0].double()).sum())
+  rhs = 2.0 * float((g * rho64).sum())
+  assert abs(lhs - rhs) <= 1e-5 * float((g * rho64).abs().sum())
+  small = DensityModel(qc, axes, precision='f32', workspace_bytes=24 << 20)
+  again = small.vjp(C, alpha, geo, g.float())
+  for a, b in zip(again, got):
+    assert rel_err(a.cpu().numpy(), b.cpu().numpy()) < 1e-6
