@@ -97,7 +97,8 @@ def cpu_baseline(qc, axes, threads, slice_length, codes, slices_per_thread=2, re
   contiguous run from the middle of the flattened grid, slices_per_thread
   slices of slice_length points per thread.  One warm-up, then the median of
   `reps` timed runs; the same on one thread (one warm-up, median of 3) on
-  a sample of one slice."""
+  a sample of one slice; and the tuned CPU form (oracle/cpu_tuned.c) on the
+  first sample with the same threads."""
   from oracle import core_oracle
   X, Y, Z = numpy.meshgrid(*axes, indexing='ij')
   P = numpy.stack([X.ravel(), Y.ravel(), Z.ravel()])
@@ -107,26 +108,32 @@ def cpu_baseline(qc, axes, threads, slice_length, codes, slices_per_thread=2, re
     a = max(0, P.shape[1] // 2 - n // 2)
     return n, P[0, a:a + n].copy(), P[1, a:a + n].copy(), P[2, a:a + n].copy()
 
-  def run(nthreads, n, reps):
+  def run(nthreads, n, reps, fn=core_oracle.rho_slices_c):
     n, x, y, z = sample(n)
-    core_oracle.rho_slices_c(qc, x[:slice_length // 4], y[:slice_length // 4], z[:slice_length // 4], codes,
-                             slice_length=slice_length, nthreads=nthreads)
+    fn(qc, x[:slice_length // 4], y[:slice_length // 4], z[:slice_length // 4], codes,
+       slice_length=slice_length, nthreads=nthreads)
     ts = []
     for _ in range(reps):
       t = time.perf_counter()
-      core_oracle.rho_slices_c(qc, x, y, z, codes, slice_length=slice_length, nthreads=nthreads)
+      fn(qc, x, y, z, codes, slice_length=slice_length, nthreads=nthreads)
       ts.append(time.perf_counter() - t)
     return n, float(numpy.median(ts)), ts
 
   info = cpu_info()
   n, dt, ts = run(threads, threads * slice_length * slices_per_thread, reps)
   n1, dt1, _ = run(1, slice_length, 3)
+  nt, dtt, tst = run(threads, threads * slice_length * slices_per_thread, reps, core_oracle.rho_slices_tuned)
   return {'value': n / dt, 'unit': 'grid points/s', 'cores': threads, 'kind': 'port',
           'sample': '%d of the %d grid points (%d slices of %d, rho + derivative codes %s, FP64); median of %d '
                     'runs after one warm-up: %.2f s (runs %s)'
                     % (n, P.shape[1], -(-n // slice_length), slice_length, [int(c) for c in codes], reps, dt,
                        ', '.join('%.2f' % t for t in ts)),
           'one_core': {'value': n1 / dt1, 'sample': '%d points, median of 3 after one warm-up: %.2f s' % (n1, dt1)},
+          'tuned': {'value': nt / dtt, 'cores': threads,
+                    'sample': 'the same %d points and threads; orbkit\'s redundant work removed (one exponential '
+                              'per primitive and point shared by all components and AO sets, row map folded into '
+                              'C, blocked MO contraction: oracle/cpu_tuned.c); median of %d runs after one '
+                              'warm-up: %.2f s (runs %s)' % (nt, reps, dtt, ', '.join('%.2f' % t for t in tst))},
           'cpu_model': info['model'], 'nproc': info['nproc'], 'affinity_cpus': info['affinity'],
           'threads_note': 'threads = the CPUs this job may use (affinity, capped by OMP_NUM_THREADS where set)'}
 

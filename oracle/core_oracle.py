@@ -31,12 +31,14 @@ import numpy
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, 'liborbkit_oracle.so')
+TUNED_LIB = os.path.join(HERE, 'libcpu_tuned.so')
 REF_LIB = os.path.join(HERE, '_ref', 'libref_orbkit.so')
 
 _P_i = ctypes.POINTER(ctypes.c_int)
 _P_d = ctypes.POINTER(ctypes.c_double)
 _lib = None
 _ref = None
+_tuned = None
 
 
 def build():
@@ -318,3 +320,45 @@ def rho_slices_c(qc, x, y, z, drv_codes=(), slice_length=10000, nthreads=1):
   if rc:
     raise RuntimeError('orc_rho_slices failed (%d)' % rc)
   return rho, delta[:len(codes)], mo_norm
+
+
+def tuned_lib():
+  global _tuned
+  if _tuned is None:
+    if not os.path.exists(TUNED_LIB):
+      build()
+    t = ctypes.CDLL(TUNED_LIB)
+    t.orc_rho_slices_tuned.restype = ctypes.c_int
+    t.orc_rho_slices_tuned.argtypes = [_P_d, _P_d, _P_i, ctypes.c_int, _P_i, _P_i, _P_d, _P_i, _P_d, _P_i,
+                                       ctypes.c_int, ctypes.c_int, _P_d, _P_d, ctypes.c_int, _P_d, _P_d, _P_d,
+                                       ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
+    _tuned = t
+  return _tuned
+
+
+def rho_slices_tuned(qc, x, y, z, drv_codes=(), slice_length=10000, nthreads=1):
+  """rho_slices_c's results from the tuned CPU form (oracle/cpu_tuned.c):
+  exponentials shared across components and AO sets, the row map folded into
+  C, a blocked MO contraction.  Returns (rho, delta (len(codes), n))."""
+  from orbkit_amd.tools import get_cart2sph
+  L = tuned_lib()
+  geo, ao_spec, mo_spec = qc.geo_spec, qc.ao_spec, qc.mo_spec
+  f = flat_basis(geo, ao_spec)
+  nao_cart = int(numpy.sum(f['ncart']))
+  N = numpy.asarray(ao_spec[0]['N'], dtype=float).reshape(-1) if 'N' in ao_spec[0] else numpy.ones(nao_cart)
+  T = cartesian2spherical(numpy.diag(N), ao_spec, get_cart2sph) if ao_spec.spherical else numpy.diag(N)
+  TC = numpy.ascontiguousarray(mo_spec.get_coeffs() @ T)
+  occ = numpy.ascontiguousarray(mo_spec.get_occ())
+  x, px = _d(x); y, py = _d(y); z, pz = _d(z)
+  n = len(x)
+  codes = numpy.ascontiguousarray(drv_codes, dtype=numpy.intc)
+  rho = numpy.zeros(n)
+  delta = numpy.zeros((max(len(codes), 1), n))
+  keep = [_i(f['lxlylz']), _i(f['ncart']), _d(f['prims']), _i(f['nprim']), _d(f['geo']), _i(f['atoms'])]
+  rc = L.orc_rho_slices_tuned(rho.ctypes.data_as(_P_d), delta.ctypes.data_as(_P_d), codes.ctypes.data_as(_P_i),
+                              len(codes), keep[0][1], keep[1][1], keep[2][1], keep[3][1], keep[4][1], keep[5][1],
+                              len(f['ncart']), nao_cart, _d(TC)[1], _d(occ)[1], len(TC), px, py, pz, n,
+                              int(slice_length), f['normalized'], int(nthreads))
+  if rc:
+    raise RuntimeError('orc_rho_slices_tuned failed (%d)' % rc)
+  return rho, delta[:len(codes)]

@@ -253,3 +253,22 @@ def test_sliced_c_driver_matches_numpy_oracle():
   r, d, nrm = O.rho_slices_c(qc, v[0], v[1], v[2], [1, 2, 3, 5], slice_length=97, nthreads=2)
   rr, dd = O.rho_compute(qc, v[0], v[1], v[2], True, drv=['x', 'y', 'z', 'yy'])
   assert rel_err(r, rr) < 1e-14 and rel_err(d, dd) < 1e-14
+
+
+@pytest.mark.parametrize('name,prenorm', [('small', False), ('medium', False), ('medium', True)])
+def test_tuned_cpu_baseline_matches_c_driver(name, prenorm):
+  '''The tuned CPU form (shared exponentials, row map folded into C, blocked
+  contraction) gives the sliced C driver's results: it is timed as a CPU
+  baseline, so it must compute the same thing.'''
+  qc, axes, _ = workloads.make(name, npts_side=8)
+  if prenorm:
+    for c in qc.ao_spec.data:
+      c['pnum'] = -abs(c['pnum'])
+    qc.ao_spec.update()
+  v = O.expand(*axes)
+  codes = [1, 2, 3, 4, 5, 6]
+  r, d, _ = O.rho_slices_c(qc, v[0], v[1], v[2], codes, slice_length=97, nthreads=2)
+  rt, dt = O.rho_slices_tuned(qc, v[0], v[1], v[2], codes, slice_length=61, nthreads=2)
+  assert rel_err(rt, r) < 1e-13
+  for k in range(len(codes)):
+    assert rel_err(dt[k], d[k]) < 1e-12, codes[k]
