@@ -37,8 +37,9 @@ from .basis import DeviceBasis, flatten
 from .core import DeviceGrid, _stream
 
 #: device workspace of the backward pass (per chunk of points: AO values, MO
-#: values, dL/dA, about 7 KB per point at 400 AOs and 40 MOs in FP64)
-WORKSPACE_BYTES = 1 << 30
+#: values, dL/dA, about 7 KB per point at 400 AOs and 40 MOs in FP64): 4 GiB
+#: holds a 512,000-point grid in one chunk, enough waves to fill the GPU
+WORKSPACE_BYTES = 4 << 30
 PRECISIONS = ('f64', 'f32')
 
 

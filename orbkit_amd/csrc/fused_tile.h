@@ -317,12 +317,19 @@ __device__ __forceinline__ void load_c(const FusedArgs& a, int k0, int g, int n1
             for (int ks = 0; ks < KC / 4; ++ks)
                 c.cd[mt][ks] = C[(int64_t)(mt * 16 + n16) * a.kpad + k0 + 4 * ks + g];
     } else if constexpr (PREC == OKMI_PREC_F32) {
+        // K step ks of lane group g takes chunk row 8 g + ks (the same in
+        // contract), so a lane's 8 values are contiguous: two 16-byte loads
         const float* C = (const float*)a.C;
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
+        for (int mt = 0; mt < MT; ++mt) {
+            const f4* src = (const f4*)(C + (int64_t)(mt * 16 + n16) * a.kpad + k0 + 8 * g);
+            const f4 lo = src[0], hi = src[1];
 #pragma unroll
-            for (int ks = 0; ks < KC / 4; ++ks)
-                c.cf[mt][ks] = C[(int64_t)(mt * 16 + n16) * a.kpad + k0 + 4 * ks + g];
+            for (int j = 0; j < 4; ++j) {
+                c.cf[mt][j] = lo[j];
+                c.cf[mt][4 + j] = hi[j];
+            }
+        }
     } else {
         const __bf16* C = (const __bf16*)a.C;
         constexpr int NP = PREC == OKMI_PREC_BF16X6 ? 3 : 2;
@@ -462,7 +469,7 @@ __device__ __forceinline__ void contract(const typename Tile<PREC>::T* L, int co
         } else if constexpr (PREC == OKMI_PREC_F32) {
 #pragma unroll
             for (int ks = 0; ks < KC / 4; ++ks) {
-                const float b = src[(4 * ks + g) * LD];
+                const float b = src[(8 * g + ks) * LD];   // chunk row 8 g + ks, as load_c
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
                     acc[mt][s] = __builtin_amdgcn_mfma_f32_16x16x4f32(c.cf[mt][ks], b, acc[mt][s], 0, 0, 0);
@@ -591,14 +598,18 @@ __device__ __forceinline__ Box tile_box(const FusedArgs& a, int64_t tile, double
 // AO tile plus up to 11.5 KB of C stage) and run at 3 waves/SIMD; the others
 // keep two buffers (66.5 KB, 2 waves/SIMD), where holding C through the
 // evaluation hides its latency.  FP64 always has one buffer (64 KB for four
-// AO sets) and two workgroups per CU; its C operand is loaded by each wave
-// before the evaluation and held through it (c_early).
+// AO sets) and two workgroups per CU.  FP64 and FP32 load their C operand
+// into registers before the evaluation and hold it through it (hold_c).
 template <int PREC, int PASS, int LM, int MT, bool BATCH> struct FusedPlan {
     static constexpr bool f64 = PREC == OKMI_PREC_F64;
     static constexpr bool single = f64 || (PASS <= 1 && LM <= 3 && MT <= (BATCH ? 2 : 3));
     static constexpr int buffers = single ? 1 : 2;
     static constexpr int occupancy = f64 ? 2 : single ? 3 : 2;
     static constexpr bool c_early = f64 ? MT <= 3 : PASS <= 1 ? MT <= 3 : MT <= 2;
+    // one-buffer schedule, C in registers (FP64, FP32): loaded before the
+    // evaluation and held through it, its latency hidden there (FP32 with 3
+    // MO tiles and four AO sets would exceed the 168-VGPR budget)
+    static constexpr bool hold_c = c_early && (f64 || (PREC == OKMI_PREC_F32 && (PASS == 0 || MT <= 2)));
 };
 
 // One tile of 64 points by four waves that all evaluate and all contract:
@@ -641,7 +652,7 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, typen
             CRegs<PREC, MT> cr;
             COperand<PREC, MT> c;
             if constexpr (CStage<PREC, MT>::on) fetch_c<PREC, MT>(a, ch * KC, cr);
-            else if constexpr (P::f64 && P::c_early) load_c<PREC, MT>(a, ch * KC, g, n16, c);
+            else if constexpr (P::hold_c) load_c<PREC, MT>(a, ch * KC, g, n16, c);
             eval_chunk<PASS, LM>(a, ch, box, px, py, pz, lds, &next[0], &live[0], ch, wave == 0, lane);
             // the staging area is free: every wave read it before the
             // previous barrier
@@ -652,7 +663,7 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, typen
             if (threadIdx.x == 0) next[0] = 0;
             if (live[0] == ch) {
                 if constexpr (CStage<PREC, MT>::on) read_c<PREC, MT>(cst, g, n16, c);
-                else if constexpr (!(P::f64 && P::c_early)) load_c<PREC, MT>(a, ch * KC, g, n16, c);
+                else if constexpr (!P::hold_c) load_c<PREC, MT>(a, ch * KC, g, n16, c);
                 contract<PREC, PASS, MT>(lds, wave, g, n16, c, acc);
             }
             __syncthreads();

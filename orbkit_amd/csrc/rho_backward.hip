@@ -36,7 +36,7 @@
 namespace {
 
 constexpr int MAXP = 16;     // primitives per shell handled by one block (more: extra windows)
-constexpr int RANGE = 16384; // points per shell-gradient block
+constexpr int RANGE = 4096;  // points per shell-gradient block (16 steps of 256: many blocks in flight)
 
 // 16x16x4 MFMA of each arithmetic.  Item i of lane l of the accumulator
 // holds D[row(i, l/16)][l%16]: FP64 row 4i + l/16, FP32 row 4(l/16) + i.
@@ -255,15 +255,24 @@ __global__ __launch_bounds__(64, 2) void k_gemm_tn(const R* __restrict__ B, int 
     }
 }
 
-// dC[e] += sum_s part[s][e] for e < n, s = 0 .. nsplit - 1 in order (FP64 sum)
+// dC[e] += sum_s part[s][e] for e < n over s = 0 .. nsplit - 1 in FP64, in
+// a fixed order: eight interleaved partial sums (s mod 8, eight independent
+// loads in flight per thread), then combined in index order
 template <typename R>
 __global__ void k_reduce_dc(const R* part, int nsplit, int64_t n, R* dC)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
-    double v = 0.0;
-    for (int s = 0; s < nsplit; ++s) v += (double)part[(int64_t)s * n + e];
-    dC[e] = (R)((double)dC[e] + v);
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int s = 0;
+    for (; s + 8 <= nsplit; s += 8)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] += (double)part[(int64_t)(s + j) * n + e];
+    for (int j = 0; s + j < nsplit; ++j) v[j] += (double)part[(int64_t)(s + j) * n + e];
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) t += v[j];
+    dC[e] = (R)((double)dC[e] + t);
 }
 
 __device__ __forceinline__ double wave_sum(double v)
