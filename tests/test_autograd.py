@@ -231,8 +231,9 @@ def test_backward_f32_vs_oracle(name, prenorm):
   errs = [rel_err(rho.detach().cpu().numpy(), r_o), rel_err(dC.cpu().numpy(), dC_o), rel_err(da.cpu().numpy(), da_o),
           rel_err(dg.cpu().numpy(), dg_o)]
   print('f32 errors rho %.2e dC %.2e dalpha %.2e dgeo %.2e' % tuple(errs), flush=True)
+  # measured: rho <= 2.4e-7, gradients <= 4.6e-7
   assert errs[0] < 1e-6
-  assert max(errs[1:]) < 3e-5
+  assert max(errs[1:]) < 2e-6
   # through torch.autograd: the FP32 loss of the FP32 forward
   wt = torch.as_tensor(w, device=rho.device, dtype=torch.float32)
   tt = torch.as_tensor(t, device=rho.device, dtype=torch.float32)
@@ -265,7 +266,7 @@ def test_backward_f32_full_grid():
   got = m32.vjp(C, alpha, geo, g.float())
   errs = [rel_err(a.cpu().numpy(), b.cpu().numpy()) for a, b in zip(got, ref)]
   print('f32 vs f64 full grid: dC %.2e dalpha %.2e dgeo %.2e' % tuple(errs), flush=True)
-  assert max(errs) < 3e-5
+  assert max(errs) < 2e-6   # measured 4.0e-7
   rho32 = m32.rho(C, alpha, geo)
   assert rel_err(rho32.cpu().numpy(), rho64.cpu().numpy()) < 1e-6
   lhs = float((C64 * got[0].double()).sum())
