@@ -48,6 +48,7 @@ struct BigArgs {
     const double* occ;     // [mpad], zero padded
     double* partial;       // [nmt][S][ld]
     int nmt;
+    int nmo;               // MOs (16-MO tiles at or past nmo are padding: their MFMAs are skipped)
     int64_t nblocks;
 };
 
@@ -98,6 +99,10 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
     const int wm = wave & 1, wn = wave >> 1;
     const int g = lane >> 4, n16 = lane & 15;
     const int nk = (a.nao + KS - 1) / KS;
+    // 16-MO tiles of this wave that hold MOs (wave-uniform): the last MO
+    // tile group of a padded count skips the MFMAs (and the AO split) of
+    // its padding tiles, e.g. 400 MOs = 25 tiles of the 32 launched
+    const int nv = min(4, max(0, (a.nmo - m0 - 64 * wm + 15) / 16));
 
     // ---- global -> register staging
     constexpr int AO_V4 = S * KS * NP / 4 / NT;           // float4 per thread per step
@@ -105,9 +110,11 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
     constexpr int C_V4 = PREC == OKMI_PREC_F32 ? MB * KS / 4 / NT : NPL * MB * KS / 8 / NT;   // 16-byte pieces
     static_assert((PREC == OKMI_PREC_F32 ? MB * KS / 4 : NPL * MB * KS / 8) % NT == 0, "C tile split");
     typedef float fl4 __attribute__((ext_vector_type(4)));
-    fl4 rao[AO_V4];
-    fl4 rc[C_V4];
-    auto gload = [&](int ks) {
+    struct Stage {
+        fl4 ao[AO_V4];
+        fl4 c[C_V4];
+    };
+    auto gload = [&](int ks, Stage& R) {
         const int k0 = ks * KS;
 #pragma unroll
         for (int v = 0; v < AO_V4; ++v) {
@@ -115,24 +122,24 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
             const int pc = e % (NP / 4), r = (e / (NP / 4)) % KS, s = e / (NP / 4 * KS);
             const int k = k0 + r;
             if (k < a.nao)
-                rao[v] = *(const fl4*)(a.ao + s * a.set_stride + (int64_t)k * a.ld + p0 + 4 * pc);
+                R.ao[v] = *(const fl4*)(a.ao + s * a.set_stride + (int64_t)k * a.ld + p0 + 4 * pc);
             else
-                rao[v] = fl4{0.f, 0.f, 0.f, 0.f};
+                R.ao[v] = fl4{0.f, 0.f, 0.f, 0.f};
         }
 #pragma unroll
         for (int v = 0; v < C_V4; ++v) {
             const int e = v * NT + threadIdx.x;
             if constexpr (PREC == OKMI_PREC_F32) {
                 const int kc = e % (KS / 4), r = e / (KS / 4);
-                rc[v] = *(const fl4*)((const float*)a.C + (int64_t)(m0 + r) * a.kpad + k0 + 4 * kc);
+                R.c[v] = *(const fl4*)((const float*)a.C + (int64_t)(m0 + r) * a.kpad + k0 + 4 * kc);
             } else {
                 const int kc = e % (KS / 8), r = (e / (KS / 8)) % MB, pl = e / (KS / 8 * MB);
-                rc[v] = *(const fl4*)((const __bf16*)a.C + (int64_t)pl * a.mpad * a.kpad + (int64_t)(m0 + r) * a.kpad +
-                                      k0 + 8 * kc);
+                R.c[v] = *(const fl4*)((const __bf16*)a.C + (int64_t)pl * a.mpad * a.kpad + (int64_t)(m0 + r) * a.kpad +
+                                       k0 + 8 * kc);
             }
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](int buf, const Stage& R) {
         float* ao = (float*)(smem + buf * LD::STAGE);
         unsigned char* cl = smem + buf * LD::STAGE + LD::AO_FLOATS * 4;
 #pragma unroll
@@ -140,18 +147,18 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
             const int e = v * NT + threadIdx.x;
             const int pc = e % (NP / 4), r = (e / (NP / 4)) % KS, s = e / (NP / 4 * KS);
             float* d = ao + (s * KS + r) * AOLD + 4 * pc;   // 8-byte aligned (AOLD even)
-            *(float2*)d = float2{rao[v][0], rao[v][1]};
-            *(float2*)(d + 2) = float2{rao[v][2], rao[v][3]};
+            *(float2*)d = float2{R.ao[v][0], R.ao[v][1]};
+            *(float2*)(d + 2) = float2{R.ao[v][2], R.ao[v][3]};
         }
 #pragma unroll
         for (int v = 0; v < C_V4; ++v) {
             const int e = v * NT + threadIdx.x;
             if constexpr (PREC == OKMI_PREC_F32) {
                 const int kc = e % (KS / 4), r = e / (KS / 4);
-                *(fl4*)((float*)cl + r * CLD_F + 4 * kc) = rc[v];
+                *(fl4*)((float*)cl + r * CLD_F + 4 * kc) = R.c[v];
             } else {
                 const int kc = e % (KS / 8), r = (e / (KS / 8)) % MB, pl = e / (KS / 8 * MB);
-                *(fl4*)((__bf16*)cl + (pl * MB + r) * CLD_BF + 8 * kc) = rc[v];
+                *(fl4*)((__bf16*)cl + (pl * MB + r) * CLD_BF + 8 * kc) = R.c[v];
             }
         }
     };
@@ -164,12 +171,7 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
 #pragma unroll
             for (int s = 0; s < S; ++s) acc[rb][c][s] = f4{0.f, 0.f, 0.f, 0.f};
 
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    int buf = 0;
-    for (int ks = 0; ks < nk; ++ks) {
-        if (ks + 1 < nk) gload(ks + 1);
+    auto compute = [&](int buf) {
         const float* ao = (const float*)(smem + buf * LD::STAGE);
         const unsigned char* cl = smem + buf * LD::STAGE + LD::AO_FLOATS * 4;
         if constexpr (PREC == OKMI_PREC_F32) {
@@ -186,7 +188,9 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
                         const float bv = ao[(s * KS + 4 * kk + g) * AOLD + (wn * NPW + c) * 16 + n16];
 #pragma unroll
                         for (int rb = 0; rb < 4; ++rb)
-                            acc[rb][c][s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rb], bv, acc[rb][c][s], 0, 0, 0);
+                            if (rb < nv)
+                                acc[rb][c][s] =
+                                    __builtin_amdgcn_mfma_f32_16x16x4f32(av[rb], bv, acc[rb][c][s], 0, 0, 0);
                     }
             }
         } else {
@@ -196,22 +200,58 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
 #pragma unroll
                 for (int pl = 0; pl < NPL; ++pl)
                     A[rb].p[pl] = *(const bf8*)((const __bf16*)cl + (pl * MB + 64 * wm + 16 * rb + n16) * CLD_BF + 8 * g);
+            if (nv > 0) {
 #pragma unroll
-            for (int s = 0; s < S; ++s)
+                for (int s = 0; s < S; ++s)
 #pragma unroll
-                for (int c = 0; c < NPW; ++c) {
-                    float bx[8];
+                    for (int c = 0; c < NPW; ++c) {
+                        float bx[8];
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) bx[j] = ao[(s * KS + 8 * g + j) * AOLD + (wn * NPW + c) * 16 + n16];
-                    Split<PREC> B;
-                    split8<PREC>(bx, B);
+                        for (int j = 0; j < 8; ++j) bx[j] = ao[(s * KS + 8 * g + j) * AOLD + (wn * NPW + c) * 16 + n16];
+                        Split<PREC> B;
+                        split8<PREC>(bx, B);
 #pragma unroll
-                    for (int rb = 0; rb < 4; ++rb) acc[rb][c][s] = mfma_split<PREC>(A[rb], B, acc[rb][c][s]);
-                }
+                        for (int rb = 0; rb < 4; ++rb)
+                            if (rb < nv) acc[rb][c][s] = mfma_split<PREC>(A[rb], B, acc[rb][c][s]);
+                    }
+            }
         }
-        if (ks + 1 < nk) sstore(buf ^ 1);
+    };
+
+    // K loop: LDS double buffered, one barrier per step; step ks computes on
+    // buffer ks & 1.  Up to 4 sets the next step's global loads are issued
+    // before the step's MFMAs (one register stage).  With 7 or 10 sets (one
+    // wave per SIMD, nothing else to hide the latency) they are issued two
+    // steps ahead in two register stages: R_next holds step ks + 1
+    // (loading), R_free held step ks (already in LDS) and takes step ks + 2
+    // (measured at C5: S = 7 280 -> 266 ms; S = 4 125 -> 137 ms, so S = 4
+    // keeps one stage).
+    Stage R0, R1;
+    gload(0, R0);
+    sstore(0, R0);
+    if constexpr (S >= 7) {
+        if (nk > 1) gload(1, R1);
         __syncthreads();
-        buf ^= 1;
+        auto body = [&](int ks, int buf, Stage& Rnext, Stage& Rfree) {
+            if (ks + 2 < nk) gload(ks + 2, Rfree);
+            compute(buf);
+            if (ks + 1 < nk) sstore(buf ^ 1, Rnext);
+            __syncthreads();
+        };
+        for (int ks = 0; ks < nk; ks += 2) {
+            body(ks, 0, R1, R0);
+            if (ks + 1 < nk) body(ks + 1, 1, R0, R1);
+        }
+    } else {
+        __syncthreads();
+        int buf = 0;
+        for (int ks = 0; ks < nk; ++ks) {
+            if (ks + 1 < nk) gload(ks + 1, R0);
+            compute(buf);
+            if (ks + 1 < nk) sstore(buf ^ 1, R0);
+            __syncthreads();
+            buf ^= 1;
+        }
     }
 
     // ---- epilogue: lane holds Phi_s[m = m0 + 64 wm + 16 rb + 4 g + i][p0 + (wn NPW + c) 16 + n16]
@@ -447,7 +487,7 @@ int okmi_launch_rho_big(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np,
         if (rc) return rc;
         BigArgs a{};
         a.ao = ao; a.ld = chunk; a.set_stride = (int64_t)b->nao * chunk; a.nao = b->nao; a.np = n;
-        a.C = cpl; a.mpad = L.mpad; a.kpad = L.kpad; a.occ = occp; a.partial = part; a.nmt = nmt;
+        a.C = cpl; a.mpad = L.mpad; a.kpad = L.kpad; a.occ = occp; a.partial = part; a.nmt = nmt; a.nmo = nmo;
         a.nblocks = (int64_t)nmt * ((n + NP - 1) / NP);
         switch (prec) {
         case OKMI_PREC_F32: rc = launch_big<OKMI_PREC_F32>(a, S, st); break;
