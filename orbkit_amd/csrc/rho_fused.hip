@@ -57,18 +57,26 @@ int upload(T** d, const std::vector<T>& h, const char* what)
 }
 
 // Internal row layout of the fused kernels: every 32-row chunk holds whole
-// shells, packed from its first row.  Shells are taken in basis order, each
-// placed in the first chunk with room for it (first fit), so the chunks are
-// nearly full (every padding row still costs MFMA work) and the shells of a
-// chunk come from neighbouring atoms.  E.g. 20 atoms x [s,s,p,p,d,f]
-// (spherical, 400 rows): 13 chunks, where starting a new chunk at every
-// shell that does not fit took 14.  Returns the padded row count (-1 if a
+// shells, packed from its first row.  Shells are taken by screening radius,
+// largest first (ties in basis order), each placed in the first chunk with
+// room for it (first fit), so the chunks are nearly full (every padding row
+// still costs MFMA work) and the diffuse shells, which reach every tile,
+// share the first chunks while the last chunks hold only tight shells,
+// which reach the tiles near their own atoms: a tile skips the contraction
+// of every chunk none of whose shells reaches it.  E.g. 20 atoms x
+// [s,s,p,p,d,f] (spherical, 400 rows, exponents 0.05 .. 1e5): 13 chunks
+// (starting a new chunk at every shell that does not fit took 14), and of
+// the 100^3 grid's 4x4x4 tiles an average of 7.0 chunks live per tile
+// instead of 11.4 in basis order.  Returns the padded row count (-1 if a
 // shell has more than 32 rows) and each shell's first internal row.
 int fused_layout(const OkmiBasis* b, std::vector<int>* krow)
 {
     std::vector<int> fill;   // rows used per chunk
     if (krow) krow->assign(b->nshell, 0);
-    for (int s = 0; s < b->nshell; ++s) {
+    std::vector<int> order(b->nshell);
+    for (int s = 0; s < b->nshell; ++s) order[s] = s;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->h_frcut2[x] > b->h_frcut2[y]; });
+    for (int s : order) {
         const int n = b->h_shells[s].nrow;
         if (n > KC) return -1;
         size_t c = 0;
