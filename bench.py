@@ -183,7 +183,19 @@ def timed(step, args, L, names, sync):
 
 # ---------------------------------------------------------------- workloads
 
-def density_leg(args, torch, info, basis, dg, C, occ, codes, path):
+def screened_rows(L, basis, axes):
+  """Sum over the grid's 4x4x4 bricks of (AO rows of the chunks the fused
+  kernels contract there) x (points in the brick): okmi_fused_live_rows."""
+  from orbkit_amd import _native
+  ax = [numpy.ascontiguousarray(a, dtype=numpy.float64) for a in axes]
+  P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+  out = ctypes.c_double()
+  _native.check(L.okmi_fused_live_rows(basis.handle, P(ax[0]), len(ax[0]), P(ax[1]), len(ax[1]), P(ax[2]), len(ax[2]),
+                                       ctypes.byref(out)), 'okmi_fused_live_rows')
+  return out.value
+
+
+def density_leg(args, torch, info, basis, dg, C, occ, codes, path, axes=None):
   """One way of computing rho (+ grad, + Laplacian) over the whole grid,
   timed: path 'f64' (fused FP64), 'f64_staged' (AO sets through HBM, then
   the FP64 GEMM with density epilogue), or a reduced precision of the fused
@@ -235,6 +247,15 @@ def density_leg(args, torch, info, basis, dg, C, occ, codes, path):
           'algorithmic_flop_per_step': flops}
   if dom[3] > 1:
     roof.update(executed_tflops=achieved * dom[3], mfma_dtype=dom[2], products=dom[3])
+  if axes is not None and dom[0] in ('rho_fused_f64', 'rho_fused', 'rho_big'):
+    # the screened contraction: per brick only the AO chunks with a shell in
+    # reach (the work the kernel does besides MO-tile padding)
+    sflops = 2.0 * nmo * nsets * screened_rows(L, basis, axes)
+    roof['screened'] = {'flop_per_step': sflops, 'achieved': sflops / t_launch / 1e12,
+                        'frac': sflops / t_launch / 1e12 / peak, 'share_of_algorithmic': sflops / flops,
+                        'note': 'algorithmic flops count every AO at every point; the kernel skips the AO chunks '
+                                'whose shells do not reach a brick (values below 1e-24), so frac (algorithmic) can '
+                                'exceed 1 and screened.frac is the hardware utilisation'}
   roof['other_kernels_ms_per_step'] = {k: v[1] / args.steps for k, v in times.items() if k != dom[0]}
   return {'value': npts * args.steps / dt, 'ms_per_step': dt / args.steps * 1e3, 'roofline': roof}, result
 
@@ -269,7 +290,7 @@ def density_step(args, torch, info, qc, axes):
   codes = numpy.array([1, 2, 3] + ([4, 5, 6] if args.laplacian else []), dtype=numpy.intc)
   nsets = 7 if args.laplacian else 4
   nao, nmo = info['nao'], info['nmo']
-  head, head_result = density_leg(args, torch, info, basis, dg, C, occ, codes, args.precision)
+  head, head_result = density_leg(args, torch, info, basis, dg, C, occ, codes, args.precision, axes)
   out = {'value': head['value'], 'ms_per_step': head['ms_per_step'],
          'dtype': {'bf16x6': 'bf16x6 (fp32 acc, fp64 epilogue)', 'bf16x3': 'bf16x3 (fp32 acc, fp64 epilogue)',
                    'f32': 'f32 (fp64 epilogue)', 'f64': 'f64'}[args.precision],
@@ -290,7 +311,7 @@ def density_step(args, torch, info, qc, axes):
     out['error_vs_f64'] = error_vs(r, d, rr, dd, axes, args.laplacian)
   out['legs'] = {}
   for p in legs:
-    leg, res = density_leg(args, torch, info, basis, dg, C, occ, codes, p)
+    leg, res = density_leg(args, torch, info, basis, dg, C, occ, codes, p, axes)
     r, d = res()
     leg['error_vs_f64'] = error_vs(r, d, rr, dd, axes, args.laplacian)
     leg['speedup_vs_headline'] = head['ms_per_step'] / leg['ms_per_step']
@@ -389,6 +410,11 @@ def batch_step(args, torch):
           'frac': flops / t / 1e12 / peak, 'traffic': measured_traffic('k_rho_batch', 'batch', prec),
           'kernel': 'k_rho_batch', 'kernel_ms_per_step': t * 1e3, 'algorithmic_flop_per_step': flops,
           'tuned': {'tile_shape': tune['shape'], 'workgroups': tune['workgroups']}}
+  # the screened contraction (AO chunks in reach of each brick), per molecule
+  sflops = sum(2.0 * c.shape[0] * 4 * screened_rows(L, b, axes)
+               for c, b, (_, axes) in zip(plan.C, plan.bases, mols))
+  roof['screened'] = {'flop_per_step': sflops, 'achieved': sflops / t / 1e12, 'frac': sflops / t / 1e12 / peak,
+                      'share_of_algorithmic': sflops / flops}
   nao = [b.nao for b in plan.bases]
   return {'value': plan.npts * args.steps / dt, 'ms_per_step': dt / args.steps * 1e3,
           'dtype': '%s (fp32 acc, fp64 epilogue)' % prec,

@@ -353,6 +353,8 @@ extern "C" int okmi_basis_destroy(okmi_basis* b)
     if (b->d_fprims) (void)hipFree(b->d_fprims);
     if (b->d_fgroups) (void)hipFree(b->d_fgroups);
     if (b->d_oshells) (void)hipFree(b->d_oshells);
+    if (b->d_ishells) (void)hipFree(b->d_ishells);
+    if (b->d_padrows) (void)hipFree(b->d_padrows);
     if (b->d_colmap) (void)hipFree(b->d_colmap);
     delete[] b->h_shells;
     delete[] b->h_prims;

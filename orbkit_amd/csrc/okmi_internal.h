@@ -32,12 +32,17 @@ struct OkmiBasis {
     int2* d_chunks = nullptr;   // per chunk: first work entry, shells | used rows << 16
     ShellRec* d_fshells = nullptr;   // row0 = internal row, rcut2 = fused radius
     ShellRec* d_oshells = nullptr;   // output rows, rcut2 = fused radius
+    ShellRec* d_ishells = nullptr;   // internal rows (prim0 into d_fprims), rcut2 = fused radius
+    int* d_padrows = nullptr;        // internal rows that hold no AO (chunk padding)
+    int npadrows = 0;
     PrimRec* d_fprims = nullptr;     // rcut2 = fused radius
     PrimGroup* d_fgroups = nullptr;  // fused-kernel primitives, groups of 4 per shell in work order
     int* d_colmap = nullptr;         // [nkpad]
     int nchunk = 0;
     int nkpad = 0;
     std::vector<double> h_frcut2, h_fprcut2;
+    std::vector<int2> h_chunks;          // host copies of d_chunks and d_fshells
+    std::vector<ShellRec> h_fshells;
     int is_normalized = 0;
     int max_nrow = 0;      // most output rows of one shell (the fused kernels take up to 32)
 };

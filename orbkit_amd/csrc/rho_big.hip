@@ -49,6 +49,9 @@ struct BigArgs {
     double* partial;       // [nmt][S][ld]
     int nmt;
     int nmo;               // MOs (16-MO tiles at or past nmo are padding: their MFMAs are skipped)
+    const int* live;       // [point tile][nchunk]: the chunks (32-row K steps) that reach the tile, in order
+    const int* live_count; // [point tile]
+    int nchunk;
     int64_t nblocks;
 };
 
@@ -98,7 +101,10 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int g = lane >> 4, n16 = lane & 15;
-    const int nk = (a.nao + KS - 1) / KS;
+    // the K steps of this point tile: the 32-row chunks with a shell that
+    // reaches one of its points (k_big_live), in row order
+    const int* lst = a.live + tn * a.nchunk;
+    const int cnt = a.live_count[tn];
     // 16-MO tiles of this wave that hold MOs (wave-uniform): the last MO
     // tile group of a padded count skips the MFMAs (and the AO split) of
     // its padding tiles, e.g. 400 MOs = 25 tiles of the 32 launched
@@ -218,39 +224,41 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
         }
     };
 
-    // K loop: LDS double buffered, one barrier per step; step ks computes on
-    // buffer ks & 1.  Up to 4 sets the next step's global loads are issued
-    // before the step's MFMAs (one register stage).  With 7 or 10 sets (one
-    // wave per SIMD, nothing else to hide the latency) they are issued two
-    // steps ahead in two register stages: R_next holds step ks + 1
-    // (loading), R_free held step ks (already in LDS) and takes step ks + 2
-    // (measured at C5: S = 7 280 -> 266 ms; S = 4 125 -> 137 ms, so S = 4
-    // keeps one stage).
-    Stage R0, R1;
-    gload(0, R0);
-    sstore(0, R0);
-    if constexpr (S >= 7) {
-        if (nk > 1) gload(1, R1);
-        __syncthreads();
-        auto body = [&](int ks, int buf, Stage& Rnext, Stage& Rfree) {
-            if (ks + 2 < nk) gload(ks + 2, Rfree);
-            compute(buf);
-            if (ks + 1 < nk) sstore(buf ^ 1, Rnext);
+    // K loop over the live chunks: LDS double buffered, one barrier per
+    // step; step i computes on buffer i & 1.  Up to 4 sets the next step's
+    // global loads are issued before the step's MFMAs (one register stage).
+    // With 7 or 10 sets (one wave per SIMD, nothing else to hide the
+    // latency) they are issued two steps ahead in two register stages:
+    // R_next holds step i + 1 (loading), R_free held step i (already in LDS)
+    // and takes step i + 2 (measured at C5: S = 7 280 -> 266 ms; S = 4
+    // 125 -> 137 ms, so S = 4 keeps one stage).
+    if (cnt > 0) {
+        Stage R0, R1;
+        gload(lst[0], R0);
+        sstore(0, R0);
+        if constexpr (S >= 7) {
+            if (cnt > 1) gload(lst[1], R1);
             __syncthreads();
-        };
-        for (int ks = 0; ks < nk; ks += 2) {
-            body(ks, 0, R1, R0);
-            if (ks + 1 < nk) body(ks + 1, 1, R0, R1);
-        }
-    } else {
-        __syncthreads();
-        int buf = 0;
-        for (int ks = 0; ks < nk; ++ks) {
-            if (ks + 1 < nk) gload(ks + 1, R0);
-            compute(buf);
-            if (ks + 1 < nk) sstore(buf ^ 1, R0);
+            auto body = [&](int i, int buf, Stage& Rnext, Stage& Rfree) {
+                if (i + 2 < cnt) gload(lst[i + 2], Rfree);
+                compute(buf);
+                if (i + 1 < cnt) sstore(buf ^ 1, Rnext);
+                __syncthreads();
+            };
+            for (int i = 0; i < cnt; i += 2) {
+                body(i, 0, R1, R0);
+                if (i + 1 < cnt) body(i + 1, 1, R0, R1);
+            }
+        } else {
             __syncthreads();
-            buf ^= 1;
+            int buf = 0;
+            for (int i = 0; i < cnt; ++i) {
+                if (i + 1 < cnt) gload(lst[i + 1], R0);
+                compute(buf);
+                if (i + 1 < cnt) sstore(buf ^ 1, R0);
+                __syncthreads();
+                buf ^= 1;
+            }
         }
     }
 
@@ -351,6 +359,104 @@ __global__ void k_reduce_big(const double* partial, int nmt, int S, int64_t ld, 
     }
 }
 
+// Live chunks of each point tile of NP columns: one wave per tile forms the
+// tile's bounding box from its points (brick order: the grid points of the
+// bricks' lanes; range order: points p0 + column), then tests every 32-row
+// chunk's shells against it (fused screening radii, the radii with which the
+// AO kernel writes zeros) and writes the chunks with a shell in reach, in
+// increasing order, to list[tile][..] and their number to count[tile].
+struct LiveArgs {
+    GridDesc grid;
+    BrickMap bm;
+    int64_t p0;          // range order: point index of column 0
+    int64_t n;           // columns of this chunk of points
+    int np;              // columns per tile
+    const int2* chunks;  // per chunk: first work entry, shells | used rows << 16
+    const ShellRec* shells;
+    int nchunk;
+    int* list;
+    int* count;
+};
+
+__device__ __forceinline__ double wave_reduce(double v, bool take_min)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double u = __shfl_xor(v, o);
+        v = take_min ? fmin(v, u) : fmax(v, u);
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(64) void k_big_live(LiveArgs a)
+{
+    const int64_t t = blockIdx.x;
+    const int lane = threadIdx.x;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int j = lane; j < a.np; j += 64) {
+        const int64_t col = t * a.np + j;
+        if (col >= a.n) continue;
+        double q[3];
+        if (a.bm.on) {
+            const int l = (int)(col & 63);
+            int64_t b = a.bm.bk0 + (col >> 6);
+            const int64_t bz = b % a.bm.nbz;
+            b /= a.bm.nbz;
+            const int64_t by = b % a.bm.nby, bx = b / a.bm.nby;
+            const int64_t ix = 4 * bx + (l >> 4), iy = 4 * by + ((l >> 2) & 3), iz = 4 * bz + (l & 3);
+            if (ix >= a.bm.nx || iy >= a.bm.ny || iz >= a.bm.nz) continue;
+            q[0] = a.grid.x[ix]; q[1] = a.grid.y[iy]; q[2] = a.grid.z[iz];
+        } else {
+            grid_point(a.grid, a.p0 + col, q[0], q[1], q[2]);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], q[d]);
+            hi[d] = fmax(hi[d], q[d]);
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = wave_reduce(lo[d], true);
+        hi[d] = wave_reduce(hi[d], false);
+    }
+    int base = 0;
+    for (int c0 = 0; c0 < a.nchunk; c0 += 64) {
+        const int c = c0 + lane;
+        bool live = false;
+        if (c < a.nchunk) {
+            const int2 cw = a.chunks[c];
+            const int nsh = cw.y & 0xffff;
+            for (int e = 0; e < nsh && !live; ++e) {
+                const ShellRec sh = a.shells[cw.x + e];
+                const double cc[3] = {sh.cx, sh.cy, sh.cz};
+                double d2 = 0.0;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double dd = fmax(fmax(lo[d] - cc[d], cc[d] - hi[d]), 0.0);
+                    d2 = fma(dd, dd, d2);
+                }
+                live = d2 < sh.rcut2;   // a tile with no point in the grid has lo > hi: d2 = inf
+            }
+        }
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(live);
+        if (live) a.list[t * a.nchunk + base + __builtin_popcountll(m & ((1ull << lane) - 1))] = c;
+        base += __builtin_popcountll(m);
+    }
+    if (lane == 0) a.count[t] = base;
+}
+
+// zero rows `rows[0 .. nrows)` of every AO set over columns [0, ncol)
+__global__ void k_zero_rows(float* ao, int S, int64_t set_stride, int64_t ld, const int* rows, int nrows,
+                            int64_t ncol)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)nrows * ncol;
+    if (i >= per * S) return;
+    const int64_t s = i / per, r = (i % per) / ncol, c = i % ncol;
+    ao[s * set_stride + (int64_t)rows[r] * ld + c] = 0.f;
+}
+
 template <int PREC, int S>
 int launch_big_s(const BigArgs& a, hipStream_t st)
 {
@@ -396,13 +502,14 @@ BigLayout big_layout(const OkmiBasis* b, int nmo, int prec, int S)
 {
     BigLayout L{};
     L.mpad = (int64_t)((nmo + MB - 1) / MB) * MB;
-    L.kpad = (int64_t)((b->nao + KS - 1) / KS) * KS;
+    L.kpad = std::max<int64_t>(b->nkpad, (int64_t)((b->nao + KS - 1) / KS) * KS);   // fused row order
     const size_t elem = prec == OKMI_PREC_F32 ? 4 : prec == OKMI_PREC_BF16X3 ? 4 : 6;
     L.c_bytes = a256(elem * L.mpad * L.kpad);
     L.occ_bytes = a256(8 * L.mpad);
-    // AO sets (FP32) + partials (FP64) per point
-    L.per_point = (size_t)S * b->nao * 4 + (size_t)(L.mpad / MB) * S * 8;
-    L.fixed = L.c_bytes + L.occ_bytes + 2 * 256 + 256;
+    // AO sets (FP32), partials (FP64) and the live-chunk lists (at least 32
+    // points per tile) per point
+    L.per_point = (size_t)S * L.kpad * 4 + (size_t)(L.mpad / MB) * S * 8 + ((size_t)L.kpad / KS + 1) * 4 / 32 + 1;
+    L.fixed = L.c_bytes + L.occ_bytes + 4 * 256 + 256;
     return L;
 }
 
@@ -459,11 +566,26 @@ int okmi_launch_rho_big(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np,
     void* cpl = w;
     double* occp = (double*)(w + L.c_bytes);
     float* ao = (float*)(w + L.c_bytes + L.occ_bytes);
-    double* part = (double*)((char*)ao + a256((size_t)S * b->nao * 4 * chunk));
     const int nmt = (int)(L.mpad / MB);
+    double* part = (double*)((char*)ao + a256((size_t)S * L.kpad * 4 * chunk));
+    const int NP = 32 * (S == 1 ? 4 : S == 4 ? 2 : 1);
+    const int64_t ntile_max = (chunk + NP - 1) / NP;
+    int* live = (int*)((char*)part + a256((size_t)nmt * S * 8 * chunk));
+    int* live_count = live + ntile_max * b->nchunk;
+    if (L.kpad != b->nkpad || (size_t)((char*)(live_count + ntile_max) - (char*)ws) > ws_bytes) {
+        okmi_set_error("rho_big: inconsistent layout (%lld rows, %d fused rows)", (long long)L.kpad, b->nkpad);
+        return 1;
+    }
 
+    // C in the fused row order (zero columns at the padding rows), the
+    // padding rows of the AO buffer zeroed once (no shell writes them)
     hipLaunchKernelGGL(k_prep_c, dim3((unsigned)((L.mpad * L.kpad + 255) / 256)), dim3(256), 0, st, d_c, nmo, b->nao,
-                       prec, L.mpad, L.kpad, (const int*)nullptr, cpl);
+                       prec, L.mpad, L.kpad, (const int*)b->d_colmap, cpl);
+    if (b->npadrows > 0) {
+        const int64_t nz = (int64_t)S * b->npadrows * chunk;
+        hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, ao, S,
+                           (int64_t)L.kpad * chunk, chunk, (const int*)b->d_padrows, b->npadrows, chunk);
+    }
     hipLaunchKernelGGL(k_pad_occ_big, dim3((unsigned)((L.mpad + 255) / 256)), dim3(256), 0, st, d_occ, nmo,
                        (int)L.mpad, occp);
     int codes[10];
@@ -471,24 +593,30 @@ int okmi_launch_rho_big(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np,
     OutCodes oc{};
     oc.n = nout;
     for (int r = 0; r < nout; ++r) oc.c[r] = drv_out[r];
-    const int NP = 32 * (S == 1 ? 4 : S == 4 ? 2 : 1);
     for (int64_t c0 = 0; c0 < total; c0 += chunk) {
         const int64_t n = total - c0 < chunk ? total - c0 : chunk;
         int rc;
         if (bm.on) {
             AoBricks bk{c0 / 64, n / 64, bm.nbx, bm.nby, bm.nbz};
             bm.bk0 = c0 / 64;
-            rc = okmi_launch_ao_eval_ex(b, b->d_oshells, b->d_fprims, g, 0, n, &bk, codes, S, OKMI_F32, ao, chunk,
-                                        (int64_t)b->nao * chunk, st);
+            rc = okmi_launch_ao_eval_ex(b, b->d_ishells, b->d_fprims, g, 0, n, &bk, codes, S, OKMI_F32, ao, chunk,
+                                        (int64_t)L.kpad * chunk, st);
         } else {
-            rc = okmi_launch_ao_eval_ex(b, b->d_oshells, b->d_fprims, g, p0 + c0, n, nullptr, codes, S, OKMI_F32, ao,
-                                        chunk, (int64_t)b->nao * chunk, st);
+            rc = okmi_launch_ao_eval_ex(b, b->d_ishells, b->d_fprims, g, p0 + c0, n, nullptr, codes, S, OKMI_F32, ao,
+                                        chunk, (int64_t)L.kpad * chunk, st);
         }
         if (rc) return rc;
+        const int64_t ntile = (n + NP - 1) / NP;
+        LiveArgs la{};
+        la.grid = g; la.bm = bm; la.p0 = p0 + c0; la.n = n; la.np = NP;
+        la.chunks = b->d_chunks; la.shells = b->d_fshells; la.nchunk = b->nchunk;
+        la.list = live; la.count = live_count;
+        hipLaunchKernelGGL(k_big_live, dim3((unsigned)ntile), dim3(64), 0, st, la);
         BigArgs a{};
-        a.ao = ao; a.ld = chunk; a.set_stride = (int64_t)b->nao * chunk; a.nao = b->nao; a.np = n;
+        a.ao = ao; a.ld = chunk; a.set_stride = (int64_t)L.kpad * chunk; a.nao = (int)L.kpad; a.np = n;
         a.C = cpl; a.mpad = L.mpad; a.kpad = L.kpad; a.occ = occp; a.partial = part; a.nmt = nmt; a.nmo = nmo;
-        a.nblocks = (int64_t)nmt * ((n + NP - 1) / NP);
+        a.live = live; a.live_count = live_count; a.nchunk = b->nchunk;
+        a.nblocks = (int64_t)nmt * ntile;
         switch (prec) {
         case OKMI_PREC_F32: rc = launch_big<OKMI_PREC_F32>(a, S, st); break;
         case OKMI_PREC_BF16X3: rc = launch_big<OKMI_PREC_BF16X3>(a, S, st); break;

@@ -533,3 +533,30 @@ def test_electron_count(ok):
   print('electron count', got, exact, abs(got - exact) / exact, flush=True)
   assert abs(got - exact) <= 4e-16 * exact
   assert ok['core'].electron_count(numpy.zeros(0), 1.0) == 0.0
+
+
+def test_fused_live_rows(ok):
+  '''okmi_fused_live_rows (the screened-work measure of bench.py): within
+  [0, padded rows x points], zero on a grid far from every atom, additive
+  over bricks (a grid split at a brick boundary), and on the medium grid
+  between a third and all of the dense count.'''
+  import ctypes
+  from orbkit_amd import _native
+  from orbkit_amd.basis import device_basis
+  qc, axes, info = ok['W'].make('medium', npts_side=24)
+  b = device_basis(qc.geo_spec, qc.ao_spec)
+  L = _native.lib()
+
+  def rows(x, y, z):
+    ax = [numpy.ascontiguousarray(a, dtype=numpy.float64) for a in (x, y, z)]
+    P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    out = ctypes.c_double()
+    _native.check(L.okmi_fused_live_rows(b.handle, P(ax[0]), len(ax[0]), P(ax[1]), len(ax[1]), P(ax[2]), len(ax[2]),
+                                         ctypes.byref(out)), 'okmi_fused_live_rows')
+    return out.value
+  x, y, z = axes
+  npts = len(x) * len(y) * len(z)
+  full = rows(x, y, z)
+  assert 0.33 * b.nao * npts < full <= 416 * npts
+  assert rows(x + 1e4, y, z) == 0.0
+  assert rows(x[:12], y, z) + rows(x[12:], y, z) == full
