@@ -592,6 +592,54 @@ __device__ __forceinline__ Box tile_box(const FusedArgs& a, int64_t tile, double
     return box;
 }
 
+// Chunk liveness of a tile, computed once per tile before the chunk loop:
+// bit c of mask[c / 64] is set when a shell of chunk c reaches the tile's
+// box (the test eval_chunk applies shell by shell, so a chunk without the
+// bit would evaluate nothing and skip its contraction).  The chunk loop
+// visits only these chunks, which saves a dead chunk's shell claims, row
+// clearing and barriers.  Word j is formed by wave j % 4 (lane l tests chunk
+// 64 j + l) and written whole, so no clearing is needed; callers run this
+// after a barrier that follows every read of the previous tile's mask and
+// put a barrier before the first read.  Bases with more chunks than the
+// mask holds have every chunk visited.
+constexpr int LIVE_WORDS = 32;
+
+__device__ __forceinline__ void live_chunks(const FusedArgs& a, const Box& box, int wave, int lane,
+                                            unsigned long long* mask)
+{
+    const int nw = (a.nchunk + 63) / 64;
+    if (nw > LIVE_WORDS) return;
+    for (int j = wave; j < nw; j += 4) {
+        const int c = 64 * j + lane;
+        bool hit = false;
+        if (c < a.nchunk) {
+            const int2 cw = load_int2(a.chunks, c);
+            const int nsh = cw.y & 0xffff;
+            for (int e = 0; e < nsh && !hit; ++e) {
+                const ShellRec sh = load_shell(a.shells, cw.x + e);
+                hit = box.dist2(sh) < sh.rcut2;
+            }
+        }
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
+        if (lane == 0) mask[j] = m;
+    }
+}
+
+// first chunk c' >= c with its bit set (a.nchunk if none), wave-uniform
+__device__ __forceinline__ int next_live(const FusedArgs& a, const unsigned long long* mask, int c)
+{
+    if (a.nchunk > 64 * LIVE_WORDS) return c;
+    while (c < a.nchunk) {
+        const unsigned long long w0 = mask[c >> 6];
+        const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)w0);
+        const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(w0 >> 32));
+        const unsigned long long w = (((unsigned long long)hi << 32) | lo) >> (c & 63);
+        if (w) return c + __builtin_ctzll(w);
+        c = (c | 63) + 1;
+    }
+    return a.nchunk;
+}
+
 // Whether an instance uses one LDS buffer and three workgroups per CU: rho
 // and rho + grad rho with l <= 3 and up to 48 MOs (32 in the batch kernel,
 // which needs a few more registers) fit in 168 VGPRs without spilling (33 KB
@@ -620,7 +668,7 @@ template <int PREC, int PASS, int LM, int MT, bool BATCH> struct FusedPlan {
 // LM: highest angular momentum of the basis (3 or 4).
 template <int PREC, int PASS, int MT, int LM, bool BATCH>
 __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, typename Tile<PREC>::T* lds, bf8* cst,
-                                         int* next, int* live)
+                                         int* next, int* live, unsigned long long* lmask)
 {
     using P = FusedPlan<PREC, PASS, LM, MT, BATCH>;
     using Acc = typename Tile<PREC>::Acc;
@@ -635,6 +683,8 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, typen
     double px, py, pz;
     tile_point(a, tile, lane, pidx, pact, px, py, pz);   // lane l of every wave: point l of the tile
     const Box box = tile_box(a, tile, px, py, pz);
+    live_chunks(a, box, wave, lane, lmask);
+    __syncthreads();
 
     Acc acc[MT][NS];
 #pragma unroll
@@ -648,7 +698,7 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, typen
         // that the evaluation does not count their loads as outstanding
         // (which would make it wait for the C loads issued after them).
         asm volatile("" ::"v"(px), "v"(py), "v"(pz));
-        for (int ch = 0; ch < a.nchunk; ++ch) {
+        for (int ch = next_live(a, lmask, 0); ch < a.nchunk; ch = next_live(a, lmask, ch + 1)) {
             CRegs<PREC, MT> cr;
             COperand<PREC, MT> c;
             if constexpr (CStage<PREC, MT>::on) fetch_c<PREC, MT>(a, ch * KC, cr);
@@ -677,7 +727,7 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, typen
     // load it after the barrier.
     constexpr bool c_early = P::c_early;
     int buf = 0;
-    for (int ch = 0; ch < a.nchunk; ++ch) {
+    for (int ch = next_live(a, lmask, 0); ch < a.nchunk; ch = next_live(a, lmask, ch + 1)) {
         typename Tile<PREC>::T* L = lds + buf * BUF;
         COperand<PREC, MT> c;
         if constexpr (c_early) load_c<PREC, MT>(a, ch * KC, g, n16, c);
@@ -711,7 +761,8 @@ __global__ __launch_bounds__(256, (FusedPlan<PREC, PASS, LM, MT, false>::occupan
     __shared__ bf8 cst[P::single && CStage<PREC, MT>::on ? CStage<PREC, MT>::slots : 1];
     __shared__ int next[2];   // per buffer: next work entry to evaluate
     __shared__ int live[2];   // per buffer: index of the last chunk with a live shell
-    rho_tile<PREC, PASS, MT, LM, false>(a, blockIdx.x, lds, cst, next, live);
+    __shared__ unsigned long long lmask[LIVE_WORDS];   // chunks with a shell in reach of the tile
+    rho_tile<PREC, PASS, MT, LM, false>(a, blockIdx.x, lds, cst, next, live, lmask);
 }
 
 template <int PREC, int PASS, int LM>

@@ -399,10 +399,11 @@ __global__ __launch_bounds__(256, (FusedPlan<PREC, PASS, LM, MT, true>::occupanc
     __shared__ bf8 cst[P::single && CStage<PREC, MT>::on ? CStage<PREC, MT>::slots : 1];
     __shared__ int next[2];
     __shared__ int live[2];
+    __shared__ unsigned long long lmask[LIVE_WORDS];
     for (int64_t t = blockIdx.x; t < b.ntile; t += gridDim.x) {
         const int m = find_item(b.tile_start, b.nmol, t);
         const FusedArgs a = load_fused_args(b.mols, m);
-        rho_tile<PREC, PASS, MT, LM, true>(a, t - cptr(b.tile_start)[m], lds, cst, next, live);
+        rho_tile<PREC, PASS, MT, LM, true>(a, t - cptr(b.tile_start)[m], lds, cst, next, live, lmask);
     }
 }
 

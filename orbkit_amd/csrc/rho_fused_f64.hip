@@ -195,6 +195,7 @@ __global__ __launch_bounds__(256, 2) void k_rho_fused_f64(FusedArgs a)
     __shared__ double cs[KC * MP];   // the chunk's C columns, [k][m]
     __shared__ int next[1];          // next work entry of the chunk being evaluated
     __shared__ int live[1];          // last chunk with a live shell
+    __shared__ unsigned long long lmask[LIVE_WORDS];   // chunks with a shell in reach of the tile
 
     const int64_t tile = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -209,6 +210,8 @@ __global__ __launch_bounds__(256, 2) void k_rho_fused_f64(FusedArgs a)
     double px, py, pz;
     tile_point(a, tile, lane, pidx, pact, px, py, pz);   // lane l of every wave: point l of the tile
     const Box box = tile_box(a, tile, px, py, pz);
+    live_chunks(a, box, wave, lane, lmask);
+    __syncthreads();
     // the coordinates are consumed here so that the evaluation does not count
     // their loads as outstanding (it would then also wait for the C loads)
     asm volatile("" ::"v"(px), "v"(py), "v"(pz));
@@ -226,7 +229,7 @@ __global__ __launch_bounds__(256, 2) void k_rho_fused_f64(FusedArgs a)
 
     typedef const __attribute__((address_space(1))) double gdouble;
     gdouble* C = (gdouble*)a.C;
-    for (int ch = 0; ch < a.nchunk; ++ch) {
+    for (int ch = next_live(a, lmask, 0); ch < a.nchunk; ch = next_live(a, lmask, ch + 1)) {
         const int k0 = ch * KC;
         // this thread's C elements e = threadIdx.x + 256 j of [m][k] (k
         // fastest: coalesced rows); clamped to the last element
