@@ -248,8 +248,8 @@ def _errs(rho, grad, r, d):
           float(numpy.max(numpy.abs(rho - r)[mask] / r[mask])))
 
 
-# bars just above the measured worst molecule (bf16x6: rho 2.3e-7, grad
-# 2.9e-7, pointwise 6.7e-7; f32: 1.8e-7, 3.0e-7, 9.1e-7): FP32 rounding of
+# bars just above the measured worst molecule (bf16x6: rho 1.9e-7, grad
+# 3.0e-7, pointwise 6.5e-7; f32: 2.4e-7, 3.3e-7, 9.3e-7): FP32 rounding of
 # the AO values and coefficients and FP32 accumulation over 50-600 AOs, with
 # fewer MOs (8-30) to average over than the 40 of the medium workload
 @pytest.mark.parametrize('precision,bar,bar_max', [('bf16x6', 4e-7, 1.5e-6), ('f32', 5e-7, 1.5e-6)])

@@ -379,13 +379,13 @@ def test_fused_f64_vs_oracle(ok, name, nmo, side):
 
 # The precision target: <= 1e-7 relative density error against FP64 (norm-wise)
 # for the FP32-operand paths at the medium workload's full size.  Measured:
-# bf16x6 rho 7.5e-8, grad 5.9e-8, Laplacian 5.5e-8; f32 rho 6.5e-8, grad
-# 4.2e-8, Laplacian 5.8e-8; the largest pointwise rho error (rho > 1e-10 max
-# rho) 4.0e-7 / 5.5e-7, which FP32 operands cannot bring below 1e-7
+# bf16x6 rho 3.2e-8, grad 3.6e-8, Laplacian 2.2e-8; f32 rho 4.0e-8, grad
+# 3.0e-8, Laplacian 3.6e-8; the largest pointwise rho error (rho > 1e-10 max
+# rho) 4.6e-7 / 5.2e-7, which FP32 operands cannot bring below 1e-7
 # (test_fp32_operand_floor in test_precision_floor.py).  The errors depend on
-# the order of the FP32 accumulation over the AO rows (with the rows in basis
-# order bf16x6 measured rho 3.2e-8, grad 3.1e-8).  bf16x3 (two bf16 parts,
-# 16 significant bits): rho 1.9e-6, pointwise 2.0e-5.
+# the order of the FP32 accumulation over the AO rows (with the rows ordered
+# by screening radius alone bf16x6 measured rho 7.5e-8, grad 5.9e-8).
+# bf16x3 (two bf16 parts, 16 significant bits): rho 1.9e-6, pointwise 2.0e-5.
 @pytest.mark.parametrize('prec,l2_bar,max_bar', [('f32', 1e-7, 1e-6), ('bf16x6', 1e-7, 1e-6), ('bf16x3', 5e-6, 5e-5)])
 def test_fused_precision(ok, prec, l2_bar, max_bar):
   """Reduced-precision fused path against the FP64 path (which equals the
@@ -413,9 +413,9 @@ def test_fused_precision(ok, prec, l2_bar, max_bar):
   assert numpy.linalg.norm(fd2 - d2) / numpy.linalg.norm(d2) <= l2_bar
 
 
-# bars just above the measured errors (bf16x6 rho / grad: 5 MOs 9.1e-8 /
-# 1.5e-7, 20 MOs 3.5e-8 / 7.3e-8, 40 MOs 3.2e-8 / 5.0e-8; bf16x3 up to
-# 4.3e-6 / 1.1e-5): fewer MOs average the rounding over fewer terms
+# bars just above the measured errors (bf16x6 rho / grad: 5 MOs 1.0e-7 /
+# 8.8e-8, 20 MOs 2.9e-8 / 4.7e-8, 40 MOs 2.7e-8 / 4.4e-8; bf16x3 up to
+# 4.4e-6 / 1.1e-5): fewer MOs average the rounding over fewer terms
 @pytest.mark.parametrize('nmo', [5, 20, 40])
 @pytest.mark.parametrize('prec', ['bf16x6', 'bf16x3'])
 def test_fused_mo_tiles(ok, prec, nmo):
@@ -435,8 +435,8 @@ def test_fused_mo_tiles(ok, prec, nmo):
   assert e_rho <= tol and e_g <= tol, (e_rho, e_g)
 
 
-# measured (largest of the seven errors): bf16x6 1.6e-7, f32 1.7e-7 (both the
-# vector grid's xx row), bf16x3 5.0e-6
+# measured (largest of the seven errors): bf16x6 6.5e-8 (the Laplacian), f32
+# 1.4e-7 (the vector grid's xx row), bf16x3 5.0e-6
 @pytest.mark.parametrize('prec,tol', [('f32', 2.5e-7), ('bf16x6', 2.5e-7), ('bf16x3', 2e-5)])
 def test_many_mo_precision(ok, prec, tol):
   '''More MOs than the fused kernel holds (130: two 128-MO tiles, the second
