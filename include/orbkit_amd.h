@@ -95,7 +95,10 @@ int okmi_mocreator(const double* d_ao, int nao, int64_t npts, int64_t ld_ao,
 
 /*
  * Workspace bytes okmi_rho / okmi_mo_eval need to process `chunk` points per
- * pass with `nsets` AO derivative sets (FP64 AO staging).
+ * pass with `nsets` AO derivative sets (FP64 AO staging, with the density's
+ * live-chunk lists).  The staged density (okmi_rho_staged, okmi_rho beyond
+ * 64 MOs) also keeps a copy of C in its row order: nmo x (nao rounded up to
+ * whole 32-row chunks) doubles more.
  */
 size_t okmi_workspace_bytes(const okmi_basis* b, int nsets, int64_t chunk);
 
@@ -150,9 +153,13 @@ int okmi_rho_noalloc(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_
 /*
  * The same as okmi_rho in orbkit's own two stages, any number of MOs: the
  * FP64 AO sets of a chunk of points written to the workspace (okmi_ao_eval's
- * kernel, the workspace sized by okmi_workspace_bytes), then the FP64 MFMA
- * contraction with the density assembled in its epilogue.  Replaces the same
- * code as okmi_rho (orbkit/core.py:179-308, orbkit/cy_core.pyx:49-101).
+ * kernel, the workspace sized by okmi_workspace_bytes plus the copy of C),
+ * then the FP64 MFMA contraction with the density assembled in its epilogue.
+ * The AO rows are staged in the fused kernels' row order (shells in 32-row
+ * chunks) and each 64-point tile (a 4x4x4 brick of a whole regular grid,
+ * else 64 consecutive points) contracts only the chunks with a shell whose
+ * screening radius reaches it.  Replaces the same code as okmi_rho
+ * (orbkit/core.py:179-308, orbkit/cy_core.pyx:49-101).
  */
 int okmi_rho_staged(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np,
                     const double* d_c, const double* d_occ, int nmo,

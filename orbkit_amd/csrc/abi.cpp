@@ -399,22 +399,32 @@ extern "C" int okmi_mocreator(const double* d_ao, int nao, int64_t npts, int64_t
     return okmi_launch_gemm_f64(d_ao, ld_ao, 0, 1, nao, d_c, nmo, npts, d_mo, ld_mo, (hipStream_t)stream);
 }
 
-extern "C" size_t okmi_workspace_bytes(const okmi_basis* b, int nsets, int64_t chunk)
-{
-    if (!b || nsets <= 0 || chunk <= 0) return 0;
-    return (size_t)nsets * (size_t)b->nao * (size_t)chunk * sizeof(double);
-}
-
 namespace {
+// Workspace of the staged paths per point: the AO sets (FP64; in the fused
+// row order, nkpad >= nao rows, for the screened density) and the
+// live-chunk lists of 64-point tiles; plus alignment slack
+size_t staged_per_point(const okmi_basis* b, int nsets)
+{
+    const size_t rows = std::max(b->nao, b->nkpad);
+    return (size_t)nsets * rows * sizeof(double) + ((size_t)b->nchunk + 1) * sizeof(int) / 64 + 1;
+}
+constexpr size_t STAGED_SLACK = 1024;
+
 int64_t chunk_points(const okmi_basis* b, int nsets, int64_t np, size_t ws_bytes)
 {
-    const size_t per = (size_t)nsets * (size_t)b->nao * sizeof(double);
-    int64_t c = (int64_t)(ws_bytes / per);
+    if (ws_bytes <= STAGED_SLACK) return 0;
+    int64_t c = (int64_t)((ws_bytes - STAGED_SLACK) / staged_per_point(b, nsets));
     if (c >= np) return np;
     c &= ~(int64_t)63;
     return c;
 }
 }  // namespace
+
+extern "C" size_t okmi_workspace_bytes(const okmi_basis* b, int nsets, int64_t chunk)
+{
+    if (!b || nsets <= 0 || chunk <= 0) return 0;
+    return staged_per_point(b, nsets) * (size_t)chunk + STAGED_SLACK;
+}
 
 extern "C" int okmi_mo_eval(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np, const int* drv,
                             int ndrv, const double* d_c, int nmo, double* d_mo, int64_t ld_mo, void* d_ws,
@@ -501,16 +511,74 @@ extern "C" int okmi_rho_staged(const okmi_basis* b, const okmi_grid* g, int64_t 
     }
     hipStream_t st = (hipStream_t)stream;
     const GridDesc gd = to_desc(g);
-    for (int64_t c0 = 0; c0 < np; c0 += chunk) {
-        const int64_t n = np - c0 < chunk ? np - c0 : chunk;
-        int rc = okmi_launch_ao_eval(b, gd, p0 + c0, n, codes, nsets, OKMI_F64, d_ws, chunk, (int64_t)b->nao * chunk,
+    if (!(b->nkpad > 0 && b->d_ishells)) {   // no fused tables: every AO row of every point
+        for (int64_t c0 = 0; c0 < np; c0 += chunk) {
+            const int64_t n = np - c0 < chunk ? np - c0 : chunk;
+            int rc = okmi_launch_ao_eval(b, gd, p0 + c0, n, codes, nsets, OKMI_F64, d_ws, chunk,
+                                         (int64_t)b->nao * chunk, st);
+            if (rc) return rc;
+            rc = okmi_launch_rho_f64((const double*)d_ws, chunk, set_off, set_mask, b->nao, d_c, nmo, d_occ, n,
+                                     drv_out, nout, d_rho + c0, d_delta ? d_delta + c0 : nullptr, ld_delta, d_mo_norm,
                                      st);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    // Screened: the AO sets in the fused row order (shells packed into
+    // 32-row chunks by radius class and Morton code), per 64-point tile the
+    // chunks with a shell in reach, and the GEMM contracts only those.  A
+    // whole regular grid runs in 4x4x4 bricks (compact tiles), other grids
+    // in point order.
+    // C in the fused row order at the front of the workspace, then the AO
+    // sets and the live-chunk lists
+    const int64_t rows = b->nkpad;
+    const bool brick = g->regular && p0 == 0 && np == g->nx * g->ny * g->nz;
+    const int64_t nbx = (g->nx + 3) / 4, nby = (g->ny + 3) / 4, nbz = (g->nz + 3) / 4;
+    const int64_t total = brick ? nbx * nby * nbz * 64 : np;
+    const size_t cbytes = (((size_t)nmo * rows * sizeof(double)) + 255) & ~(size_t)255;
+    const int64_t ch = ws_bytes > cbytes + 256 ? chunk_points(b, nsets, total, ws_bytes - cbytes - 256) : 0;
+    if (ch <= 0 || (brick && ch % 64)) {
+        okmi_set_error("rho: workspace of %zu bytes holds no chunk (%zu bytes for C and 64 points)", ws_bytes,
+                       cbytes + 256 + okmi_workspace_bytes(b, nsets, 64));
+        return 1;
+    }
+    double* Cp = (double*)(((uintptr_t)d_ws + 255) & ~(uintptr_t)255);
+    double* A = Cp + cbytes / sizeof(double);
+    int* live = (int*)(A + (size_t)nsets * rows * ch);
+    const int64_t ntile = (ch + 63) / 64;
+    int* live_count = live + ntile * b->nchunk;
+    if ((size_t)((char*)(live_count + ntile) - (char*)d_ws) > ws_bytes) {
+        okmi_set_error("rho: workspace of %zu bytes too small for the screened layout", ws_bytes);
+        return 1;
+    }
+    for (int s = 0; s < nsets; ++s) set_off[codes[s]] = (int64_t)s * rows * ch;
+    int rc = okmi_launch_permute_c(d_c, nmo, b->nao, rows, b->d_colmap, Cp, st);
+    if (!rc) rc = okmi_launch_zero_rows(A, nsets, rows * ch, ch, b->d_padrows, b->npadrows, ch, st);
+    if (rc) return rc;
+    F64Screen scr{};
+    scr.colmap = nullptr; scr.live = live; scr.live_count = live_count; scr.nchunk = b->nchunk;
+    scr.ldc = (int)rows; scr.brick = brick ? 1 : 0;
+    scr.nbx = nbx; scr.nby = nby; scr.nbz = nbz; scr.nx = g->nx; scr.ny = g->ny; scr.nz = g->nz;
+    for (int64_t c0 = 0; c0 < total; c0 += ch) {
+        const int64_t n = total - c0 < ch ? total - c0 : ch;
+        if (brick) {
+            AoBricks bk{c0 / 64, n / 64, nbx, nby, nbz};
+            rc = okmi_launch_ao_eval_ex(b, b->d_ishells, b->d_fprims, gd, 0, n, &bk, codes, nsets, OKMI_F64, A, ch,
+                                        rows * ch, st);
+        } else {
+            rc = okmi_launch_ao_eval_ex(b, b->d_ishells, b->d_fprims, gd, p0 + c0, n, nullptr, codes, nsets, OKMI_F64,
+                                        A, ch, rows * ch, st);
+        }
         if (rc) return rc;
-        rc = okmi_launch_rho_f64((const double*)d_ws, chunk, set_off, set_mask, b->nao, d_c, nmo, d_occ, n, drv_out,
-                                 nout, d_rho + c0, d_delta ? d_delta + c0 : nullptr, ld_delta, d_mo_norm, st);
+        rc = okmi_launch_live_chunks(b, gd, brick ? 1 : 0, c0 / 64, p0 + c0, n, 64, live, live_count, st);
+        if (rc) return rc;
+        scr.bk0 = c0 / 64;
+        rc = okmi_launch_rho_f64(A, ch, set_off, set_mask, (int)rows, Cp, nmo, d_occ, n, drv_out, nout,
+                                 brick ? d_rho : d_rho + c0, d_delta ? (brick ? d_delta : d_delta + c0) : nullptr,
+                                 ld_delta, d_mo_norm, st, &scr);
         if (rc) return rc;
     }
-    return 0;
+    return okmi_check_launch("rho");
 }
 
 extern "C" int okmi_rho(const okmi_basis* b, const okmi_grid* g, int64_t p0, int64_t np, const double* d_c,
@@ -596,7 +664,7 @@ extern "C" int okmi_rho_noalloc(const okmi_basis* b, const okmi_grid* g, int64_t
     // path okmi_rho takes
     const bool fused = nmo <= 64 && b->max_nrow <= 32;
     const size_t need = fused ? okmi_fused_ws_bytes(const_cast<okmi_basis*>(b), nmo, OKMI_PREC_F64, 64)
-                              : okmi_workspace_bytes(b, nsets, 64);
+                              : okmi_workspace_bytes(b, nsets, 64) + (size_t)nmo * std::max(b->nao, b->nkpad) * 8 + 512;
     if (need > OKMI_NOALLOC_SCRATCH_BYTES) {
         okmi_set_error("rho_noalloc: %d MOs x %d AOs need %zu bytes of scratch for one 64-point pass "
                        "(the scratch holds %zu); use okmi_rho with a workspace",

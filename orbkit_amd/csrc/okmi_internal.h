@@ -72,12 +72,41 @@ int okmi_prepare_fused(OkmiBasis* b);
 int okmi_launch_gemm_f64(const double* A, int64_t ld, int64_t set_stride, int nsets, int nao, const double* C,
                          int nmo, int64_t np, double* mo_out, int64_t ld_mo, hipStream_t st);
 
+// Screening of the staged FP64 density: AO rows in the fused row order
+// (C read through colmap), per 64-column tile the chunks with a shell in
+// reach, and optionally brick-ordered columns (column c is lane c % 64 of
+// brick bk0 + c / 64 of an nbx x nby x nbz brick grid over nx x ny x nz).
+struct F64Screen {
+    const int* colmap;       // C column of each A row (-1: zero); null: C is already in the A row order
+    const int* live;         // [tile][nchunk]
+    const int* live_count;   // [tile]
+    int nchunk;
+    int ldc;                 // row stride of C (the basis's output rows)
+    int brick;
+    int64_t bk0, nbx, nby, nbz, nx, ny, nz;
+};
+
 // Density and derivative outputs from AO sets indexed by derivative code:
 // set_off[c] is the offset of code c's AO set in A (codes in set_mask),
-// out_code[r] the code of output row r.
+// out_code[r] the code of output row r.  With `scr` the AO rows (nao of
+// them) are in the fused row order and each 64-column tile contracts only
+// its live chunks.
 int okmi_launch_rho_f64(const double* A, int64_t ld, const int64_t* set_off, int set_mask, int nao,
                         const double* C, int nmo, const double* occ, int64_t np, const int* out_code, int nout,
-                        double* rho, double* delta, int64_t ld_delta, double* mo_norm, hipStream_t st);
+                        double* rho, double* delta, int64_t ld_delta, double* mo_norm, hipStream_t st,
+                        const F64Screen* scr = nullptr);
+
+// per tile of `np` columns (brick lanes from brick bk0 when `brick`, else
+// points p0 + column) the 32-row chunks of the fused layout with a shell in
+// reach, in order: list[tile][..], count[tile]
+int okmi_launch_live_chunks(const OkmiBasis* b, const GridDesc& g, int brick, int64_t bk0, int64_t p0, int64_t n,
+                            int np, int* list, int* count, hipStream_t st);
+// C [nmo][nao] -> [nmo][kpad] in the fused row order (colmap[k]: C column of row k, -1: zero)
+int okmi_launch_permute_c(const double* C, int nmo, int nao, int64_t kpad, const int* colmap, double* out,
+                          hipStream_t st);
+// zero rows rows[0 .. nrows) of S sets [S][set_stride] (row stride ld) over columns [0, ncol)
+int okmi_launch_zero_rows(double* a, int S, int64_t set_stride, int64_t ld, const int* rows, int nrows, int64_t ncol,
+                          hipStream_t st);
 
 // codes whose AO sets a density request needs: 0, the requested codes and
 // the first derivatives entering second-derivative product terms

@@ -447,14 +447,14 @@ __global__ __launch_bounds__(64) void k_big_live(LiveArgs a)
 }
 
 // zero rows `rows[0 .. nrows)` of every AO set over columns [0, ncol)
-__global__ void k_zero_rows(float* ao, int S, int64_t set_stride, int64_t ld, const int* rows, int nrows,
-                            int64_t ncol)
+template <typename T>
+__global__ void k_zero_rows(T* ao, int S, int64_t set_stride, int64_t ld, const int* rows, int nrows, int64_t ncol)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t per = (int64_t)nrows * ncol;
     if (i >= per * S) return;
     const int64_t s = i / per, r = (i % per) / ncol, c = i % ncol;
-    ao[s * set_stride + (int64_t)rows[r] * ld + c] = 0.f;
+    ao[s * set_stride + (int64_t)rows[r] * ld + c] = T(0);
 }
 
 template <int PREC, int S>
@@ -514,6 +514,35 @@ BigLayout big_layout(const OkmiBasis* b, int nmo, int prec, int S)
 }
 
 }  // namespace
+
+int okmi_launch_live_chunks(const OkmiBasis* b, const GridDesc& g, int brick, int64_t bk0, int64_t p0, int64_t n,
+                            int np, int* list, int* count, hipStream_t st)
+{
+    if (n <= 0) return 0;
+    LiveArgs la{};
+    la.grid = g;
+    la.bm.on = brick;
+    if (brick) {
+        la.bm.nbx = (g.nx + 3) / 4; la.bm.nby = (g.ny + 3) / 4; la.bm.nbz = (g.nz + 3) / 4;
+        la.bm.nx = g.nx; la.bm.ny = g.ny; la.bm.nz = g.nz;
+        la.bm.bk0 = bk0;
+    }
+    la.p0 = p0; la.n = n; la.np = np;
+    la.chunks = b->d_chunks; la.shells = b->d_fshells; la.nchunk = b->nchunk;
+    la.list = list; la.count = count;
+    hipLaunchKernelGGL(k_big_live, dim3((unsigned)((n + np - 1) / np)), dim3(64), 0, st, la);
+    return 0;
+}
+
+int okmi_launch_zero_rows(double* a, int S, int64_t set_stride, int64_t ld, const int* rows, int nrows, int64_t ncol,
+                          hipStream_t st)
+{
+    const int64_t nz = (int64_t)S * nrows * ncol;
+    if (nz <= 0) return 0;
+    hipLaunchKernelGGL(k_zero_rows<double>, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, a, S, set_stride,
+                       ld, rows, nrows, ncol);
+    return 0;
+}
 
 size_t okmi_big_ws_bytes(const OkmiBasis* b, int nmo, int prec, int S, int64_t np)
 {
@@ -583,7 +612,7 @@ int okmi_launch_rho_big(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np,
                        prec, L.mpad, L.kpad, (const int*)b->d_colmap, cpl);
     if (b->npadrows > 0) {
         const int64_t nz = (int64_t)S * b->npadrows * chunk;
-        hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, ao, S,
+        hipLaunchKernelGGL(k_zero_rows<float>, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, ao, S,
                            (int64_t)L.kpad * chunk, chunk, (const int*)b->d_padrows, b->npadrows, chunk);
     }
     hipLaunchKernelGGL(k_pad_occ_big, dim3((unsigned)((L.mpad + 255) / 256)), dim3(256), 0, st, d_occ, nmo,

@@ -36,6 +36,12 @@ struct GemmArgs {
     int64_t set_off[OKMI_MAX_SETS];   // offset of code's AO set in A
     int nout;
     int out_code[16];        // derivative code of each output row
+    // density mode, screened (okmi_launch_rho_f64 with F64Screen): A rows in
+    // the fused row order, C columns through scr.colmap, each 64-column
+    // block contracts its live chunks only, brick-ordered columns mapped to
+    // grid points for the outputs
+    int screened;
+    F64Screen scr;
 };
 
 template <int MG, int NS>
@@ -164,7 +170,27 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n = lane & 15, kq = lane >> 4;
     const int64_t p = (int64_t)blockIdx.x * 64 + wave * 16 + n;
-    const bool pin = p < a.np;
+    bool pin = p < a.np;
+    int64_t q = p;   // output index of this lane's point
+    if (a.screened && a.scr.brick) {
+        const int l = (int)(p & 63);
+        int64_t bk = a.scr.bk0 + (p >> 6);
+        const int64_t bz = bk % a.scr.nbz;
+        bk /= a.scr.nbz;
+        const int64_t by = bk % a.scr.nby, bx = bk / a.scr.nby;
+        const int64_t ix = 4 * bx + (l >> 4), iy = 4 * by + ((l >> 2) & 3), iz = 4 * bz + (l & 3);
+        pin = pin && ix < a.scr.nx && iy < a.scr.ny && iz < a.scr.nz;   // brick lanes outside the grid add nothing
+        q = (ix * a.scr.ny + iy) * a.scr.nz + iz;
+    }
+    // K steps of 4 rows: every row, or the 8 steps of each live chunk of
+    // this block's 64 columns
+    const int* lst = nullptr;
+    int nsteps = (a.nao + 3) / 4;
+    if (a.screened) {
+        lst = a.scr.live + (int64_t)blockIdx.x * a.scr.nchunk;
+        nsteps = 8 * a.scr.live_count[blockIdx.x];
+    }
+    const int ldc = a.screened ? a.scr.ldc : a.nao;
     const int nmt = (a.nmo + 15) >> 4;
     const int ngroups = (nmt + MG - 1) / MG;
 
@@ -185,12 +211,20 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
         for (int mt = 0; mt < MG; ++mt) {
             const int m = (g * MG + mt) * 16 + n;
             min_[mt] = m < a.nmo;
-            cbase[mt] = a.C + (int64_t)(min_[mt] ? m : 0) * a.nao;
+            cbase[mt] = a.C + (int64_t)(min_[mt] ? m : 0) * ldc;
         }
-        const double* abase = a.A + (pin ? p : 0);
-        for (int k0 = 0; k0 < a.nao; k0 += 4) {
+        const double* abase = a.A + (p < a.np ? p : 0);
+        for (int ks = 0; ks < nsteps; ++ks) {
+            const int k0 = a.screened ? lst[ks >> 3] * 32 + 4 * (ks & 7) : 4 * ks;
             const int k = k0 + kq;
             const bool kin = k < a.nao;
+            // C column of A row k (fused row order: through colmap, -1 = zero row)
+            int kcol = k;
+            bool cin = kin;
+            if (a.screened && a.scr.colmap) {
+                kcol = a.scr.colmap[kin ? k : 0];
+                cin = kin && kcol >= 0;
+            }
             double av[MG];
             if constexpr (FULL) {
                 // Rows past the end read row nao - 1 and are zeroed through
@@ -201,8 +235,9 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
                 // masked load in its own branch with its own wait, which
                 // serialised the K step's loads.
                 const int kc = kin ? k : a.nao - 1;
+                const int cc = cin ? kcol : 0;
 #pragma unroll
-                for (int mt = 0; mt < MG; ++mt) av[mt] = cbase[mt][kc] * ((kin && min_[mt]) ? 1.0 : 0.0);
+                for (int mt = 0; mt < MG; ++mt) av[mt] = cbase[mt][cc] * ((cin && min_[mt]) ? 1.0 : 0.0);
 #pragma unroll
                 for (int d = 0; d < NSC; ++d) {
                     const double bv = abase[a.set_off[d] + (int64_t)kc * a.ld] * (pin ? 1.0 : 0.0);
@@ -212,7 +247,7 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
                 }
             } else {
 #pragma unroll
-                for (int mt = 0; mt < MG; ++mt) av[mt] = (kin && min_[mt]) ? cbase[mt][k] : 0.0;
+                for (int mt = 0; mt < MG; ++mt) av[mt] = (cin && min_[mt]) ? cbase[mt][kcol] : 0.0;
 #pragma unroll
                 for (int d = 0; d < NSC; ++d) {
                     if (a.set_mask & (1 << d)) {
@@ -255,14 +290,14 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
         }
     }
     if (kq == 0 && pin) {
-        a.rho[p] = rho_acc;
+        a.rho[q] = rho_acc;
         for (int r = 0; r < a.nout; ++r) {
             const int c = a.out_code[r];
             double v = 0.0;
 #pragma unroll
             for (int d = 0; d < NSC; ++d)
                 if (d == c) v = d == 0 ? 2.0 * rho_acc : dsum[d];
-            a.delta[(int64_t)r * a.ld_delta + p] = v;
+            a.delta[(int64_t)r * a.ld_delta + q] = v;
         }
     }
 }
