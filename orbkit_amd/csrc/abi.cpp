@@ -354,6 +354,7 @@ extern "C" int okmi_basis_destroy(okmi_basis* b)
     if (b->d_fgroups) (void)hipFree(b->d_fgroups);
     if (b->d_oshells) (void)hipFree(b->d_oshells);
     if (b->d_ishells) (void)hipFree(b->d_ishells);
+    if (b->d_wshells) (void)hipFree(b->d_wshells);
     if (b->d_padrows) (void)hipFree(b->d_padrows);
     if (b->d_colmap) (void)hipFree(b->d_colmap);
     delete[] b->h_shells;
@@ -406,7 +407,8 @@ namespace {
 size_t staged_per_point(const okmi_basis* b, int nsets)
 {
     const size_t rows = std::max(b->nao, b->nkpad);
-    return (size_t)nsets * rows * sizeof(double) + ((size_t)b->nchunk + 1) * sizeof(int) / 64 + 1;
+    return (size_t)nsets * rows * sizeof(double) + ((size_t)b->nchunk + 1) * sizeof(int) / 64 +
+           (size_t)live_mask_words(b) * 8 / 64 + 2;
 }
 constexpr size_t STAGED_SLACK = 1024;
 
@@ -547,7 +549,10 @@ extern "C" int okmi_rho_staged(const okmi_basis* b, const okmi_grid* g, int64_t 
     int* live = (int*)(A + (size_t)nsets * rows * ch);
     const int64_t ntile = (ch + 63) / 64;
     int* live_count = live + ntile * b->nchunk;
-    if ((size_t)((char*)(live_count + ntile) - (char*)d_ws) > ws_bytes) {
+    const int nw = live_mask_words(b);
+    unsigned long long* lmask =
+        (unsigned long long*)(((uintptr_t)(live_count + ntile) + 7) & ~(uintptr_t)7);
+    if ((size_t)((char*)(lmask + ntile * nw) - (char*)d_ws) > ws_bytes) {
         okmi_set_error("rho: workspace of %zu bytes too small for the screened layout", ws_bytes);
         return 1;
     }
@@ -559,18 +564,20 @@ extern "C" int okmi_rho_staged(const okmi_basis* b, const okmi_grid* g, int64_t 
     scr.colmap = nullptr; scr.live = live; scr.live_count = live_count; scr.nchunk = b->nchunk;
     scr.ldc = (int)rows; scr.brick = brick ? 1 : 0;
     scr.nbx = nbx; scr.nby = nby; scr.nbz = nbz; scr.nx = g->nx; scr.ny = g->ny; scr.nz = g->nz;
+    const AoLiveMask am{lmask, nw, 0};
     for (int64_t c0 = 0; c0 < total; c0 += ch) {
         const int64_t n = total - c0 < ch ? total - c0 : ch;
+        // live chunks first: the AO kernel leaves the rows of dead chunks unwritten
+        rc = okmi_launch_live_chunks(b, gd, brick ? 1 : 0, c0 / 64, p0 + c0, n, 64, live, live_count, st, lmask);
+        if (rc) return rc;
         if (brick) {
             AoBricks bk{c0 / 64, n / 64, nbx, nby, nbz};
             rc = okmi_launch_ao_eval_ex(b, b->d_ishells, b->d_fprims, gd, 0, n, &bk, codes, nsets, OKMI_F64, A, ch,
-                                        rows * ch, st);
+                                        rows * ch, st, &am);
         } else {
             rc = okmi_launch_ao_eval_ex(b, b->d_ishells, b->d_fprims, gd, p0 + c0, n, nullptr, codes, nsets, OKMI_F64,
-                                        A, ch, rows * ch, st);
+                                        A, ch, rows * ch, st, &am);
         }
-        if (rc) return rc;
-        rc = okmi_launch_live_chunks(b, gd, brick ? 1 : 0, c0 / 64, p0 + c0, n, 64, live, live_count, st);
         if (rc) return rc;
         scr.bk0 = c0 / 64;
         rc = okmi_launch_rho_f64(A, ch, set_off, set_mask, (int)rows, Cp, nmo, d_occ, n, drv_out, nout,

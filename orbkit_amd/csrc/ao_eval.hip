@@ -27,6 +27,13 @@ struct AoArgs {
     // nearest grid point, so every column of the nbk bricks is written
     int brick;
     int64_t bk0, nbk, nbx, nby, nbz;
+    // optional live-chunk mask (AoLiveMask): the wave walks only the shells
+    // of its tile's live chunks (chunk table and work-ordered shells of the
+    // fused layout); null: every shell of `shells`
+    const unsigned long long* live_mask;
+    int mask_words, mask_shift;
+    const int2* chunks;        // per chunk: first work entry, shells | used rows << 16
+    const ShellRec* wshells;   // work order, internal rows, fused radii, PrimRec primitives
 };
 
 // squared distance from a shell centre to the wave's points' bounding box
@@ -177,18 +184,18 @@ void k_ao_eval(AoArgs a, OutT* __restrict__ out)
             lo[k] = fmin(lo[k], __shfl_xor(lo[k], o));
             hi[k] = fmax(hi[k], __shfl_xor(hi[k], o));
         }
-    for (int s = 0; s < a.nshell; ++s) {
-        const ShellRec sh = load_shell(a.shells, s);
+    // one shell for the wave's points: its rows evaluated, or zero when it is
+    // out of reach (every value and derivative below the screening bound)
+    auto shell = [&](const ShellRec& sh) {
         const double d2 = __longlong_as_double(__builtin_amdgcn_readfirstlane(__double_as_longlong(box_dist2(lo, hi, sh))));
         if (d2 >= sh.rcut2) {
-            // every value and derivative of the shell is below 1e-40 here
             if (act)
                 for (int c = 0; c < OKMI_MAX_SETS; ++c) {
                     if (a.dst[c] < 0) continue;
                     OutT* o = out + a.dst[c] * a.set_stride + (int64_t)sh.row0 * a.ld + i;
                     for (int r = 0; r < sh.nrow; ++r) __builtin_nontemporal_store((OutT)0, &o[(int64_t)r * a.ld]);
                 }
-            continue;
+            return;
         }
         switch (sh.l) {
         case 0: shell_out<0, GRP, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
@@ -196,6 +203,31 @@ void k_ao_eval(AoArgs a, OutT* __restrict__ out)
         case 2: shell_out<2, GRP, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
         case 3: shell_out<3, GRP, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
         default: shell_out<4, GRP, OutT>(a, sh, d2, px, py, pz, i, act, out); break;
+        }
+    };
+    if (!a.live_mask) {
+        for (int s = 0; s < a.nshell; ++s) shell(load_shell(a.shells, s));
+        return;
+    }
+    // the shells of the live chunks of this wave's tile; the rows of the
+    // other chunks are never read, and are left unwritten.  Masks exist for
+    // the tiles holding the launch's points only: a wave past the last point
+    // (the grid's rounding) has nothing to write and stops here.
+    const int64_t wv = a.brick ? (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)
+                               : ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >> 6;
+    if (a.brick ? wv >= a.nbk : wv * 64 >= a.np) return;
+    const int64_t wt = wv >> a.mask_shift;
+    const unsigned long long* mrow = a.live_mask + wt * a.mask_words;
+    for (int w = 0; w < a.mask_words; ++w) {
+        const unsigned long long m0 = mrow[w];
+        unsigned long long m = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(m0 >> 32)) << 32) |
+                               (unsigned)__builtin_amdgcn_readfirstlane((unsigned)m0);
+        while (m) {
+            const int c = 64 * w + __builtin_ctzll(m);
+            m &= m - 1;
+            const int2 cw = load_int2(a.chunks, c);
+            const int nsh = cw.y & 0xffff;
+            for (int e = 0; e < nsh; ++e) shell(load_shell(a.wshells, cw.x + e));
         }
     }
 }
@@ -211,10 +243,17 @@ int okmi_launch_ao_eval(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64
 
 int okmi_launch_ao_eval_ex(const OkmiBasis* b, const ShellRec* shells, const PrimRec* prims, const GridDesc& g,
                            int64_t p0, int64_t np, const AoBricks* bricks, const int* drv, int ndrv, int out_dtype,
-                           void* out, int64_t ld, int64_t set_stride, hipStream_t st)
+                           void* out, int64_t ld, int64_t set_stride, hipStream_t st, const AoLiveMask* live)
 {
     if (np <= 0 && !bricks) return 0;
     AoArgs a{};
+    if (live) {
+        a.live_mask = live->mask;
+        a.mask_words = live->words;
+        a.mask_shift = live->shift;
+        a.chunks = b->d_chunks;
+        a.wshells = b->d_wshells;
+    }
     a.shells = shells;
     a.prims = prims;
     a.T = b->d_T;

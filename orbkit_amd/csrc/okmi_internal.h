@@ -33,6 +33,7 @@ struct OkmiBasis {
     ShellRec* d_fshells = nullptr;   // row0 = internal row, rcut2 = fused radius
     ShellRec* d_oshells = nullptr;   // output rows, rcut2 = fused radius
     ShellRec* d_ishells = nullptr;   // internal rows (prim0 into d_fprims), rcut2 = fused radius
+    ShellRec* d_wshells = nullptr;   // the same in work order (entry j of a chunk's list is record j)
     int* d_padrows = nullptr;        // internal rows that hold no AO (chunk padding)
     int npadrows = 0;
     PrimRec* d_fprims = nullptr;     // rcut2 = fused radius
@@ -60,9 +61,19 @@ int okmi_launch_ao_eval(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64
 struct AoBricks {
     int64_t bk0, nbk, nbx, nby, nbz;
 };
+// With the shells in the fused row order (d_ishells): per tile of 64 << shift
+// columns, bit c of mask[tile * words + c / 64] marks chunk c live; a shell
+// out of reach of a wave's points whose chunk is dead for the wave's tile
+// writes nothing (the contraction never reads those rows)
+struct AoLiveMask {
+    const unsigned long long* mask;
+    int words;
+    int shift;
+};
 int okmi_launch_ao_eval_ex(const OkmiBasis* b, const ShellRec* shells, const PrimRec* prims, const GridDesc& g,
                            int64_t p0, int64_t np, const AoBricks* bricks, const int* drv, int ndrv, int out_dtype,
-                           void* out, int64_t ld, int64_t set_stride, hipStream_t st);
+                           void* out, int64_t ld, int64_t set_stride, hipStream_t st,
+                           const AoLiveMask* live = nullptr);
 
 // fused-path tables (internal row order, fused screening radii) and the
 // output-row shell table with the fused radii (d_oshells); built once
@@ -100,7 +111,9 @@ int okmi_launch_rho_f64(const double* A, int64_t ld, const int64_t* set_off, int
 // points p0 + column) the 32-row chunks of the fused layout with a shell in
 // reach, in order: list[tile][..], count[tile]
 int okmi_launch_live_chunks(const OkmiBasis* b, const GridDesc& g, int brick, int64_t bk0, int64_t p0, int64_t n,
-                            int np, int* list, int* count, hipStream_t st);
+                            int np, int* list, int* count, hipStream_t st, unsigned long long* mask = nullptr);
+// 64-bit words per tile of a live-chunk mask
+inline int live_mask_words(const OkmiBasis* b) { return (b->nchunk + 63) / 64; }
 // C [nmo][nao] -> [nmo][kpad] in the fused row order (colmap[k]: C column of row k, -1: zero)
 int okmi_launch_permute_c(const double* C, int nmo, int nao, int64_t kpad, const int* colmap, double* out,
                           hipStream_t st);

@@ -374,8 +374,10 @@ struct LiveArgs {
     const int2* chunks;  // per chunk: first work entry, shells | used rows << 16
     const ShellRec* shells;
     int nchunk;
-    int* list;
+    int* list;                 // null: no lists
     int* count;
+    unsigned long long* mask;  // null: no masks; else [tile][words], bit c = chunk c live
+    int words;
 };
 
 __device__ __forceinline__ double wave_reduce(double v, bool take_min)
@@ -440,10 +442,11 @@ __global__ __launch_bounds__(64) void k_big_live(LiveArgs a)
             }
         }
         const unsigned long long m = __builtin_amdgcn_ballot_w64(live);
-        if (live) a.list[t * a.nchunk + base + __builtin_popcountll(m & ((1ull << lane) - 1))] = c;
+        if (live && a.list) a.list[t * a.nchunk + base + __builtin_popcountll(m & ((1ull << lane) - 1))] = c;
+        if (lane == 0 && a.mask) a.mask[t * a.words + c0 / 64] = m;
         base += __builtin_popcountll(m);
     }
-    if (lane == 0) a.count[t] = base;
+    if (lane == 0 && a.count) a.count[t] = base;
 }
 
 // zero rows `rows[0 .. nrows)` of every AO set over columns [0, ncol)
@@ -508,7 +511,8 @@ BigLayout big_layout(const OkmiBasis* b, int nmo, int prec, int S)
     L.occ_bytes = a256(8 * L.mpad);
     // AO sets (FP32), partials (FP64) and the live-chunk lists (at least 32
     // points per tile) per point
-    L.per_point = (size_t)S * L.kpad * 4 + (size_t)(L.mpad / MB) * S * 8 + ((size_t)L.kpad / KS + 1) * 4 / 32 + 1;
+    L.per_point = (size_t)S * L.kpad * 4 + (size_t)(L.mpad / MB) * S * 8 + ((size_t)L.kpad / KS + 1) * 4 / 32 +
+                  ((size_t)L.kpad / KS / 64 + 1) * 8 / 64 + 2;
     L.fixed = L.c_bytes + L.occ_bytes + 4 * 256 + 256;
     return L;
 }
@@ -516,7 +520,7 @@ BigLayout big_layout(const OkmiBasis* b, int nmo, int prec, int S)
 }  // namespace
 
 int okmi_launch_live_chunks(const OkmiBasis* b, const GridDesc& g, int brick, int64_t bk0, int64_t p0, int64_t n,
-                            int np, int* list, int* count, hipStream_t st)
+                            int np, int* list, int* count, hipStream_t st, unsigned long long* mask)
 {
     if (n <= 0) return 0;
     LiveArgs la{};
@@ -530,6 +534,7 @@ int okmi_launch_live_chunks(const OkmiBasis* b, const GridDesc& g, int brick, in
     la.p0 = p0; la.n = n; la.np = np;
     la.chunks = b->d_chunks; la.shells = b->d_fshells; la.nchunk = b->nchunk;
     la.list = list; la.count = count;
+    la.mask = mask; la.words = live_mask_words(b);
     hipLaunchKernelGGL(k_big_live, dim3((unsigned)((n + np - 1) / np)), dim3(64), 0, st, la);
     return 0;
 }
@@ -601,7 +606,13 @@ int okmi_launch_rho_big(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np,
     const int64_t ntile_max = (chunk + NP - 1) / NP;
     int* live = (int*)((char*)part + a256((size_t)nmt * S * 8 * chunk));
     int* live_count = live + ntile_max * b->nchunk;
-    if (L.kpad != b->nkpad || (size_t)((char*)(live_count + ntile_max) - (char*)ws) > ws_bytes) {
+    // per 64-point brick (128-point tile at S = 1) the live-chunk mask of the AO kernel
+    const int nw = live_mask_words(b);
+    const int mnp = NP == 128 ? 128 : 64;
+    unsigned long long* lmask =
+        (unsigned long long*)(((uintptr_t)(live_count + ntile_max) + 7) & ~(uintptr_t)7);
+    const int64_t nmask = (chunk + mnp - 1) / mnp;
+    if (L.kpad != b->nkpad || (size_t)((char*)(lmask + nmask * nw) - (char*)ws) > ws_bytes) {
         okmi_set_error("rho_big: inconsistent layout (%lld rows, %d fused rows)", (long long)L.kpad, b->nkpad);
         return 1;
     }
@@ -622,25 +633,36 @@ int okmi_launch_rho_big(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np,
     OutCodes oc{};
     oc.n = nout;
     for (int r = 0; r < nout; ++r) oc.c[r] = drv_out[r];
+    const AoLiveMask am{lmask, nw, mnp == 128 ? 1 : 0};
     for (int64_t c0 = 0; c0 < total; c0 += chunk) {
         const int64_t n = total - c0 < chunk ? total - c0 : chunk;
         int rc;
-        if (bm.on) {
-            AoBricks bk{c0 / 64, n / 64, bm.nbx, bm.nby, bm.nbz};
-            bm.bk0 = c0 / 64;
-            rc = okmi_launch_ao_eval_ex(b, b->d_ishells, b->d_fprims, g, 0, n, &bk, codes, S, OKMI_F32, ao, chunk,
-                                        (int64_t)L.kpad * chunk, st);
-        } else {
-            rc = okmi_launch_ao_eval_ex(b, b->d_ishells, b->d_fprims, g, p0 + c0, n, nullptr, codes, S, OKMI_F32, ao,
-                                        chunk, (int64_t)L.kpad * chunk, st);
-        }
-        if (rc) return rc;
+        if (bm.on) bm.bk0 = c0 / 64;
+        // live chunks first (the GEMM's lists per NP-point tile, the AO
+        // kernel's masks per brick or 128-point tile): the AO kernel leaves
+        // the rows of dead chunks unwritten
         const int64_t ntile = (n + NP - 1) / NP;
         LiveArgs la{};
         la.grid = g; la.bm = bm; la.p0 = p0 + c0; la.n = n; la.np = NP;
         la.chunks = b->d_chunks; la.shells = b->d_fshells; la.nchunk = b->nchunk;
         la.list = live; la.count = live_count;
+        la.words = nw;
+        if (NP == mnp) la.mask = lmask;
         hipLaunchKernelGGL(k_big_live, dim3((unsigned)ntile), dim3(64), 0, st, la);
+        if (NP != mnp) {
+            LiveArgs lm = la;
+            lm.np = mnp; lm.list = nullptr; lm.count = nullptr; lm.mask = lmask;
+            hipLaunchKernelGGL(k_big_live, dim3((unsigned)((n + mnp - 1) / mnp)), dim3(64), 0, st, lm);
+        }
+        if (bm.on) {
+            AoBricks bk{c0 / 64, n / 64, bm.nbx, bm.nby, bm.nbz};
+            rc = okmi_launch_ao_eval_ex(b, b->d_ishells, b->d_fprims, g, 0, n, &bk, codes, S, OKMI_F32, ao, chunk,
+                                        (int64_t)L.kpad * chunk, st, &am);
+        } else {
+            rc = okmi_launch_ao_eval_ex(b, b->d_ishells, b->d_fprims, g, p0 + c0, n, nullptr, codes, S, OKMI_F32, ao,
+                                        chunk, (int64_t)L.kpad * chunk, st, &am);
+        }
+        if (rc) return rc;
         BigArgs a{};
         a.ao = ao; a.ld = chunk; a.set_stride = (int64_t)L.kpad * chunk; a.nao = (int)L.kpad; a.np = n;
         a.C = cpl; a.mpad = L.mpad; a.kpad = L.kpad; a.occ = occp; a.partial = part; a.nmt = nmt; a.nmo = nmo;

@@ -190,6 +190,8 @@ int prepare_fused_tables(OkmiBasis* b)
     // writes (zeroed by that path)
     std::vector<ShellRec> is(os);
     for (int i = 0; i < b->nshell; ++i) is[i].row0 = krow[i];
+    std::vector<ShellRec> wsh(work.size());
+    for (size_t j = 0; j < work.size(); ++j) wsh[j] = is[work[j]];
     std::vector<int> padrows;
     for (int r = 0; r < nkpad; ++r)
         if (colmap[r] < 0) padrows.push_back(r);
@@ -201,6 +203,7 @@ int prepare_fused_tables(OkmiBasis* b)
     if (!rc) rc = upload(&b->d_fgroups, groups, "rho_fused: primitive groups");
     if (!rc) rc = upload(&b->d_oshells, os, "rho_fused: output-row shells");
     if (!rc) rc = upload(&b->d_ishells, is, "rho_fused: internal-row shells");
+    if (!rc) rc = upload(&b->d_wshells, wsh, "rho_fused: work-order shells");
     if (!rc) rc = upload(&b->d_padrows, padrows, "rho_fused: padding rows");
     if (!rc) rc = upload(&b->d_colmap, colmap, "rho_fused: row map");
     if (rc) return rc;

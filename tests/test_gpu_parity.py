@@ -560,3 +560,40 @@ def test_fused_live_rows(ok):
   assert 0.33 * b.nao * npts < full <= 416 * npts
   assert rows(x + 1e4, y, z) == 0.0
   assert rows(x[:12], y, z) + rows(x[12:], y, z) == full
+
+
+def test_okmi_rho_staged_subrange(ok):
+  '''okmi_rho_staged on points [p0, p0 + np) of a regular grid (the screened
+  path's point-order tiles, not bricks), 130 MOs (beyond the fused kernel),
+  against the oracle on the same points.'''
+  import ctypes
+  import torch
+  from orbkit_amd import _native
+  from orbkit_amd.basis import device_basis
+  from orbkit_amd.core import DeviceGrid
+  O = ok['O']
+  qc, axes, _ = ok['W'].make('medium', npts_side=11, nmo=130)
+  basis = device_basis(qc.geo_spec, qc.ao_spec)
+  C, occ = qc.mo_spec.get_coeffs(), qc.mo_spec.get_occ()
+  dg = DeviceGrid(*axes, regular=True)
+  x, y, z = O.expand(*axes)
+  p0, np_ = 100, 1000
+  codes = numpy.array([1, 2, 3], dtype=numpy.intc)
+  L, dev = _native.lib(), dg.x.device
+  dC, docc = torch.as_tensor(C, device=dev), torch.as_tensor(occ, device=dev)
+  nb = L.okmi_workspace_bytes(basis.handle, 4, 256) + C.shape[0] * 512 * 8
+  ws = torch.empty(nb // 8 + 1, dtype=torch.float64, device=dev)
+  rho = torch.empty(np_, dtype=torch.float64, device=dev)
+  delta = torch.empty((3, np_), dtype=torch.float64, device=dev)
+  print('case staged subrange', p0, np_, flush=True)
+  _native.check(L.okmi_rho_staged(basis.handle, ctypes.byref(dg.desc), p0, np_, ctypes.c_void_p(dC.data_ptr()),
+                                  ctypes.c_void_p(docc.data_ptr()), C.shape[0], codes.ctypes.data_as(_native.P_int),
+                                  3, ctypes.c_void_p(rho.data_ptr()), ctypes.c_void_p(delta.data_ptr()), np_, None,
+                                  ctypes.c_void_p(ws.data_ptr()), nb, ctypes.c_void_p(torch.cuda.current_stream()
+                                                                                      .cuda_stream)),
+                'okmi_rho_staged')
+  s = slice(p0, p0 + np_)
+  r = O.slice_rho(qc.geo_spec, qc.ao_spec, qc.mo_spec, x[s], y[s], z[s], ['x', 'y', 'z'], False, False,
+                  ok['tools'].get_cart2sph)
+  assert rel_err(rho.cpu().numpy(), r[0]) <= TOL
+  assert rel_err(delta.cpu().numpy(), r[2]) <= TOL
