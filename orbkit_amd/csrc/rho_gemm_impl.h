@@ -109,7 +109,7 @@ int launch_ns(const GemmArgs& a, int nsets, hipStream_t st)
     switch (nsets) {
 #define OKMI_NS_CASE(NS)                                                          \
     case NS:                                                                      \
-        if constexpr (MG * NS <= 32) {                                            \
+        if constexpr (MG * NS <= 24) {                                            \
             hipLaunchKernelGGL((k_gemm_f64<MG, NS>), grd, blk, 0, st, a);   \
             return 0;                                                             \
         }                                                                         \
@@ -123,11 +123,20 @@ int launch_ns(const GemmArgs& a, int nsets, hipStream_t st)
     return 1;
 }
 
+// MO tiles per group: at most 24 accumulator tiles (MG x sets, 4 doubles
+// each), which keeps every accumulator in VGPRs with 2 waves per SIMD; more
+// (4 x 7 and up) overflowed into AGPR copies at 1 wave per SIMD
+inline int mo_group(int nsets, int nmt)
+{
+    int mg = 24 / (nsets > 0 ? nsets : 1);
+    if (mg > 4) mg = 4;
+    if (mg < 1) mg = 1;
+    return nmt < mg ? nmt : mg;
+}
+
 inline int launch_mg(const GemmArgs& a, int nsets, hipStream_t st)
 {
-    const int nmt = (a.nmo + 15) / 16;
-    int mg = nsets <= 8 ? 4 : 3;
-    if (nmt < mg) mg = nmt;
+    const int mg = mo_group(nsets, (a.nmo + 15) / 16);
     switch (mg) {
     case 1: return launch_ns<1>(a, nsets, st);
     case 2: return launch_ns<2>(a, nsets, st);
@@ -309,9 +318,14 @@ int launch_rho_full(const GemmArgs& a, int nsc, hipStream_t st)
     switch (nsc) {
     case 1: hipLaunchKernelGGL((k_rho_f64<MG, 1, FULL>), grd, blk, 0, st, a); return 0;
     case 4: hipLaunchKernelGGL((k_rho_f64<MG, 4, FULL>), grd, blk, 0, st, a); return 0;
-    case 7: hipLaunchKernelGGL((k_rho_f64<MG, 7, FULL>), grd, blk, 0, st, a); return 0;
-    default:
+    case 7:
         if constexpr (MG <= 3) {
+            hipLaunchKernelGGL((k_rho_f64<MG, 7, FULL>), grd, blk, 0, st, a);
+            return 0;
+        }
+        break;
+    default:
+        if constexpr (MG <= 2) {
             hipLaunchKernelGGL((k_rho_f64<MG, 10, FULL>), grd, blk, 0, st, a);
             return 0;
         }
@@ -332,9 +346,7 @@ int launch_rho_nsc(const GemmArgs& a, int nsc, hipStream_t st)
 
 inline int launch_rho(const GemmArgs& a, int nsc, hipStream_t st)
 {
-    const int nmt = (a.nmo + 15) / 16;
-    int mg = nsc >= 10 ? 3 : 4;
-    if (nmt < mg) mg = nmt;
+    const int mg = mo_group(nsc, (a.nmo + 15) / 16);
     switch (mg) {
     case 1: return launch_rho_nsc<1>(a, nsc, st);
     case 2: return launch_rho_nsc<2>(a, nsc, st);
