@@ -331,8 +331,8 @@ int launch_pass_f64(const FusedArgs& a, int pass, int mpad, int lm, hipStream_t 
     switch (pass) {
     case 0: return l3 ? launch_f64_m<0, 3>(a, mpad, st) : launch_f64_m<0, 4>(a, mpad, st);
     case 1: return l3 ? launch_f64_m<1, 3>(a, mpad, st) : launch_f64_m<1, 4>(a, mpad, st);
-    case 2: return launch_f64_m<2, 4>(a, mpad, st);
-    default: return launch_f64_m<3, 4>(a, mpad, st);
+    case 2: return l3 ? launch_f64_m<2, 3>(a, mpad, st) : launch_f64_m<2, 4>(a, mpad, st);
+    default: return l3 ? launch_f64_m<3, 3>(a, mpad, st) : launch_f64_m<3, 4>(a, mpad, st);
     }
 }
 
