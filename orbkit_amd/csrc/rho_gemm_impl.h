@@ -8,7 +8,10 @@
 // A workgroup is 4 waves; each wave owns 16 points and loops over MO tiles
 // of 16 in groups of MG, accumulating every AO set (NS) for those tiles in
 // registers.  In density mode the MOs are reduced in registers and never
-// written; in MO mode Phi is stored [set][mo][point].
+// written; in MO mode Phi is stored [set][mo][point].  With more than one MO
+// group (density mode) the 4 waves instead share 16 points and split the
+// groups between them, so that the AO rows of a K step are fetched once for
+// four groups.
 #pragma once
 #include "okmi_internal.h"
 
@@ -42,6 +45,11 @@ struct GemmArgs {
     // grid points for the outputs
     int screened;
     F64Screen scr;
+    // density mode: the 4 waves of a workgroup share 16 points and take the
+    // MO groups round-robin (A rows read by 4 waves together, not once per
+    // group), their sums added in wave order at the end; else each wave owns
+    // 16 points of the workgroup's 64 and runs every group
+    int wave_groups;
 };
 
 template <int MG, int NS>
@@ -178,7 +186,8 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n = lane & 15, kq = lane >> 4;
-    const int64_t p = (int64_t)blockIdx.x * 64 + wave * 16 + n;
+    const bool wg = a.wave_groups;
+    const int64_t p = wg ? (int64_t)blockIdx.x * 16 + n : (int64_t)blockIdx.x * 64 + wave * 16 + n;
     bool pin = p < a.np;
     int64_t q = p;   // output index of this lane's point
     if (a.screened && a.scr.brick) {
@@ -192,12 +201,13 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
         q = (ix * a.scr.ny + iy) * a.scr.nz + iz;
     }
     // K steps of 4 rows: every row, or the 8 steps of each live chunk of
-    // this block's 64 columns
+    // the 64-column tile holding this block's columns
     const int* lst = nullptr;
     int nsteps = (a.nao + 3) / 4;
     if (a.screened) {
-        lst = a.scr.live + (int64_t)blockIdx.x * a.scr.nchunk;
-        nsteps = 8 * a.scr.live_count[blockIdx.x];
+        const int64_t tile = wg ? blockIdx.x >> 2 : blockIdx.x;
+        lst = a.scr.live + tile * a.scr.nchunk;
+        nsteps = 8 * a.scr.live_count[tile];
     }
     const int ldc = a.screened ? a.scr.ldc : a.nao;
     const int nmt = (a.nmo + 15) >> 4;
@@ -208,7 +218,7 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
 #pragma unroll
     for (int d = 0; d < NSC; ++d) dsum[d] = 0.0;
 
-    for (int g = 0; g < ngroups; ++g) {
+    for (int g = wg ? wave : 0; g < ngroups; g += wg ? 4 : 1) {
         d4 acc[MG][NSC];
 #pragma unroll
         for (int mt = 0; mt < MG; ++mt)
@@ -298,6 +308,26 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
             dsum[d] += __shfl_xor(dsum[d], 32);
         }
     }
+    if (wg) {
+        // the four waves' sums over their MO groups, added in wave order
+        __shared__ double red[4][16][NSC];
+        if (kq == 0) {
+            red[wave][n][0] = rho_acc;
+#pragma unroll
+            for (int d = 1; d < NSC; ++d) red[wave][n][d] = dsum[d];
+        }
+        __syncthreads();
+        if (wave != 0) return;
+        rho_acc = red[0][n][0];
+#pragma unroll
+        for (int d = 1; d < NSC; ++d) dsum[d] = red[0][n][d];
+#pragma unroll
+        for (int w = 1; w < 4; ++w) {
+            rho_acc += red[w][n][0];
+#pragma unroll
+            for (int d = 1; d < NSC; ++d) dsum[d] += red[w][n][d];
+        }
+    }
     if (kq == 0 && pin) {
         a.rho[q] = rho_acc;
         for (int r = 0; r < a.nout; ++r) {
@@ -314,7 +344,7 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
 template <int MG, bool FULL>
 int launch_rho_full(const GemmArgs& a, int nsc, hipStream_t st)
 {
-    dim3 blk(256), grd((unsigned)((a.np + 63) / 64));
+    dim3 blk(256), grd((unsigned)(a.wave_groups ? (a.np + 15) / 16 : (a.np + 63) / 64));
     switch (nsc) {
     case 1: hipLaunchKernelGGL((k_rho_f64<MG, 1, FULL>), grd, blk, 0, st, a); return 0;
     case 4: hipLaunchKernelGGL((k_rho_f64<MG, 4, FULL>), grd, blk, 0, st, a); return 0;
@@ -344,9 +374,11 @@ int launch_rho_nsc(const GemmArgs& a, int nsc, hipStream_t st)
     return launch_rho_full<MG, false>(a, nsc, st);
 }
 
-inline int launch_rho(const GemmArgs& a, int nsc, hipStream_t st)
+inline int launch_rho(const GemmArgs& args, int nsc, hipStream_t st)
 {
-    const int mg = mo_group(nsc, (a.nmo + 15) / 16);
+    const int nmt = (args.nmo + 15) / 16, mg = mo_group(nsc, nmt);
+    GemmArgs a = args;
+    a.wave_groups = (nmt + mg - 1) / mg >= 2;
     switch (mg) {
     case 1: return launch_rho_nsc<1>(a, nsc, st);
     case 2: return launch_rho_nsc<2>(a, nsc, st);
