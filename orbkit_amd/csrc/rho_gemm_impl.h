@@ -9,9 +9,10 @@
 // of 16 in groups of MG, accumulating every AO set (NS) for those tiles in
 // registers.  In density mode the MOs are reduced in registers and never
 // written; in MO mode Phi is stored [set][mo][point].  With more than one MO
-// group (density mode) the 4 waves instead share 16 points and split the
-// groups between them, so that the AO rows of a K step are fetched once for
-// four groups.
+// group (density mode) the waves of a workgroup instead share 16 points and
+// split the groups between them (one wave per group up to 8 groups, else 4
+// waves), so that the AO rows of a K step are fetched once for several
+// groups.
 #pragma once
 #include "okmi_internal.h"
 
@@ -45,10 +46,11 @@ struct GemmArgs {
     // grid points for the outputs
     int screened;
     F64Screen scr;
-    // density mode: the 4 waves of a workgroup share 16 points and take the
-    // MO groups round-robin (A rows read by 4 waves together, not once per
-    // group), their sums added in wave order at the end; else each wave owns
-    // 16 points of the workgroup's 64 and runs every group
+    // density mode: the W waves of a workgroup (W = blockDim / 64) share 16
+    // points and take the MO groups round-robin (A rows read by W waves
+    // together, not once per group), their sums added in wave order at the
+    // end; else 4 waves each own 16 points of the workgroup's 64 and run
+    // every group
     int wave_groups;
 };
 
@@ -182,9 +184,9 @@ __device__ __forceinline__ void rho_terms(const d4 (&acc)[NSC], int i, double f0
 // unconditional (rows clamped, the zeroing done by selects), a straight
 // sequence of MG x NSC MFMAs per K step
 template <int MG, int NSC, bool FULL>
-__global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
+__global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     const int n = lane & 15, kq = lane >> 4;
     const bool wg = a.wave_groups;
     const int64_t p = wg ? (int64_t)blockIdx.x * 16 + n : (int64_t)blockIdx.x * 64 + wave * 16 + n;
@@ -218,7 +220,7 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
 #pragma unroll
     for (int d = 0; d < NSC; ++d) dsum[d] = 0.0;
 
-    for (int g = wg ? wave : 0; g < ngroups; g += wg ? 4 : 1) {
+    for (int g = wg ? wave : 0; g < ngroups; g += wg ? nwave : 1) {
         d4 acc[MG][NSC];
 #pragma unroll
         for (int mt = 0; mt < MG; ++mt)
@@ -309,8 +311,8 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
         }
     }
     if (wg) {
-        // the four waves' sums over their MO groups, added in wave order
-        __shared__ double red[4][16][NSC];
+        // the waves' sums over their MO groups, added in wave order
+        __shared__ double red[8][16][NSC];
         if (kq == 0) {
             red[wave][n][0] = rho_acc;
 #pragma unroll
@@ -321,8 +323,7 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
         rho_acc = red[0][n][0];
 #pragma unroll
         for (int d = 1; d < NSC; ++d) dsum[d] = red[0][n][d];
-#pragma unroll
-        for (int w = 1; w < 4; ++w) {
+        for (int w = 1; w < nwave; ++w) {
             rho_acc += red[w][n][0];
 #pragma unroll
             for (int d = 1; d < NSC; ++d) dsum[d] += red[w][n][d];
@@ -344,7 +345,10 @@ __global__ __launch_bounds__(256) void k_rho_f64(GemmArgs a)
 template <int MG, bool FULL>
 int launch_rho_full(const GemmArgs& a, int nsc, hipStream_t st)
 {
-    dim3 blk(256), grd((unsigned)(a.wave_groups ? (a.np + 15) / 16 : (a.np + 63) / 64));
+    // wave_groups: one wave per MO group up to 8 groups, else 4 waves
+    const int nmt = (a.nmo + 15) / 16, ng = (nmt + MG - 1) / MG;
+    dim3 blk(a.wave_groups ? 64 * (ng <= 8 ? ng : 4) : 256);
+    dim3 grd((unsigned)(a.wave_groups ? (a.np + 15) / 16 : (a.np + 63) / 64));
     switch (nsc) {
     case 1: hipLaunchKernelGGL((k_rho_f64<MG, 1, FULL>), grd, blk, 0, st, a); return 0;
     case 4: hipLaunchKernelGGL((k_rho_f64<MG, 4, FULL>), grd, blk, 0, st, a); return 0;
