@@ -235,38 +235,66 @@ __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
             cbase[mt] = a.C + (int64_t)(min_[mt] ? m : 0) * ldc;
         }
         const double* abase = a.A + (p < a.np ? p : 0);
-        for (int ks = 0; ks < nsteps; ++ks) {
-            const int k0 = a.screened ? lst[ks >> 3] * 32 + 4 * (ks & 7) : 4 * ks;
-            const int k = k0 + kq;
-            const bool kin = k < a.nao;
-            // C column of A row k (fused row order: through colmap, -1 = zero row)
-            int kcol = k;
-            bool cin = kin;
+        // A row of K step ks for this lane, and the C column it pairs with
+        // (fused row order: through colmap, -1 = zero row)
+        auto row_of = [&](int ks, int& k, bool& kin, int& kcol, bool& cin) {
+            k = (a.screened ? lst[ks >> 3] * 32 + 4 * (ks & 7) : 4 * ks) + kq;
+            kin = k < a.nao;
+            kcol = k;
+            cin = kin;
             if (a.screened && a.scr.colmap) {
                 kcol = a.scr.colmap[kin ? k : 0];
                 cin = kin && kcol >= 0;
             }
-            double av[MG];
-            if constexpr (FULL) {
-                // Rows past the end read row nao - 1 and are zeroed through
-                // av; points past the end read point 0 and are zeroed
-                // through bv (they must add nothing to mo_norm).  The zeroing
-                // multiplies by 0 or 1 (the operands are finite) instead of
-                // selecting: the compiler turns a select of a load into a
-                // masked load in its own branch with its own wait, which
-                // serialised the K step's loads.
+        };
+        if constexpr (FULL) {
+            // Rows past the end read row nao - 1 and are zeroed through av;
+            // points past the end read point 0 and are zeroed through bv
+            // (they must add nothing to mo_norm).  The zeroing multiplies by
+            // 0 or 1 (the operands are finite) instead of selecting: the
+            // compiler turns a select of a load into a masked load in its own
+            // branch with its own wait, which serialised the K step's loads.
+            // Up to 16 accumulator tiles the operands of step ks + 1 are
+            // loaded before the MFMAs of step ks (raw loads; the masks
+            // applied when they are used); with more, the registers of a
+            // second operand set would spill.
+            constexpr bool PREF = MG * NSC <= 16;
+            const double pm = pin ? 1.0 : 0.0;
+            double ra[MG], rb[NSC], rm = 0.0;
+            auto fetch = [&](int ks) {
+                int k, kcol;
+                bool kin, cin;
+                row_of(ks, k, kin, kcol, cin);
                 const int kc = kin ? k : a.nao - 1;
                 const int cc = cin ? kcol : 0;
 #pragma unroll
-                for (int mt = 0; mt < MG; ++mt) av[mt] = cbase[mt][cc] * ((cin && min_[mt]) ? 1.0 : 0.0);
+                for (int mt = 0; mt < MG; ++mt) ra[mt] = cbase[mt][cc];
 #pragma unroll
-                for (int d = 0; d < NSC; ++d) {
-                    const double bv = abase[a.set_off[d] + (int64_t)kc * a.ld] * (pin ? 1.0 : 0.0);
+                for (int d = 0; d < NSC; ++d) rb[d] = abase[a.set_off[d] + (int64_t)kc * a.ld];
+                rm = cin ? 1.0 : 0.0;
+            };
+            if (PREF && nsteps > 0) fetch(0);
+            for (int ks = 0; ks < nsteps; ++ks) {
+                if constexpr (!PREF) fetch(ks);
+                double av[MG], bv[NSC];
+#pragma unroll
+                for (int mt = 0; mt < MG; ++mt) av[mt] = ra[mt] * (min_[mt] ? rm : 0.0);
+#pragma unroll
+                for (int d = 0; d < NSC; ++d) bv[d] = rb[d] * pm;
+                if constexpr (PREF)
+                    if (ks + 1 < nsteps) fetch(ks + 1);
+#pragma unroll
+                for (int d = 0; d < NSC; ++d)
 #pragma unroll
                     for (int mt = 0; mt < MG; ++mt)
-                        acc[mt][d] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bv, acc[mt][d], 0, 0, 0);
-                }
-            } else {
+                        acc[mt][d] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bv[d], acc[mt][d], 0, 0, 0);
+            }
+        } else {
+            for (int ks = 0; ks < nsteps; ++ks) {
+                int k, kcol;
+                bool kin, cin;
+                row_of(ks, k, kin, kcol, cin);
+                double av[MG];
 #pragma unroll
                 for (int mt = 0; mt < MG; ++mt) av[mt] = (cin && min_[mt]) ? cbase[mt][kcol] : 0.0;
 #pragma unroll
