@@ -247,9 +247,10 @@ def density_leg(args, torch, info, basis, dg, C, occ, codes, path, axes=None):
           'algorithmic_flop_per_step': flops}
   if dom[3] > 1:
     roof.update(executed_tflops=achieved * dom[3], mfma_dtype=dom[2], products=dom[3])
-  if axes is not None and dom[0] in ('rho_fused_f64', 'rho_fused', 'rho_big'):
-    # the screened contraction: per brick only the AO chunks with a shell in
-    # reach (the work the kernel does besides MO-tile padding)
+  if axes is not None:
+    # the screened contraction (every path: fused, many-MO and staged): per
+    # brick only the AO chunks with a shell in reach (the work the kernel
+    # does besides MO-tile padding)
     sflops = 2.0 * nmo * nsets * screened_rows(L, basis, axes)
     roof['screened'] = {'flop_per_step': sflops, 'achieved': sflops / t_launch / 1e12,
                         'frac': sflops / t_launch / 1e12 / peak, 'share_of_algorithmic': sflops / flops,
