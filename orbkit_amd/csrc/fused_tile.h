@@ -89,7 +89,7 @@ struct FusedArgs {
     int need_mask;            // codes whose terms are wanted
     int brick;                // regular grid tiled in bricks of 64 points (p0 = 0, all points)
     int tlx, tly, tlz;        // brick extents 2^tlx x 2^tly x 2^tlz (tlx + tly + tlz = 6)
-    int tl_pad;
+    int mt_live;              // MO tiles of 16 holding a real MO (0: all MT of the instance)
     int64_t nbx, nby, nbz;    // bricks per axis
     double* norm_part;        // FP64 kernel: per tile sum_p Phi_m(p)^2, [tile][mpad] (null: not wanted)
 };
@@ -448,11 +448,14 @@ __device__ __forceinline__ void eval_chunk(const FusedArgs& a, int ch, const Box
 
 // Contraction of one chunk (LDS tile L) for the 16-point column `col` of the
 // tile: acc[mt][s] += C[16 mt .. 16 mt + 15][chunk] * AO_s[chunk][column].
+// MO tiles from mtl on (mtl > 0) hold only padding: their MFMAs are skipped
+// (their accumulators stay zero, and their occupations are zero).
 template <int PREC, int PASS, int MT>
 __device__ __forceinline__ void contract(const typename Tile<PREC>::T* L, int col, int g, int n16,
                                          const COperand<PREC, MT>& c,
-                                         typename Tile<PREC>::Acc (&acc)[MT][PassCodes<PASS>::n])
+                                         typename Tile<PREC>::Acc (&acc)[MT][PassCodes<PASS>::n], int mtl)
 {
+    const int nmt = mtl > 0 && mtl < MT ? mtl : MT;   // wave-uniform
     constexpr int NS = PassCodes<PASS>::n;
     constexpr int LD = Tile<PREC>::LD;
 #pragma unroll
@@ -464,7 +467,7 @@ __device__ __forceinline__ void contract(const typename Tile<PREC>::T* L, int co
                 const double b = src[(4 * ks + g) * LD];
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
-                    acc[mt][s] = __builtin_amdgcn_mfma_f64_16x16x4f64(c.cd[mt][ks], b, acc[mt][s], 0, 0, 0);
+                    if (mt < nmt) acc[mt][s] = __builtin_amdgcn_mfma_f64_16x16x4f64(c.cd[mt][ks], b, acc[mt][s], 0, 0, 0);
             }
         } else if constexpr (PREC == OKMI_PREC_F32) {
 #pragma unroll
@@ -472,7 +475,7 @@ __device__ __forceinline__ void contract(const typename Tile<PREC>::T* L, int co
                 const float b = src[(8 * g + ks) * LD];   // chunk row 8 g + ks, as load_c
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
-                    acc[mt][s] = __builtin_amdgcn_mfma_f32_16x16x4f32(c.cf[mt][ks], b, acc[mt][s], 0, 0, 0);
+                    if (mt < nmt) acc[mt][s] = __builtin_amdgcn_mfma_f32_16x16x4f32(c.cf[mt][ks], b, acc[mt][s], 0, 0, 0);
             }
         } else {
             float bx[8];
@@ -481,7 +484,8 @@ __device__ __forceinline__ void contract(const typename Tile<PREC>::T* L, int co
             Split<PREC> bs;
             split8<PREC>(bx, bs);
 #pragma unroll
-            for (int mt = 0; mt < MT; ++mt) acc[mt][s] = mfma_split<PREC>(c.ca[mt], bs, acc[mt][s]);
+            for (int mt = 0; mt < MT; ++mt)
+                if (mt < nmt) acc[mt][s] = mfma_split<PREC>(c.ca[mt], bs, acc[mt][s]);
         }
     }
 }
@@ -714,7 +718,7 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, typen
             if (live[0] == ch) {
                 if constexpr (CStage<PREC, MT>::on) read_c<PREC, MT>(cst, g, n16, c);
                 else if constexpr (!P::hold_c) load_c<PREC, MT>(a, ch * KC, g, n16, c);
-                contract<PREC, PASS, MT>(lds, wave, g, n16, c, acc);
+                contract<PREC, PASS, MT>(lds, wave, g, n16, c, acc, a.mt_live);
             }
             __syncthreads();
         }
@@ -745,7 +749,7 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, typen
         // one, and a stale value can only let a zero chunk through.
         if (live[buf] == ch) {
             if constexpr (!c_early) load_c<PREC, MT>(a, ch * KC, g, n16, c);
-            contract<PREC, PASS, MT>(L, wave, g, n16, c, acc);
+            contract<PREC, PASS, MT>(L, wave, g, n16, c, acc, a.mt_live);
         }
         buf ^= 1;
     }

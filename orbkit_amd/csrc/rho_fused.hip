@@ -616,6 +616,7 @@ int okmi_batch_build(int nmol, OkmiBasis* const* bases, const GridDesc* grids, c
         a.ld_term = npts[m];
         a.need_mask = d_grad[m] ? 0xE : 0;
         a.brick = g.regular;
+        a.mt_live = (nmo[m] + 15) / 16;
         args[j] = a;
         PrepItem it{};
         it.colmap = bs->d_colmap; it.C = d_c[m]; it.occ = d_occ[m]; it.planes = (void*)a.C; it.occ_pad = (double*)a.occ;
