@@ -448,14 +448,15 @@ __device__ __forceinline__ void eval_chunk(const FusedArgs& a, int ch, const Box
 
 // Contraction of one chunk (LDS tile L) for the 16-point column `col` of the
 // tile: acc[mt][s] += C[16 mt .. 16 mt + 15][chunk] * AO_s[chunk][column].
-// MO tiles from mtl on (mtl > 0) hold only padding: their MFMAs are skipped
-// (their accumulators stay zero, and their occupations are zero).
-template <int PREC, int PASS, int MT>
+// SKIP (the batch kernel): MO tiles from mtl on (mtl > 0) hold only
+// padding, their MFMAs are skipped (their accumulators stay zero, and their
+// occupations are zero).
+template <int PREC, int PASS, int MT, bool SKIP = false>
 __device__ __forceinline__ void contract(const typename Tile<PREC>::T* L, int col, int g, int n16,
                                          const COperand<PREC, MT>& c,
-                                         typename Tile<PREC>::Acc (&acc)[MT][PassCodes<PASS>::n], int mtl)
+                                         typename Tile<PREC>::Acc (&acc)[MT][PassCodes<PASS>::n], int mtl = 0)
 {
-    const int nmt = mtl > 0 && mtl < MT ? mtl : MT;   // wave-uniform
+    const int nmt = SKIP && mtl > 0 && mtl < MT ? mtl : MT;   // wave-uniform
     constexpr int NS = PassCodes<PASS>::n;
     constexpr int LD = Tile<PREC>::LD;
 #pragma unroll
@@ -718,7 +719,7 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, typen
             if (live[0] == ch) {
                 if constexpr (CStage<PREC, MT>::on) read_c<PREC, MT>(cst, g, n16, c);
                 else if constexpr (!P::hold_c) load_c<PREC, MT>(a, ch * KC, g, n16, c);
-                contract<PREC, PASS, MT>(lds, wave, g, n16, c, acc, a.mt_live);
+                contract<PREC, PASS, MT, BATCH>(lds, wave, g, n16, c, acc, a.mt_live);
             }
             __syncthreads();
         }
@@ -749,7 +750,7 @@ __device__ __forceinline__ void rho_tile(const FusedArgs& a, int64_t tile, typen
         // one, and a stale value can only let a zero chunk through.
         if (live[buf] == ch) {
             if constexpr (!c_early) load_c<PREC, MT>(a, ch * KC, g, n16, c);
-            contract<PREC, PASS, MT>(L, wave, g, n16, c, acc, a.mt_live);
+            contract<PREC, PASS, MT, BATCH>(L, wave, g, n16, c, acc, a.mt_live);
         }
         buf ^= 1;
     }
