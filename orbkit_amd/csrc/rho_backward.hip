@@ -108,46 +108,64 @@ __global__ void k_pack_w(const R* W, int M, int K, R* Wp)
 // operands of K step s + D are loaded while step s is multiplied (D-deep
 // register ring), which hides the HBM latency of X behind the MFMAs.  Rows
 // k >= K of the last step read row K - 1 (finite), whose W entries are 0.
-template <typename R, int MG>
+//
+// Screened modes (rows in the fused order, 32-row chunks; the wave's 64
+// points are one tile of the live-chunk lists live[tile][..], lcount[tile]):
+//   MODE 1  K runs over the tile's live chunks only (Phi = C A: the rows of
+//           the other chunks are not evaluated);
+//   MODE 2  only the M tiles of the tile's live chunks are computed, two per
+//           chunk (MG = 2; G = C^T B: the other rows are never read).
+template <typename R, int MG, int MODE>
 __global__ __launch_bounds__(256, 2) void k_gemm_nn(const R* __restrict__ Wp, int M, int K,
                                                     const R* __restrict__ X, int64_t ldx, int64_t npad,
-                                                    R* __restrict__ out, int64_t ldo)
+                                                    R* __restrict__ out, int64_t ldo, const int* __restrict__ live,
+                                                    const int* __restrict__ lcount, int nchunk)
 {
+    static_assert(MODE != 2 || MG == 2, "MODE 2 computes the two M tiles of one chunk per group");
     typedef typename Mfma<R>::v4 V;
     constexpr int D = 4;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, q = lane >> 4;
-    const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * 64;
+    const int64_t tile = (int64_t)blockIdx.x * 4 + wave;
+    const int64_t p0 = tile * 64;
     if (p0 >= npad) return;
     const R* xp = X + p0 + 4 * r;
     const int nmt = (M + 15) / 16, nks = (K + 3) / 4;
-    for (int g0 = 0; g0 < nmt; g0 += MG) {
+    const int* lst = MODE ? live + tile * nchunk : nullptr;
+    const int nlive = MODE ? __builtin_amdgcn_readfirstlane(lcount[tile]) : 0;
+    // K steps of this wave and the packed step index of step st
+    const int nst = MODE == 1 ? 8 * nlive : nks;
+    auto kstep = [&](int st) { return MODE == 1 ? lst[st >> 3] * 8 + (st & 7) : st; };
+    const int ngroup = MODE == 2 ? nlive : (nmt + MG - 1) / MG;
+    for (int gi = 0; gi < ngroup; ++gi) {
+        const int g0 = MODE == 2 ? 2 * lst[gi] : gi * MG;   // first M tile of the group
         const R* wp = Wp + (int64_t)g0 * nks * 64 + lane;
         V xr[D];
         R wr[D][MG];
         auto load = [&](int st, V& xv, R (&wv)[MG]) {
-            xv = *(const V*)(xp + (int64_t)min(4 * st + q, K - 1) * ldx);
+            const int ks = kstep(st);
+            xv = *(const V*)(xp + (int64_t)min(4 * ks + q, K - 1) * ldx);
 #pragma unroll
-            for (int mt = 0; mt < MG; ++mt) wv[mt] = g0 + mt < nmt ? wp[((int64_t)mt * nks + st) * 64] : R(0);
+            for (int mt = 0; mt < MG; ++mt) wv[mt] = g0 + mt < nmt ? wp[((int64_t)mt * nks + ks) * 64] : R(0);
         };
 #pragma unroll
         for (int d = 0; d < D; ++d)
-            if (d < nks) load(d, xr[d], wr[d]);
+            if (d < nst) load(d, xr[d], wr[d]);
         V acc[MG][4];
 #pragma unroll
         for (int mt = 0; mt < MG; ++mt)
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[mt][j] = V{0, 0, 0, 0};
-        for (int s0 = 0; s0 < nks; s0 += D) {
+        for (int s0 = 0; s0 < nst; s0 += D) {
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const int st = s0 + d;
-                if (st < nks) {
+                if (st < nst) {
                     const V xv = xr[d];
                     R wv[MG];
 #pragma unroll
                     for (int mt = 0; mt < MG; ++mt) wv[mt] = wr[d][mt];
-                    if (st + D < nks) load(st + D, xr[d], wr[d]);
+                    if (st + D < nst) load(st + D, xr[d], wr[d]);
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -178,12 +196,17 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(const R* __restrict__ Wp, in
 // row.  Rows are padded with zeros up to a multiple of 16 points (krange a
 // multiple of 16, rows aligned to 4 elements); the rows of the next step are
 // loaded while a step is multiplied.  The partials are written (no atomics)
-// and summed in FP64 in a fixed order by k_reduce_dc.
+// and summed in FP64 in a fixed order by k_reduce_dc.  With `mask` (rows in
+// the fused order, points in 64-point tiles with live-chunk masks
+// mask[tile][nw]) the rows of a tile's dead chunks are not read: they count
+// as zero (their values are below the screening bound and were not
+// evaluated), and a step with both of the wave's chunks dead is skipped.
 constexpr int NT = 4;
 template <typename R, int MG>
 __global__ __launch_bounds__(64, 2) void k_gemm_tn(const R* __restrict__ B, int nmo, const R* __restrict__ A, int nao,
                                                 int64_t kend, int64_t ld, int64_t krange, int nsplit,
-                                                R* __restrict__ part)
+                                                R* __restrict__ part, const unsigned long long* __restrict__ mask,
+                                                int nw, int nchunk)
 {
     typedef typename Mfma<R>::v4 V;
     const int ntile = (nao + 16 * NT - 1) / (16 * NT);
@@ -220,11 +243,23 @@ __global__ __launch_bounds__(64, 2) void k_gemm_tn(const R* __restrict__ B, int 
             for (int u = 0; u < NT; ++u) acc[mt][u] = V{0, 0, 0, 0};
         // the next step's rows are loaded while this step is multiplied
         V nb[MG], na[NT];
+        bool nany = true;
         auto load = [&](int64_t k) {
+            bool lv[2] = {true, true};   // the wave's two 32-row chunks live in this point tile
+            if (mask) {
+                const unsigned long long* mrow = mask + (k >> 6) * nw;
 #pragma unroll
-            for (int mt = 0; mt < MG; ++mt) nb[mt] = bp[mt][k >> 2];
+                for (int h = 0; h < 2; ++h) {
+                    const int c = (n0 >> 5) + h;
+                    lv[h] = c < nchunk && ((mrow[c >> 6] >> (c & 63)) & 1);
+                }
+            }
+            nany = lv[0] || lv[1];
+            if (nany)
 #pragma unroll
-            for (int u = 0; u < NT; ++u) na[u] = ap[u][k >> 2];
+                for (int mt = 0; mt < MG; ++mt) nb[mt] = bp[mt][k >> 2];
+#pragma unroll
+            for (int u = 0; u < NT; ++u) na[u] = lv[u >> 1] ? ap[u][k >> 2] : V{0, 0, 0, 0};
         };
         if (k0 < k1) load(k0);
         for (int64_t k = k0; k < k1; k += 16) {
@@ -235,7 +270,9 @@ __global__ __launch_bounds__(64, 2) void k_gemm_tn(const R* __restrict__ B, int 
             for (int mt = 0; mt < MG; ++mt) vb[mt] = nb[mt];
 #pragma unroll
             for (int u = 0; u < NT; ++u) va[u] = na[u];
+            const bool any = nany;
             if (k + 16 < k1) load(k + 16);
+            if (!any) continue;
 #pragma unroll
             for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -282,6 +319,32 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Columns of a chunk laid out in 4x4x4 bricks of a whole regular grid, from
+// brick bk0 (z fastest within a brick; bricks z fastest, then y, then x),
+// lanes of a brick outside the grid repeating its nearest grid point.
+struct BwdBricks {
+    int on;
+    int64_t bk0, nbx, nby, nbz, nx, ny, nz;
+};
+
+// grid indices of column `col`; returns whether the point lies in the grid
+__device__ __forceinline__ bool brick_index(const BwdBricks& bm, int64_t col, int64_t& ix, int64_t& iy, int64_t& iz)
+{
+    const int l = (int)(col & 63);
+    int64_t t = bm.bk0 + (col >> 6);
+    const int64_t bz = t % bm.nbz;
+    t /= bm.nbz;
+    const int64_t by = t % bm.nby, bx = t / bm.nby;
+    ix = 4 * bx + (l >> 4);
+    iy = 4 * by + ((l >> 2) & 3);
+    iz = 4 * bz + (l & 3);
+    const bool in = ix < bm.nx && iy < bm.ny && iz < bm.nz;
+    ix = ix < bm.nx ? ix : bm.nx - 1;
+    iy = iy < bm.ny ? iy : bm.ny - 1;
+    iz = iz < bm.nz ? iz : bm.nz - 1;
+    return in;
+}
+
 template <typename R>
 struct ShellGradArgs {
     const ShellRec* shells;
@@ -294,6 +357,11 @@ struct ShellGradArgs {
     const R* G;          // [nao][ld]
     double* partial;     // [nrange][nprim_total + 3 nshell]
     int64_t ncol;
+    // screened (whole regular grid): columns in bricks, G rows in the fused
+    // order written only for the live chunks of each 64-point tile
+    BwdBricks bm;
+    const unsigned long long* mask;   // [tile][nw]
+    int nw;
 };
 
 // Per-lane contributions of one point to the shell's position gradient and
@@ -373,11 +441,21 @@ __device__ void shell_grad(const ShellGradArgs<R>& a, const ShellRec& sh, int pw
     const int64_t r1 = min(a.np, r0 + RANGE);
     for (int64_t b = r0; b < r1; b += 256) {
         const int64_t i = b + threadIdx.x;
-        const bool act = i < r1;
+        bool act = i < r1;
         double px, py, pz;
-        grid_point(a.grid, a.p0 + (act ? i : r0), px, py, pz);
+        if (a.bm.on) {
+            int64_t ix, iy, iz;
+            act = brick_index(a.bm, act ? i : r0, ix, iy, iz) && act;
+            px = a.grid.x[ix]; py = a.grid.y[iy]; pz = a.grid.z[iz];
+        } else {
+            grid_point(a.grid, a.p0 + (act ? i : r0), px, py, pz);
+        }
         const double X = px - sh.cx, Y = py - sh.cy, Z = pz - sh.cz;
-        const bool near = act && X * X + Y * Y + Z * Z < sh.rcut2;
+        bool near = act && X * X + Y * Y + Z * Z < sh.rcut2;
+        if (a.mask && near) {   // the shell's rows of G exist where its chunk is live
+            const int c = sh.row0 >> 5;
+            near = (a.mask[(i >> 6) * a.nw + (c >> 6)] >> (c & 63)) & 1;
+        }
         if (!__builtin_amdgcn_ballot_w64(near)) continue;   // no point of this wave reaches the shell
         shell_point<L, MP, R>(a, sh, pw0, pend, px, py, pz, act ? i : r0, nfac, near, gr, ga);
     }
@@ -460,6 +538,56 @@ __global__ void k_zero_cols(R* X, int rows, int64_t ld, int64_t c0, int64_t c1)
     X[(i / n) * ld + c0 + i % n] = R(0);
 }
 
+// screened backward helpers (rows in the fused order of the basis, colmap:
+// the basis row of each fused row, -1 for padding rows)
+template <typename R>
+__global__ void k_permute_cols(const R* C, int nmo, int nao, int64_t kpad, const int* colmap, R* out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nmo * kpad) return;
+    const int64_t m = i / kpad, k = i - m * kpad;
+    const int c = colmap[k];
+    out[i] = c >= 0 ? C[m * nao + c] : R(0);
+}
+
+// dC[m][colmap[k]] += dCp[m][k] (each basis column has one fused row: no races)
+template <typename R>
+__global__ void k_unpermute_add(const R* dCp, int nmo, int64_t kpad, const int* colmap, int nao, R* dC)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nmo * kpad) return;
+    const int64_t m = i / kpad, k = i - m * kpad;
+    const int c = colmap[k];
+    if (c >= 0) dC[m * nao + c] = (R)((double)dC[m * nao + c] + (double)dCp[i]);
+}
+
+template <typename R>
+__global__ void k_fill_zero(R* x, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = R(0);
+}
+
+// rows[0 .. nrows) of X [.][ld] zeroed over columns [0, ncol)
+template <typename R>
+__global__ void k_zero_rows_r(R* X, int64_t ld, const int* rows, int nrows, int64_t ncol)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nrows * ncol) return;
+    X[(int64_t)rows[i / ncol] * ld + i % ncol] = R(0);
+}
+
+// dL/drho of each brick column (0 for lanes outside the grid)
+template <typename R>
+__global__ void k_gather_bricks(const R* g, BwdBricks bm, int64_t n, R* out)
+{
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= n) return;
+    int64_t ix, iy, iz;
+    const bool in = brick_index(bm, col, ix, iy, iz);
+    out[col] = in ? g[(ix * bm.ny + iy) * bm.nz + iz] : R(0);
+}
+
 inline size_t pad256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 struct BwdLayout {
@@ -467,14 +595,19 @@ struct BwdLayout {
     int nrange = 0;
     int64_t ncol = 0;
     int nsplit_max = 0;   // point ranges of the dL/dC contraction
+    int krows = 0;        // rows of A and G: the AO count, or the fused rows of the screened path if more
     size_t ct = 0, wc = 0, wct = 0, a = 0, phi = 0, bm = 0, g = 0, part = 0, cols = 0, dcp = 0, total = 0;   // bytes
+    size_t cp = 0, dca = 0, gb = 0, live = 0, lcount = 0, mask = 0;   // screened path
 };
+
+// rows of A and G (the screened path stages them in the fused row order)
+inline int bwd_krows(const OkmiBasis* b) { return std::max(b->nao, b->nkpad); }
 
 // dL/dC partials: up to 512 point ranges, at most 64 MB of them
 constexpr size_t DC_PART_BYTES = (size_t)64 << 20;
 int dc_split_max(const OkmiBasis* b, int nmo, size_t es)
 {
-    const size_t one = (size_t)nmo * b->nao * es;
+    const size_t one = (size_t)nmo * bwd_krows(b) * es;
     return (int)std::max<size_t>(1, std::min<size_t>(512, DC_PART_BYTES / std::max<size_t>(one, 1)));
 }
 
@@ -488,19 +621,31 @@ BwdLayout bwd_layout(const OkmiBasis* b, int nmo, int64_t chunk, size_t es)
     L.chunk = (chunk + 63) & ~(int64_t)63;
     L.nrange = (int)((L.chunk + RANGE - 1) / RANGE);
     L.ncol = b->nprim + 3 * (int64_t)b->nshell;
+    L.krows = bwd_krows(b);
+    const size_t K = (size_t)L.krows;
+    const int64_t ntile = L.chunk / 64;
+    const int nw = b->nchunk > 0 ? live_mask_words(b) : 0;
     size_t o = 0;
-    L.ct = o; o += pad256((size_t)b->nao * nmo * es);
-    L.wc = o; o += pad256(packed_w_elems(nmo, b->nao) * es);     // C packed (Phi = C A)
-    L.wct = o; o += pad256(packed_w_elems(b->nao, nmo) * es);    // C^T packed (G = C^T B)
-    L.a = o; o += pad256((size_t)b->nao * L.chunk * es);
+    L.ct = o; o += pad256(K * nmo * es);
+    L.wc = o; o += pad256(packed_w_elems(nmo, L.krows) * es);     // C packed (Phi = C A)
+    L.wct = o; o += pad256(packed_w_elems(L.krows, nmo) * es);    // C^T packed (G = C^T B)
+    L.a = o; o += pad256(K * L.chunk * es);
     L.phi = o; o += pad256((size_t)nmo * L.chunk * es);
     L.bm = o; o += pad256((size_t)nmo * L.chunk * es);
-    L.g = o; o += pad256((size_t)b->nao * L.chunk * es);
+    L.g = o; o += pad256(K * L.chunk * es);
     L.part = o; o += pad256((size_t)L.nrange * L.ncol * sizeof(double));
     L.cols = o; o += pad256((size_t)L.ncol * sizeof(double));
     // ranges of at least 256 points: a short chunk needs few partials
     L.nsplit_max = (int)std::max<int64_t>(1, std::min<int64_t>(dc_split_max(b, nmo, es), L.chunk / 256));
-    L.dcp = o; o += pad256((size_t)L.nsplit_max * nmo * b->nao * es);
+    L.dcp = o; o += pad256((size_t)L.nsplit_max * nmo * K * es);
+    // screened path: C in the fused row order, dL/dC in that order, dL/drho
+    // in brick order, live-chunk lists, counts and masks per 64-point tile
+    L.cp = o; o += pad256((size_t)nmo * K * es);
+    L.dca = o; o += pad256((size_t)nmo * K * es);
+    L.gb = o; o += pad256((size_t)L.chunk * es);
+    L.live = o; o += pad256((size_t)ntile * std::max(b->nchunk, 1) * sizeof(int));
+    L.lcount = o; o += pad256((size_t)ntile * sizeof(int));
+    L.mask = o; o += pad256((size_t)ntile * std::max(nw, 1) * sizeof(unsigned long long));
     L.total = o;
     return L;
 }
@@ -513,29 +658,40 @@ size_t ws_bytes_of(const OkmiBasis* b, int nmo, int64_t chunk, size_t es)
 // out[M][p] = W[M][K] X[K][p] for the points of [0, npad), W packed by
 // k_pack_w: M tiles in groups of at most 3 (FP64) or 5 (FP32) tiles, so that
 // the accumulators leave room for two waves per SIMD, the group size chosen
-// so that the groups divide the tiles as evenly as possible
+// so that the groups divide the tiles as evenly as possible.  mode 1 / 2:
+// the screened forms of k_gemm_nn (live-chunk lists per 64-point tile).
 template <typename R>
 void launch_gemm_nn(const R* Wp, int M, int K, const R* X, int64_t ldx, int64_t npad, R* out, int64_t ldo,
-                    hipStream_t st)
+                    hipStream_t st, int mode = 0, const int* live = nullptr, const int* lcount = nullptr,
+                    int nchunk = 0)
 {
+    const dim3 grd((unsigned)((npad + 255) / 256)), blk(256);
+#define OKMI_GNN(MG, MODE) \
+    hipLaunchKernelGGL((k_gemm_nn<R, MG, MODE>), grd, blk, 0, st, Wp, M, K, X, ldx, npad, out, ldo, live, lcount, nchunk)
+    if (mode == 2) {
+        OKMI_GNN(2, 2);
+        return;
+    }
     const int nmt = (M + 15) / 16;
     const int gmax = sizeof(R) == 8 ? 3 : 5;
     const int ngroup = (nmt + gmax - 1) / gmax;
     const int mg = (nmt + ngroup - 1) / ngroup;
-    const dim3 grd((unsigned)((npad + 255) / 256)), blk(256);
     switch (mg) {
-    case 1: hipLaunchKernelGGL((k_gemm_nn<R, 1>), grd, blk, 0, st, Wp, M, K, X, ldx, npad, out, ldo); break;
-    case 2: hipLaunchKernelGGL((k_gemm_nn<R, 2>), grd, blk, 0, st, Wp, M, K, X, ldx, npad, out, ldo); break;
-    case 3: hipLaunchKernelGGL((k_gemm_nn<R, 3>), grd, blk, 0, st, Wp, M, K, X, ldx, npad, out, ldo); break;
+    case 1: if (mode) OKMI_GNN(1, 1); else OKMI_GNN(1, 0); break;
+    case 2: if (mode) OKMI_GNN(2, 1); else OKMI_GNN(2, 0); break;
+    case 3: if (mode) OKMI_GNN(3, 1); else OKMI_GNN(3, 0); break;
     case 4:
-        if constexpr (sizeof(R) == 4)
-            hipLaunchKernelGGL((k_gemm_nn<R, 4>), grd, blk, 0, st, Wp, M, K, X, ldx, npad, out, ldo);
+        if constexpr (sizeof(R) == 4) {
+            if (mode) OKMI_GNN(4, 1); else OKMI_GNN(4, 0);
+        }
         break;
     default:
-        if constexpr (sizeof(R) == 4)
-            hipLaunchKernelGGL((k_gemm_nn<R, 5>), grd, blk, 0, st, Wp, M, K, X, ldx, npad, out, ldo);
+        if constexpr (sizeof(R) == 4) {
+            if (mode) OKMI_GNN(5, 1); else OKMI_GNN(5, 0);
+        }
         break;
     }
+#undef OKMI_GNN
 }
 
 template <typename R>
@@ -579,62 +735,122 @@ int launch_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
     double* part = (double*)(w + Lo.part);
     double* cols = (double*)(w + Lo.cols);
     R* dcp = (R*)(w + Lo.dcp);
+    // Screened form for a whole regular grid: points in 4x4x4 bricks, AO rows
+    // in the fused order of the basis (32-row chunks), and per 64-point tile
+    // only the chunks with a shell in reach (the bound of the fused kernels)
+    // are evaluated (A), contracted (Phi = C A, dL/dC = B A^T) and produced
+    // (G = C^T B); the shell gradients read G where the shell's chunk is live.
+    const bool scr = g.regular && p0 == 0 && np == g.nx * g.ny * g.nz && b->nchunk > 0 && b->d_ishells &&
+                     b->nkpad > 0 && b->nkpad <= Lo.krows;
+    const int K = scr ? b->nkpad : b->nao;   // rows of A and G
+    const R* dg = d_grad;
+    R* Cp = (R*)(w + Lo.cp);
+    R* dCa = (R*)(w + Lo.dca);
+    R* gb = (R*)(w + Lo.gb);
+    int* live = (int*)(w + Lo.live);
+    int* lcount = (int*)(w + Lo.lcount);
+    unsigned long long* lmask = (unsigned long long*)(w + Lo.mask);
+    const int nw = scr ? live_mask_words(b) : 0;
+    BwdBricks bm{};
+    int64_t total = np;
+    if (scr) {
+        bm.on = 1;
+        bm.nbx = (g.nx + 3) / 4; bm.nby = (g.ny + 3) / 4; bm.nbz = (g.nz + 3) / 4;
+        bm.nx = g.nx; bm.ny = g.ny; bm.nz = g.nz;
+        total = bm.nbx * bm.nby * bm.nbz * 64;
+    }
     OkmiProfScope prof(name, st);
-    hipLaunchKernelGGL(k_transpose<R>, dim3((unsigned)(((int64_t)nmo * b->nao + 255) / 256)), dim3(256), 0, st, d_c,
-                       nmo, b->nao, CT);
-    hipLaunchKernelGGL(k_pack_w<R>, dim3((unsigned)((packed_w_elems(nmo, b->nao) + 255) / 256)), dim3(256), 0, st,
-                       d_c, nmo, b->nao, WC);
-    hipLaunchKernelGGL(k_pack_w<R>, dim3((unsigned)((packed_w_elems(b->nao, nmo) + 255) / 256)), dim3(256), 0, st,
-                       CT, b->nao, nmo, WCT);
+    const R* Cw = d_c;   // C in the row order of A
+    if (scr) {
+        const int64_t ne = (int64_t)nmo * K;
+        hipLaunchKernelGGL(k_permute_cols<R>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, d_c, nmo, b->nao,
+                           (int64_t)K, (const int*)b->d_colmap, Cp);
+        hipLaunchKernelGGL(k_fill_zero<R>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, dCa, ne);
+        // rows no shell writes (padding of the chunks) are zero in A for good
+        if (b->npadrows > 0)
+            hipLaunchKernelGGL(k_zero_rows_r<R>, dim3((unsigned)(((int64_t)b->npadrows * chunk + 255) / 256)),
+                               dim3(256), 0, st, A, chunk, (const int*)b->d_padrows, b->npadrows, chunk);
+        Cw = Cp;
+    }
+    hipLaunchKernelGGL(k_transpose<R>, dim3((unsigned)(((int64_t)nmo * K + 255) / 256)), dim3(256), 0, st, Cw, nmo,
+                       K, CT);
+    hipLaunchKernelGGL(k_pack_w<R>, dim3((unsigned)((packed_w_elems(nmo, K) + 255) / 256)), dim3(256), 0, st, Cw, nmo,
+                       K, WC);
+    hipLaunchKernelGGL(k_pack_w<R>, dim3((unsigned)((packed_w_elems(K, nmo) + 255) / 256)), dim3(256), 0, st, CT, K,
+                       nmo, WCT);
     const int set0 = 0;
-    for (int64_t c0 = 0; c0 < np; c0 += chunk) {
-        const int64_t n = np - c0 < chunk ? np - c0 : chunk;
+    for (int64_t c0 = 0; c0 < total; c0 += chunk) {
+        const int64_t n = total - c0 < chunk ? total - c0 : chunk;
         const int64_t n16 = (n + 15) & ~(int64_t)15;
         const int64_t n64 = (n + 63) & ~(int64_t)63;
-        int rc = okmi_launch_ao_eval(b, g, p0 + c0, n, &set0, 1, es == 8 ? OKMI_F64 : OKMI_F32, A, chunk, 0, st);
-        if (rc) return rc;
-        if (n16 > n)
-            hipLaunchKernelGGL(k_zero_cols<R>, dim3((unsigned)((b->nao * (n16 - n) + 255) / 256)), dim3(256), 0, st,
-                               A, b->nao, chunk, n, n16);
+        int rc;
+        if (scr) {
+            bm.bk0 = c0 / 64;
+            rc = okmi_launch_live_chunks(b, g, 1, bm.bk0, 0, n, 64, live, lcount, st, lmask);
+            if (rc) return rc;
+            const AoBricks bk{bm.bk0, n / 64, bm.nbx, bm.nby, bm.nbz};
+            const AoLiveMask am{lmask, nw, 0};
+            rc = okmi_launch_ao_eval_ex(b, b->d_ishells, b->d_fprims, g, 0, n, &bk, &set0, 1,
+                                        es == 8 ? OKMI_F64 : OKMI_F32, A, chunk, 0, st, &am);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_gather_bricks<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_grad, bm, n,
+                               gb);
+            dg = gb;
+        } else {
+            rc = okmi_launch_ao_eval(b, g, p0 + c0, n, &set0, 1, es == 8 ? OKMI_F64 : OKMI_F32, A, chunk, 0, st);
+            if (rc) return rc;
+            if (n16 > n)
+                hipLaunchKernelGGL(k_zero_cols<R>, dim3((unsigned)((b->nao * (n16 - n) + 255) / 256)), dim3(256), 0,
+                                   st, A, b->nao, chunk, n, n16);
+            dg = d_grad + c0;
+        }
         {
             OkmiProfScope p1(es == 8 ? "gemm_nn" : "gemm_nn_f32", st);
-            launch_gemm_nn<R>(WC, nmo, b->nao, A, chunk, n64, Phi, chunk, st);
+            launch_gemm_nn<R>(WC, nmo, K, A, chunk, n64, Phi, chunk, st, scr ? 1 : 0, live, lcount, b->nchunk);
         }
-        hipLaunchKernelGGL(k_bmat<R>, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, st, Phi, d_occ, d_grad + c0,
-                           nmo, n, n16, chunk, B);
+        hipLaunchKernelGGL(k_bmat<R>, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, st, Phi, d_occ, dg, nmo, n,
+                           n16, chunk, B);
         // point ranges: about 2048 waves over the chip, at least 256 points
         // per range, at most the partial buffer's ranges
-        const int64_t tiles = (b->nao + 16 * NT - 1) / (16 * NT);
+        const int64_t tiles = (K + 16 * NT - 1) / (16 * NT);
         int64_t nsplit = std::max<int64_t>(1, std::min<int64_t>({(2048 + tiles - 1) / tiles, n16 / 256,
                                                                  (int64_t)Lo.nsplit_max}));
         int64_t krange = ((n16 + nsplit - 1) / nsplit + 15) & ~(int64_t)15;
+        if (scr) krange = (krange + 63) & ~(int64_t)63;   // whole 64-point tiles per range
         nsplit = (n16 + krange - 1) / krange;
         // blocks: (range group of 8, AO tile, XCD); range = group * 8 + XCD
         const unsigned nblk = (unsigned)(((nsplit + 7) / 8) * tiles * 8);
         const int nmt = (nmo + 15) / 16;
+        const unsigned long long* tn_mask = scr ? lmask : nullptr;
+        const int nchunk_tn = b->nchunk;
         {
             OkmiProfScope p2(es == 8 ? "gemm_tn" : "gemm_tn_f32", st);
             if (nmt == 1)
-                hipLaunchKernelGGL((k_gemm_tn<R, 1>), dim3(nblk), dim3(64), 0, st, B, nmo, A, b->nao, n16, chunk,
-                                   krange, (int)nsplit, dcp);
+                hipLaunchKernelGGL((k_gemm_tn<R, 1>), dim3(nblk), dim3(64), 0, st, B, nmo, A, K, n16, chunk,
+                                   krange, (int)nsplit, dcp, tn_mask, nw, nchunk_tn);
             else if (nmt == 2)
-                hipLaunchKernelGGL((k_gemm_tn<R, 2>), dim3(nblk), dim3(64), 0, st, B, nmo, A, b->nao, n16, chunk,
-                                   krange, (int)nsplit, dcp);
+                hipLaunchKernelGGL((k_gemm_tn<R, 2>), dim3(nblk), dim3(64), 0, st, B, nmo, A, K, n16, chunk,
+                                   krange, (int)nsplit, dcp, tn_mask, nw, nchunk_tn);
             else
-                hipLaunchKernelGGL((k_gemm_tn<R, 3>), dim3(nblk), dim3(64), 0, st, B, nmo, A, b->nao, n16, chunk,
-                                   krange, (int)nsplit, dcp);
+                hipLaunchKernelGGL((k_gemm_tn<R, 3>), dim3(nblk), dim3(64), 0, st, B, nmo, A, K, n16, chunk,
+                                   krange, (int)nsplit, dcp, tn_mask, nw, nchunk_tn);
         }
-        const int64_t ndc = (int64_t)nmo * b->nao;
+        const int64_t ndc = (int64_t)nmo * K;
         hipLaunchKernelGGL(k_reduce_dc<R>, dim3((unsigned)((ndc + 255) / 256)), dim3(256), 0, st, dcp, (int)nsplit,
-                           ndc, d_dc);
+                           ndc, scr ? dCa : d_dc);
         {
             OkmiProfScope p1(es == 8 ? "gemm_nn" : "gemm_nn_f32", st);
-            launch_gemm_nn<R>(WCT, b->nao, nmo, B, chunk, n64, G, chunk, st);
+            launch_gemm_nn<R>(WCT, K, nmo, B, chunk, n64, G, chunk, st, scr ? 2 : 0, live, lcount, b->nchunk);
         }
         ShellGradArgs<R> sa{};
-        sa.shells = b->d_shells; sa.prims = b->d_prims; sa.T = b->d_T; sa.grid = g;
+        sa.shells = scr ? b->d_ishells : b->d_shells;
+        sa.prims = scr ? b->d_fprims : b->d_prims;
+        sa.T = b->d_T; sa.grid = g;
         sa.p0 = p0 + c0; sa.np = n; sa.ld = chunk; sa.nprim_total = b->nprim; sa.is_normalized = b->is_normalized;
         sa.G = G; sa.partial = part; sa.ncol = Lo.ncol;
+        sa.bm = bm;
+        sa.mask = scr ? lmask : nullptr;
+        sa.nw = nw;
         const int nrange = (int)((n + RANGE - 1) / RANGE);
         {
             OkmiProfScope p3(es == 8 ? "shell_grad" : "shell_grad_f32", st);
@@ -650,6 +866,11 @@ int launch_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
                            Lo.ncol, cols);
         hipLaunchKernelGGL(k_scatter_grad<R>, dim3((unsigned)((b->nprim + 3 * b->natoms + 255) / 256)), dim3(256), 0,
                            st, cols, b->d_shells, b->nshell, b->nprim, b->natoms, d_dalpha, d_dr);
+    }
+    if (scr) {
+        const int64_t ne = (int64_t)nmo * K;
+        hipLaunchKernelGGL(k_unpermute_add<R>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, dCa, nmo,
+                           (int64_t)K, (const int*)b->d_colmap, b->nao, d_dc);
     }
     return okmi_check_launch(name);
 }

@@ -195,6 +195,33 @@ def test_backward_identities_and_chunking():
   assert rel_err(dg2.cpu().numpy(), geo.grad.cpu().numpy()) < 1e-12
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize('precision,tol', [('f64', 1e-12), ('f32', 2e-6)])
+def test_backward_screened_equals_dense(precision, tol):
+  """A whole regular grid takes the screened backward (points in 4x4x4
+  bricks, AO rows in 32-row chunks, only the chunks in reach of a brick
+  evaluated and contracted); the same points given as a vector grid take
+  the dense one.  The gradients agree to rounding (the dropped terms are
+  below 1e-24).  Side 30: bricks straddling the grid's end."""
+  torch = _gpu()
+  from orbkit_amd import workloads
+  from orbkit_amd.autograd import DensityModel
+  qc, axes, _ = workloads.make('gradient', npts_side=30)
+  print('case screened vs dense backward', precision, flush=True)
+  X, Y, Z = numpy.meshgrid(*axes, indexing='ij')
+  vec = tuple(numpy.ascontiguousarray(v.reshape(-1)) for v in (X, Y, Z))
+  reg = DensityModel(qc, axes, precision=precision)
+  den = DensityModel(qc, vec, is_vector=True, precision=precision)
+  C, alpha, geo = reg.parameters(requires_grad=False)
+  g = torch.randn(reg.npts, dtype=torch.float64, device=C.device,
+                  generator=torch.Generator(device=C.device).manual_seed(3)).to(C.dtype)
+  out_r = reg.vjp(C, alpha, geo, g)
+  out_d = den.vjp(C, alpha, geo, g)
+  for a, b in zip(out_r, out_d):
+    a, b = a.double().cpu().numpy(), b.double().cpu().numpy()
+    assert numpy.max(numpy.abs(a - b)) <= tol * numpy.max(numpy.abs(b))
+
+
 # ------------------------------------------------------------ FP32 backward
 #
 # The FP32 model rounds its parameters (C, exponents, positions) to FP32; the
