@@ -185,6 +185,64 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(const R* __restrict__ Wp, in
     }
 }
 
+// The same product for a short K (at most 4 KS rows, K <= 64: G = C^T B
+// with the MOs as K): the wave loads its 64 points' K rows of X into
+// registers once and then runs every M group from them (the streaming form
+// above re-reads X once per group).  MODE 0: all M tiles in groups of MG;
+// MODE 2: the M tiles of the tile's live chunks (MG = 2).
+template <typename R, int MG, int MODE, int KS>
+__global__ __launch_bounds__(256, 2) void k_gemm_nn_xr(const R* __restrict__ Wp, int M, int K,
+                                                       const R* __restrict__ X, int64_t ldx, int64_t npad,
+                                                       R* __restrict__ out, int64_t ldo, const int* __restrict__ live,
+                                                       const int* __restrict__ lcount, int nchunk)
+{
+    static_assert(MODE != 2 || MG == 2, "MODE 2 computes the two M tiles of one chunk per group");
+    typedef typename Mfma<R>::v4 V;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    const int64_t tile = (int64_t)blockIdx.x * 4 + wave;
+    const int64_t p0 = tile * 64;
+    if (p0 >= npad) return;
+    const R* xp = X + p0 + 4 * r;
+    const int nmt = (M + 15) / 16, nks = (K + 3) / 4;
+    V xr[KS];
+#pragma unroll
+    for (int st = 0; st < KS; ++st)
+        xr[st] = st < nks ? *(const V*)(xp + (int64_t)min(4 * st + q, K - 1) * ldx) : V{0, 0, 0, 0};
+    const int* lst = MODE ? live + tile * nchunk : nullptr;
+    const int ngroup = MODE == 2 ? __builtin_amdgcn_readfirstlane(lcount[tile]) : (nmt + MG - 1) / MG;
+    for (int gi = 0; gi < ngroup; ++gi) {
+        const int g0 = MODE == 2 ? 2 * lst[gi] : gi * MG;
+        const R* wp = Wp + (int64_t)g0 * nks * 64 + lane;
+        V acc[MG][4];
+#pragma unroll
+        for (int mt = 0; mt < MG; ++mt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[mt][j] = V{0, 0, 0, 0};
+#pragma unroll
+        for (int st = 0; st < KS; ++st) {
+            if (st < nks) {
+                R wv[MG];
+#pragma unroll
+                for (int mt = 0; mt < MG; ++mt) wv[mt] = g0 + mt < nmt ? wp[((int64_t)mt * nks + st) * 64] : R(0);
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int mt = 0; mt < MG; ++mt) acc[mt][j] = Mfma<R>::mac(wv[mt], xr[st][j], acc[mt][j]);
+            }
+        }
+#pragma unroll
+        for (int mt = 0; mt < MG; ++mt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = (g0 + mt) * 16 + Mfma<R>::row(i, q);
+                if (m < M)
+                    *(V*)(out + (int64_t)m * ldo + p0 + 4 * r) =
+                        V{acc[mt][0][i], acc[mt][1][i], acc[mt][2][i], acc[mt][3][i]};
+            }
+    }
+}
+
 // dL/dC partials: part[s][m][n] = sum over the points k of range s of
 // B[m][k] A[n][k].  One wave per (point range, tile of 16 NT AOs) computes
 // every MO tile of that AO tile (groups of up to MG tiles), so each AO row is
@@ -666,6 +724,24 @@ void launch_gemm_nn(const R* Wp, int M, int K, const R* X, int64_t ldx, int64_t 
                     int nchunk = 0)
 {
     const dim3 grd((unsigned)((npad + 255) / 256)), blk(256);
+    const int nks = (K + 3) / 4;
+    if (nks <= 16) {   // short K (the MO count): X rows held in registers
+#define OKMI_GXR(MG, MODE, KS) \
+    hipLaunchKernelGGL((k_gemm_nn_xr<R, MG, MODE, KS>), grd, blk, 0, st, Wp, M, K, X, ldx, npad, out, ldo, live, lcount, \
+                       nchunk)
+#define OKMI_GXR_KS(MG, MODE)                      \
+    do {                                           \
+        if (nks <= 4) OKMI_GXR(MG, MODE, 4);       \
+        else if (nks <= 8) OKMI_GXR(MG, MODE, 8);  \
+        else if (nks <= 12) OKMI_GXR(MG, MODE, 12); \
+        else OKMI_GXR(MG, MODE, 16);               \
+    } while (0)
+        if (mode == 2) OKMI_GXR_KS(2, 2);
+        else if (mode == 0) OKMI_GXR_KS(2, 0);
+#undef OKMI_GXR_KS
+#undef OKMI_GXR
+        if (mode != 1) return;
+    }
 #define OKMI_GNN(MG, MODE) \
     hipLaunchKernelGGL((k_gemm_nn<R, MG, MODE>), grd, blk, 0, st, Wp, M, K, X, ldx, npad, out, ldo, live, lcount, nchunk)
     if (mode == 2) {
