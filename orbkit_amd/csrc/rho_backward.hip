@@ -37,6 +37,7 @@ namespace {
 
 constexpr int MAXP = 16;     // primitives per shell handled by one block (more: extra windows)
 constexpr int RANGE = 4096;  // points per shell-gradient block (16 steps of 256: many blocks in flight)
+constexpr int GEN_T_MAX = 15 * 15;   // elements of a general transform block the shell gradient holds in LDS
 
 // 16x16x4 MFMA of each arithmetic.  Item i of lane l of the accumulator
 // holds D[row(i, l/16)][l%16]: FP64 row 4i + l/16, FP32 row 4(l/16) + i.
@@ -426,10 +427,25 @@ struct ShellGradArgs {
 // to the exponent gradients of primitives [pw0, pw0 + MP), accumulated into
 // gr / ga when `near` (the lane's point is within the shell's radius).
 // Every primitive's d e is computed once (kept for those of the window).
+// w_c += sum over the rows r of a standard spherical shell of G_r T[r][c]
+// (the transpose of sph_rows_to: at most three terms per row)
+template <int L, int RW, typename R>
+__device__ __forceinline__ void sph_rows_back(const R* Gi, int64_t ld, R (&w)[ncart(L)])
+{
+    if constexpr (L <= 3 && RW < 2 * L + 1) {
+        constexpr SphRow row = sph_row(L, RW);
+        const R g = Gi[(int64_t)RW * ld];
+        w[row.t[0].c] = fma(g, (R)row.t[0].v, w[row.t[0].c]);
+        if constexpr (row.n > 1) w[row.t[1].c] = fma(g, (R)row.t[1].v, w[row.t[1].c]);
+        if constexpr (row.n > 2) w[row.t[2].c] = fma(g, (R)row.t[2].v, w[row.t[2].c]);
+        sph_rows_back<L, RW + 1, R>(Gi, ld, w);
+    }
+}
+
 template <int L, int MP, typename R>
 __device__ __forceinline__ void shell_point(const ShellGradArgs<R>& a, const ShellRec& sh, int pw0, int pend,
                                             double px, double py, double pz, int64_t i, double nfac, bool near,
-                                            R (&gr)[3], R (&ga)[MP])
+                                            R (&gr)[3], R (&ga)[MP], const double* Ts)
 {
     constexpr int NC = ncart(L);
     const double X = px - sh.cx, Y = py - sh.cy, Z = pz - sh.cz;
@@ -448,15 +464,23 @@ __device__ __forceinline__ void shell_point(const ShellGradArgs<R>& a, const She
         for (int j = 0; j < MP; ++j)
             if (p == pw0 + j) dw[j] = de;
     }
-    // w_c = sum_rows G_row T[row][c]
-    const OKMI_CONST double* T = cptr(a.T) + sh.t0;
+    // w_c = sum_rows G_row T[row][c]: the standard spherical and Cartesian
+    // maps with compile-time coefficients, a general block from its LDS copy
     R w[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) w[k] = 0;
-    for (int rw = 0; rw < sh.nrow; ++rw) {
-        const R g = a.G[(int64_t)(sh.row0 + rw) * a.ld + i];
+    const R* Gi = a.G + (int64_t)sh.row0 * a.ld + i;
+    if (sh.kind == SHELL_T_SPHERICAL) {
+        sph_rows_back<L, 0, R>(Gi, a.ld, w);
+    } else if (sh.kind == SHELL_T_CARTESIAN) {
 #pragma unroll
-        for (int k = 0; k < NC; ++k) w[k] = fma(g, (R)T[rw * NC + k], w[k]);
+        for (int k = 0; k < NC; ++k) w[k] = Gi[(int64_t)k * a.ld] * (R)cart_norm(L, k);
+    } else {
+        for (int rw = 0; rw < sh.nrow; ++rw) {
+            const R g = Gi[(int64_t)rw * a.ld];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) w[k] = fma(g, (R)Ts[rw * NC + k], w[k]);
+        }
     }
     // q = sum_c w_c m_c and, with the lowered monomials u (times S0),
     // sum_c w_c d_a chi_c = R_a S1 q + sum_c w_c e_a(c) u_{c - 1_a}
@@ -488,6 +512,12 @@ template <int L, int MP, typename R>
 __device__ void shell_grad(const ShellGradArgs<R>& a, const ShellRec& sh, int pw0)
 {
     __shared__ double red[4][3 + MP];
+    __shared__ double Ts[GEN_T_MAX];   // a general shell's T block (nrow x ncart), read through LDS
+    if (sh.kind == SHELL_T_GENERAL) {
+        const OKMI_CONST double* T = cptr(a.T) + sh.t0;
+        for (int t = threadIdx.x; t < sh.nrow * ncart(L); t += blockDim.x) Ts[t] = T[t];
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int pend = min(sh.nprim, pw0 + MP);
     const double nfac = a.is_normalized ? 0.0 : (2.0 * L + 3.0) / 4.0;
@@ -515,7 +545,7 @@ __device__ void shell_grad(const ShellGradArgs<R>& a, const ShellRec& sh, int pw
             near = (a.mask[(i >> 6) * a.nw + (c >> 6)] >> (c & 63)) & 1;
         }
         if (!__builtin_amdgcn_ballot_w64(near)) continue;   // no point of this wave reaches the shell
-        shell_point<L, MP, R>(a, sh, pw0, pend, px, py, pz, act ? i : r0, nfac, near, gr, ga);
+        shell_point<L, MP, R>(a, sh, pw0, pend, px, py, pz, act ? i : r0, nfac, near, gr, ga, Ts);
     }
     // block reduction in FP64, in a fixed order (deterministic)
 #pragma unroll
@@ -799,7 +829,15 @@ int launch_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
             return 1;
         }
     int maxp = 1;
-    for (int s = 0; s < b->nshell; ++s) maxp = std::max(maxp, b->h_shells[s].nprim);
+    for (int s = 0; s < b->nshell; ++s) {
+        const ShellRec& sh = b->h_shells[s];
+        maxp = std::max(maxp, sh.nprim);
+        if (sh.kind == SHELL_T_GENERAL && sh.nrow * ncart(sh.l) > GEN_T_MAX) {
+            okmi_set_error("%s: shell %d has a %d x %d transform block (up to %d elements)", name, s, sh.nrow,
+                           ncart(sh.l), GEN_T_MAX);
+            return 1;
+        }
+    }
     char* w = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
     R* CT = (R*)(w + Lo.ct);
     R* WC = (R*)(w + Lo.wc);
