@@ -457,6 +457,9 @@ __device__ __forceinline__ void shell_point(const ShellGradArgs<R>& a, const She
     for (int j = 0; j < MP; ++j) dw[j] = 0;
     for (int p = 0; p < sh.nprim; ++p) {
         const PrimRec pr = load_prim(a.prims, sh.prim0 + p);
+        // a primitive out of reach of every point of the wave adds nothing
+        // above its screening bound
+        if (!__builtin_amdgcn_ballot_w64(rr < pr.rcut2)) continue;
         const R de = (R)pr.d * gauss<R>(pr.alpha * rr);
         S0 += de;
         S1 = fma((R)(-2.0 * pr.alpha), de, S1);
