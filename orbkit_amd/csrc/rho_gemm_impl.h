@@ -206,11 +206,21 @@ __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
     // the 64-column tile holding this block's columns
     const int* lst = nullptr;
     int nsteps = (a.nao + 3) / 4;
+    // the list's first 128 entries are held across the wave's lanes (entry j
+    // in lane j % 64 of lv[j / 64]) and read with readlane, so that a K step
+    // does not wait on a scalar load of its chunk index
+    int lv0 = 0, lv1 = 0;
     if (a.screened) {
         const int64_t tile = wg ? blockIdx.x >> 2 : blockIdx.x;
         lst = a.scr.live + tile * a.scr.nchunk;
-        nsteps = 8 * a.scr.live_count[tile];
+        const int cnt = a.scr.live_count[tile];
+        nsteps = 8 * cnt;
+        lv0 = lane < cnt ? lst[lane] : 0;
+        lv1 = lane + 64 < cnt ? lst[lane + 64] : 0;
     }
+    auto chunk_of = [&](int j) {   // entry j of the list (wave-uniform j)
+        return j < 64 ? __builtin_amdgcn_readlane(lv0, j) : j < 128 ? __builtin_amdgcn_readlane(lv1, j - 64) : lst[j];
+    };
     const int ldc = a.screened ? a.scr.ldc : a.nao;
     const int nmt = (a.nmo + 15) >> 4;
     const int ngroups = (nmt + MG - 1) / MG;
@@ -238,7 +248,7 @@ __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
         // A row of K step ks for this lane, and the C column it pairs with
         // (fused row order: through colmap, -1 = zero row)
         auto row_of = [&](int ks, int& k, bool& kin, int& kcol, bool& cin) {
-            k = (a.screened ? lst[ks >> 3] * 32 + 4 * (ks & 7) : 4 * ks) + kq;
+            k = (a.screened ? chunk_of(ks >> 3) * 32 + 4 * (ks & 7) : 4 * ks) + kq;
             kin = k < a.nao;
             kcol = k;
             cin = kin;
