@@ -134,12 +134,18 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(const R* __restrict__ Wp, in
     const int nmt = (M + 15) / 16, nks = (K + 3) / 4;
     const int* lst = MODE ? live + tile * nchunk : nullptr;
     const int nlive = MODE ? __builtin_amdgcn_readfirstlane(lcount[tile]) : 0;
+    // the list's first 128 entries held across the lanes, read with readlane
+    // (no scalar load for a step to wait on)
+    const int lv0 = MODE && lane < nlive ? lst[lane] : 0, lv1 = MODE && lane + 64 < nlive ? lst[lane + 64] : 0;
+    auto entry = [&](int j) {
+        return j < 64 ? __builtin_amdgcn_readlane(lv0, j) : j < 128 ? __builtin_amdgcn_readlane(lv1, j - 64) : lst[j];
+    };
     // K steps of this wave and the packed step index of step st
     const int nst = MODE == 1 ? 8 * nlive : nks;
-    auto kstep = [&](int st) { return MODE == 1 ? lst[st >> 3] * 8 + (st & 7) : st; };
+    auto kstep = [&](int st) { return MODE == 1 ? entry(st >> 3) * 8 + (st & 7) : st; };
     const int ngroup = MODE == 2 ? nlive : (nmt + MG - 1) / MG;
     for (int gi = 0; gi < ngroup; ++gi) {
-        const int g0 = MODE == 2 ? 2 * lst[gi] : gi * MG;   // first M tile of the group
+        const int g0 = MODE == 2 ? 2 * entry(gi) : gi * MG;   // first M tile of the group
         const R* wp = Wp + (int64_t)g0 * nks * 64 + lane;
         V xr[D];
         R wr[D][MG];

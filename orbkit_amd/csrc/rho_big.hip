@@ -103,8 +103,16 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
     const int g = lane >> 4, n16 = lane & 15;
     // the K steps of this point tile: the 32-row chunks with a shell that
     // reaches one of its points (k_big_live), in row order
-    const int* lst = a.live + tn * a.nchunk;
+    const int* lst_mem = a.live + tn * a.nchunk;
     const int cnt = a.live_count[tn];
+    // the list's first 128 entries held across the lanes (entry j in lane
+    // j % 64 of lv[j / 64]) and read with readlane: a step's chunk index is
+    // not a scalar load the step waits on
+    const int lv0 = lane < cnt ? lst_mem[lane] : 0, lv1 = lane + 64 < cnt ? lst_mem[lane + 64] : 0;
+    auto lst = [&](int j) {
+        return j < 64 ? __builtin_amdgcn_readlane(lv0, j) : j < 128 ? __builtin_amdgcn_readlane(lv1, j - 64)
+                                                                    : lst_mem[j];
+    };
     // 16-MO tiles of this wave that hold MOs (wave-uniform): the last MO
     // tile group of a padded count skips the MFMAs (and the AO split) of
     // its padding tiles, e.g. 400 MOs = 25 tiles of the 32 launched
@@ -234,13 +242,13 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
     // 125 -> 137 ms, so S = 4 keeps one stage).
     if (cnt > 0) {
         Stage R0, R1;
-        gload(lst[0], R0);
+        gload(lst(0), R0);
         sstore(0, R0);
         if constexpr (S >= 7) {
-            if (cnt > 1) gload(lst[1], R1);
+            if (cnt > 1) gload(lst(1), R1);
             __syncthreads();
             auto body = [&](int i, int buf, Stage& Rnext, Stage& Rfree) {
-                if (i + 2 < cnt) gload(lst[i + 2], Rfree);
+                if (i + 2 < cnt) gload(lst(i + 2), Rfree);
                 compute(buf);
                 if (i + 1 < cnt) sstore(buf ^ 1, Rnext);
                 __syncthreads();
@@ -253,7 +261,7 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
             __syncthreads();
             int buf = 0;
             for (int i = 0; i < cnt; ++i) {
-                if (i + 1 < cnt) gload(lst[i + 1], R0);
+                if (i + 1 < cnt) gload(lst(i + 1), R0);
                 compute(buf);
                 if (i + 1 < cnt) sstore(buf ^ 1, R0);
                 __syncthreads();
