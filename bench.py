@@ -183,15 +183,17 @@ def timed(step, args, L, names, sync):
 
 # ---------------------------------------------------------------- workloads
 
-def screened_rows(L, basis, axes):
-  """Sum over the grid's 4x4x4 bricks of (AO rows of the chunks the fused
-  kernels contract there) x (points in the brick): okmi_fused_live_rows."""
+def screened_rows(L, basis, axes, precision='f64'):
+  """Sum over the grid's 4x4x4 bricks of (AO rows of the chunks the kernels
+  contract there) x (points in the brick): okmi_fused_live_rows, with the
+  screening radii of the precision ('f64', or one whose AO values end in
+  FP32)."""
   from orbkit_amd import _native
   ax = [numpy.ascontiguousarray(a, dtype=numpy.float64) for a in axes]
   P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
   out = ctypes.c_double()
   _native.check(L.okmi_fused_live_rows(basis.handle, P(ax[0]), len(ax[0]), P(ax[1]), len(ax[1]), P(ax[2]), len(ax[2]),
-                                       ctypes.byref(out)), 'okmi_fused_live_rows')
+                                       _native.PREC[precision], ctypes.byref(out)), 'okmi_fused_live_rows')
   return out.value
 
 
@@ -251,7 +253,7 @@ def density_leg(args, torch, info, basis, dg, C, occ, codes, path, axes=None):
     # the screened contraction (every path: fused, many-MO and staged): per
     # brick only the AO chunks with a shell in reach (the work the kernel
     # does besides MO-tile padding)
-    sflops = 2.0 * nmo * nsets * screened_rows(L, basis, axes)
+    sflops = 2.0 * nmo * nsets * screened_rows(L, basis, axes, 'f64' if path == 'f64_staged' else path)
     roof['screened'] = {'flop_per_step': sflops, 'achieved': sflops / t_launch / 1e12,
                         'frac': sflops / t_launch / 1e12 / peak, 'share_of_algorithmic': sflops / flops,
                         'note': 'algorithmic flops count every AO at every point; the kernel skips the AO chunks '
@@ -412,7 +414,7 @@ def batch_step(args, torch):
           'kernel': 'k_rho_batch', 'kernel_ms_per_step': t * 1e3, 'algorithmic_flop_per_step': flops,
           'tuned': {'tile_shape': tune['shape'], 'workgroups': tune['workgroups']}}
   # the screened contraction (AO chunks in reach of each brick), per molecule
-  sflops = sum(2.0 * c.shape[0] * 4 * screened_rows(L, b, axes)
+  sflops = sum(2.0 * c.shape[0] * 4 * screened_rows(L, b, axes, args.precision)
                for c, b, (_, axes) in zip(plan.C, plan.bases, mols))
   roof['screened'] = {'flop_per_step': sflops, 'achieved': sflops / t / 1e12, 'frac': sflops / t / 1e12 / peak,
                       'share_of_algorithmic': sflops / flops}

@@ -356,16 +356,19 @@ int okmi_profile_enable(int on);
 int okmi_profile_query(const char* kernel, int* count, double* total_ms);
 
 /*
- * Work measure for measurement tools: the fused kernels (and the many-MO
- * path) contract, for each 4x4x4 brick of a regular grid, only the 32-row
- * AO chunks with a shell whose screening radius reaches the brick.  For the
- * regular grid with host axes x, y, z this gives *rows_points = sum over
- * bricks of (AO rows of its live chunks) x (grid points in the brick); the
- * screened contraction's flops are 2 * NMO * sets * *rows_points.
+ * Work measure for measurement tools: the fused kernels (and the many-MO and
+ * staged paths) contract, for each 4x4x4 brick of a regular grid, only the
+ * 32-row AO chunks with a shell whose screening radius reaches the brick.
+ * For the regular grid with host axes x, y, z this gives *rows_points = sum
+ * over bricks of (AO rows of its live chunks) x (grid points in the brick);
+ * the screened contraction's flops are 2 * NMO * sets * *rows_points.
+ * prec: OKMI_PREC_F64 for the FP64 paths' radii (terms below 1e-24), any
+ * other precision for the radii of the paths whose AO values end in FP32
+ * (terms below 1e-14).
  * No counterpart in the reference (orbkit evaluates every AO everywhere).
  */
 int okmi_fused_live_rows(const okmi_basis* b, const double* x, int64_t nx, const double* y, int64_t ny,
-                         const double* z, int64_t nz, double* rows_points);
+                         const double* z, int64_t nz, int prec, double* rows_points);
 
 #ifdef __cplusplus
 }

@@ -46,7 +46,7 @@ SIGNATURES = {
                                  c_vp, c_vp, c_i64, c_vp, c_vp]),
     'okmi_profile_enable': (c_int, [c_int]),
     'okmi_fused_live_rows': (c_int, [c_vp, ctypes.POINTER(c_dbl), c_i64, ctypes.POINTER(c_dbl), c_i64,
-                                     ctypes.POINTER(c_dbl), c_i64, ctypes.POINTER(c_dbl)]),
+                                     ctypes.POINTER(c_dbl), c_i64, c_int, ctypes.POINTER(c_dbl)]),
     'okmi_profile_query': (c_int, [ctypes.c_char_p, ctypes.POINTER(c_int), ctypes.POINTER(c_dbl)]),
     'okmi_rho_backward_workspace_bytes': (c_sz, [c_vp, c_int, c_i64]),
     'okmi_rho_backward': (c_int, [c_vp, ctypes.POINTER(OkmiGrid), c_i64, c_i64, c_vp, c_vp, c_int, c_vp, c_vp, c_vp,

@@ -228,7 +228,7 @@ extern "C" int okmi_basis_create(int ncont, const int* ncart_per_cont, const int
     *out = nullptr;
     std::vector<ShellRec> shells(ncont);
     std::vector<PrimRec> prims;
-    std::vector<double> T, craw, frc2, fprc2;
+    std::vector<double> T, craw, frc2, fprc2, frc2s, fprc2s;
     std::vector<int> pl;
     const double pref = std::pow(2.0 / M_PI, 0.75);
     int crow = 0, prow = 0, orow = 0, tmat = 0, max_l = 0;
@@ -302,11 +302,15 @@ extern "C" int okmi_basis_create(int ncont, const int* ncart_per_cont, const int
         s.rcut2 = rc >= 1e150 ? 1e300 : rc * rc;
         const double rf = shell_rcut(l, al.data(), dd.data(), np, OKMI_SCREEN_FUSED);
         frc2.push_back(rf >= 1e150 ? 1e300 : rf * rf);
+        const double rs = shell_rcut(l, al.data(), dd.data(), np, OKMI_SCREEN_SPLIT);
+        frc2s.push_back(rs >= 1e150 ? 1e300 : rs * rs);
         for (int p = 0; p < np; ++p) {
             const double rp = shell_rcut(l, &al[p], &dd[p], 1, OKMI_SCREEN);
             prims[prims.size() - np + p].rcut2 = rp >= 1e150 ? 1e300 : rp * rp;
             const double rq = shell_rcut(l, &al[p], &dd[p], 1, OKMI_SCREEN_FUSED);
             fprc2.push_back(rq >= 1e150 ? 1e300 : rq * rq);
+            const double rr = shell_rcut(l, &al[p], &dd[p], 1, OKMI_SCREEN_SPLIT);
+            fprc2s.push_back(rr >= 1e150 ? 1e300 : rr * rr);
         }
         if (l > max_l) max_l = l;
         crow += nc; prow += np; orow += nrow;
@@ -327,6 +331,8 @@ extern "C" int okmi_basis_create(int ncont, const int* ncart_per_cont, const int
     std::memcpy(b->h_prim_l, pl.data(), sizeof(int) * prow);
     b->h_frcut2.assign(frc2.begin(), frc2.end());
     b->h_fprcut2.assign(fprc2.begin(), fprc2.end());
+    b->h_frcut2s.assign(frc2s.begin(), frc2s.end());
+    b->h_fprcut2s.assign(fprc2s.begin(), fprc2s.end());
     int rc = dev_upload(&b->d_shells, shells.data(), shells.size(), "basis_create: shells");
     if (!rc) rc = dev_upload(&b->d_prims, prims.data(), prims.size(), "basis_create: primitives");
     if (!rc) rc = dev_upload(&b->d_T, T.data(), T.size(), "basis_create: transform");
@@ -355,6 +361,9 @@ extern "C" int okmi_basis_destroy(okmi_basis* b)
     if (b->d_oshells) (void)hipFree(b->d_oshells);
     if (b->d_ishells) (void)hipFree(b->d_ishells);
     if (b->d_wshells) (void)hipFree(b->d_wshells);
+    for (void* p : {(void*)b->d_fshells_s, (void*)b->d_ishells_s, (void*)b->d_wshells_s, (void*)b->d_fprims_s,
+                    (void*)b->d_fgroups_s})
+        if (p) (void)hipFree(p);
     if (b->d_padrows) (void)hipFree(b->d_padrows);
     if (b->d_colmap) (void)hipFree(b->d_colmap);
     delete[] b->h_shells;
@@ -564,7 +573,7 @@ extern "C" int okmi_rho_staged(const okmi_basis* b, const okmi_grid* g, int64_t 
     scr.colmap = nullptr; scr.live = live; scr.live_count = live_count; scr.nchunk = b->nchunk;
     scr.ldc = (int)rows; scr.brick = brick ? 1 : 0;
     scr.nbx = nbx; scr.nby = nby; scr.nbz = nbz; scr.nx = g->nx; scr.ny = g->ny; scr.nz = g->nz;
-    const AoLiveMask am{lmask, nw, 0};
+    const AoLiveMask am{lmask, nw, 0, nullptr};
     for (int64_t c0 = 0; c0 < total; c0 += ch) {
         const int64_t n = total - c0 < ch ? total - c0 : ch;
         // live chunks first: the AO kernel leaves the rows of dead chunks unwritten

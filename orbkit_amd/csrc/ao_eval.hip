@@ -252,7 +252,7 @@ int okmi_launch_ao_eval_ex(const OkmiBasis* b, const ShellRec* shells, const Pri
         a.mask_words = live->words;
         a.mask_shift = live->shift;
         a.chunks = b->d_chunks;
-        a.wshells = b->d_wshells;
+        a.wshells = live->wshells ? live->wshells : b->d_wshells;
     }
     a.shells = shells;
     a.prims = prims;

@@ -38,12 +38,20 @@ struct OkmiBasis {
     int npadrows = 0;
     PrimRec* d_fprims = nullptr;     // rcut2 = fused radius
     PrimGroup* d_fgroups = nullptr;  // fused-kernel primitives, groups of 4 per shell in work order
+    // the same tables with the radii of the bound OKMI_SCREEN_SPLIT (the
+    // paths whose AO values end in FP32; same chunks, rows and work order)
+    ShellRec* d_fshells_s = nullptr;
+    ShellRec* d_ishells_s = nullptr;
+    ShellRec* d_wshells_s = nullptr;
+    PrimRec* d_fprims_s = nullptr;
+    PrimGroup* d_fgroups_s = nullptr;
     int* d_colmap = nullptr;         // [nkpad]
     int nchunk = 0;
     int nkpad = 0;
-    std::vector<double> h_frcut2, h_fprcut2;
-    std::vector<int2> h_chunks;          // host copies of d_chunks and d_fshells
-    std::vector<ShellRec> h_fshells;
+    std::vector<double> h_frcut2, h_fprcut2;     // radii (squared) for OKMI_SCREEN_FUSED
+    std::vector<double> h_frcut2s, h_fprcut2s;   // and for OKMI_SCREEN_SPLIT
+    std::vector<int2> h_chunks;          // host copies of d_chunks, d_fshells and d_fshells_s
+    std::vector<ShellRec> h_fshells, h_fshells_s;
     int is_normalized = 0;
     int max_nrow = 0;      // most output rows of one shell (the fused kernels take up to 32)
 };
@@ -69,6 +77,7 @@ struct AoLiveMask {
     const unsigned long long* mask;
     int words;
     int shift;
+    const ShellRec* wshells;   // work-order shells of the screening bound in use (null: d_wshells)
 };
 int okmi_launch_ao_eval_ex(const OkmiBasis* b, const ShellRec* shells, const PrimRec* prims, const GridDesc& g,
                            int64_t p0, int64_t np, const AoBricks* bricks, const int* drv, int ndrv, int out_dtype,

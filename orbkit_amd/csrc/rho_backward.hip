@@ -912,7 +912,7 @@ int launch_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
             rc = okmi_launch_live_chunks(b, g, 1, bm.bk0, 0, n, 64, live, lcount, st, lmask);
             if (rc) return rc;
             const AoBricks bk{bm.bk0, n / 64, bm.nbx, bm.nby, bm.nbz};
-            const AoLiveMask am{lmask, nw, 0};
+            const AoLiveMask am{lmask, nw, 0, nullptr};
             rc = okmi_launch_ao_eval_ex(b, b->d_ishells, b->d_fprims, g, 0, n, &bk, &set0, 1,
                                         es == 8 ? OKMI_F64 : OKMI_F32, A, chunk, 0, st, &am);
             if (rc) return rc;

@@ -641,7 +641,8 @@ int okmi_launch_rho_big(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np,
     OutCodes oc{};
     oc.n = nout;
     for (int r = 0; r < nout; ++r) oc.c[r] = drv_out[r];
-    const AoLiveMask am{lmask, nw, mnp == 128 ? 1 : 0};
+    // the AO values end in FP32: the screening bound of that arithmetic
+    const AoLiveMask am{lmask, nw, mnp == 128 ? 1 : 0, b->d_wshells_s};
     for (int64_t c0 = 0; c0 < total; c0 += chunk) {
         const int64_t n = total - c0 < chunk ? total - c0 : chunk;
         int rc;
@@ -652,7 +653,7 @@ int okmi_launch_rho_big(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np,
         const int64_t ntile = (n + NP - 1) / NP;
         LiveArgs la{};
         la.grid = g; la.bm = bm; la.p0 = p0 + c0; la.n = n; la.np = NP;
-        la.chunks = b->d_chunks; la.shells = b->d_fshells; la.nchunk = b->nchunk;
+        la.chunks = b->d_chunks; la.shells = b->d_fshells_s; la.nchunk = b->nchunk;
         la.list = live; la.count = live_count;
         la.words = nw;
         if (NP == mnp) la.mask = lmask;
@@ -664,10 +665,10 @@ int okmi_launch_rho_big(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t np,
         }
         if (bm.on) {
             AoBricks bk{c0 / 64, n / 64, bm.nbx, bm.nby, bm.nbz};
-            rc = okmi_launch_ao_eval_ex(b, b->d_ishells, b->d_fprims, g, 0, n, &bk, codes, S, OKMI_F32, ao, chunk,
+            rc = okmi_launch_ao_eval_ex(b, b->d_ishells_s, b->d_fprims_s, g, 0, n, &bk, codes, S, OKMI_F32, ao, chunk,
                                         (int64_t)L.kpad * chunk, st, &am);
         } else {
-            rc = okmi_launch_ao_eval_ex(b, b->d_ishells, b->d_fprims, g, p0 + c0, n, nullptr, codes, S, OKMI_F32, ao,
+            rc = okmi_launch_ao_eval_ex(b, b->d_ishells_s, b->d_fprims_s, g, p0 + c0, n, nullptr, codes, S, OKMI_F32, ao,
                                         chunk, (int64_t)L.kpad * chunk, st, &am);
         }
         if (rc) return rc;

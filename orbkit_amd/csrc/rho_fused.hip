@@ -154,49 +154,62 @@ int prepare_fused_tables(OkmiBasis* b)
         for (auto& cs : cost[c]) work.push_back(cs.second);
     }
     // shells in work-list order (entry j of the list is record j), each
-    // with its primitives in groups of 4 (prim0: first group)
-    std::vector<ShellRec> fs(work.size());
-    std::vector<PrimGroup> groups;
+    // with its primitives in groups of 4 (prim0: first group); one set of
+    // tables per screening bound (radii frc2 / fprc2)
     std::vector<int> colmap(nkpad, -1);
-    for (size_t j = 0; j < work.size(); ++j) {
-        const int i = work[j];
-        fs[j] = b->h_shells[i];
-        for (int r = 0; r < fs[j].nrow; ++r) colmap[krow[i] + r] = fs[j].row0 + r;
-        fs[j].row0 = krow[i];
-        fs[j].rcut2 = b->h_frcut2[i];
-        fs[j].prim0 = (int)groups.size();
-        const ShellRec& hs = b->h_shells[i];
-        // primitives by screening radius, largest first (ties keep file order)
-        std::vector<int> order(hs.nprim);
-        for (int k = 0; k < hs.nprim; ++k) order[k] = hs.prim0 + k;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int x, int y) { return b->h_fprcut2[x] > b->h_fprcut2[y]; });
-        for (int q = 0; q < hs.nprim; q += 4) {
-            PrimGroup pg;
-            for (int k = 0; k < 4; ++k) {
-                const bool used = q + k < hs.nprim;
-                const int pi = used ? order[q + k] : 0;
-                pg.alpha[k] = used ? b->h_prims[pi].alpha : 0.0;
-                pg.d[k] = used ? b->h_prims[pi].d : 0.0;
-                pg.rcut2[k] = used ? b->h_fprcut2[pi] : -1.0;
+    for (size_t j = 0; j < work.size(); ++j)
+        for (int r = 0; r < b->h_shells[work[j]].nrow; ++r) colmap[krow[work[j]] + r] = b->h_shells[work[j]].row0 + r;
+    auto tables = [&](const std::vector<double>& frc2, const std::vector<double>& fprc2, std::vector<ShellRec>& fs,
+                      std::vector<PrimGroup>& groups, std::vector<ShellRec>& is, std::vector<ShellRec>& wsh,
+                      std::vector<PrimRec>& fp) {
+        fs.assign(work.size(), ShellRec{});
+        groups.clear();
+        for (size_t j = 0; j < work.size(); ++j) {
+            const int i = work[j];
+            fs[j] = b->h_shells[i];
+            fs[j].row0 = krow[i];
+            fs[j].rcut2 = frc2[i];
+            fs[j].prim0 = (int)groups.size();
+            const ShellRec& hs = b->h_shells[i];
+            // primitives by screening radius, largest first (ties keep file order)
+            std::vector<int> order(hs.nprim);
+            for (int k = 0; k < hs.nprim; ++k) order[k] = hs.prim0 + k;
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return fprc2[x] > fprc2[y]; });
+            for (int q = 0; q < hs.nprim; q += 4) {
+                PrimGroup pg;
+                for (int k = 0; k < 4; ++k) {
+                    const bool used = q + k < hs.nprim;
+                    const int pi = used ? order[q + k] : 0;
+                    pg.alpha[k] = used ? b->h_prims[pi].alpha : 0.0;
+                    pg.d[k] = used ? b->h_prims[pi].d : 0.0;
+                    pg.rcut2[k] = used ? fprc2[pi] : -1.0;
+                }
+                groups.push_back(pg);
             }
-            groups.push_back(pg);
         }
-    }
+        // the same shells writing their internal rows (the many-MO and
+        // staged paths stage the AO sets in the fused row order), also in
+        // work order, and the primitives with these radii
+        is.assign(b->h_shells, b->h_shells + b->nshell);
+        for (int i = 0; i < b->nshell; ++i) {
+            is[i].rcut2 = frc2[i];
+            is[i].row0 = krow[i];
+        }
+        wsh.resize(work.size());
+        for (size_t j = 0; j < work.size(); ++j) wsh[j] = is[work[j]];
+        fp.assign(b->h_prims, b->h_prims + b->nprim);
+        for (int i = 0; i < b->nprim; ++i) fp[i].rcut2 = fprc2[i];
+    };
+    std::vector<ShellRec> fs, is, wsh, fs_s, is_s, wsh_s;
+    std::vector<PrimGroup> groups, groups_s;
+    std::vector<PrimRec> fp, fp_s;
+    tables(b->h_frcut2, b->h_fprcut2, fs, groups, is, wsh, fp);
+    tables(b->h_frcut2s, b->h_fprcut2s, fs_s, groups_s, is_s, wsh_s, fp_s);
     std::vector<ShellRec> os(b->h_shells, b->h_shells + b->nshell);
     for (int i = 0; i < b->nshell; ++i) os[i].rcut2 = b->h_frcut2[i];
-    // the same shells writing their internal rows (the many-MO path stages
-    // the AO sets in the fused row order), and the internal rows no shell
-    // writes (zeroed by that path)
-    std::vector<ShellRec> is(os);
-    for (int i = 0; i < b->nshell; ++i) is[i].row0 = krow[i];
-    std::vector<ShellRec> wsh(work.size());
-    for (size_t j = 0; j < work.size(); ++j) wsh[j] = is[work[j]];
     std::vector<int> padrows;
     for (int r = 0; r < nkpad; ++r)
         if (colmap[r] < 0) padrows.push_back(r);
-    std::vector<PrimRec> fp(b->h_prims, b->h_prims + b->nprim);
-    for (int i = 0; i < b->nprim; ++i) fp[i].rcut2 = b->h_fprcut2[i];
     int rc = upload(&b->d_chunks, ch, "rho_fused: chunk map");
     if (!rc) rc = upload(&b->d_fshells, fs, "rho_fused: shells");
     if (!rc) rc = upload(&b->d_fprims, fp, "rho_fused: primitives");
@@ -206,12 +219,18 @@ int prepare_fused_tables(OkmiBasis* b)
     if (!rc) rc = upload(&b->d_wshells, wsh, "rho_fused: work-order shells");
     if (!rc) rc = upload(&b->d_padrows, padrows, "rho_fused: padding rows");
     if (!rc) rc = upload(&b->d_colmap, colmap, "rho_fused: row map");
+    if (!rc) rc = upload(&b->d_fshells_s, fs_s, "rho_fused: shells (FP32 bound)");
+    if (!rc) rc = upload(&b->d_fgroups_s, groups_s, "rho_fused: primitive groups (FP32 bound)");
+    if (!rc) rc = upload(&b->d_ishells_s, is_s, "rho_fused: internal-row shells (FP32 bound)");
+    if (!rc) rc = upload(&b->d_wshells_s, wsh_s, "rho_fused: work-order shells (FP32 bound)");
+    if (!rc) rc = upload(&b->d_fprims_s, fp_s, "rho_fused: primitives (FP32 bound)");
     if (rc) return rc;
     b->nchunk = nchunk;
     b->nkpad = nkpad;
     b->npadrows = (int)padrows.size();
     b->h_chunks = ch;
     b->h_fshells = fs;
+    b->h_fshells_s = fs_s;
     return 0;
 }
 }  // namespace
@@ -304,7 +323,11 @@ int okmi_launch_rho_fused(OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
     for (int i = 0; i < b->nshell; ++i) lm = std::max(lm, b->h_shells[i].l);
 
     FusedArgs a{};
-    a.shells = b->d_fshells; a.groups = b->d_fgroups; a.T = b->d_T; a.chunks = b->d_chunks;
+    // FP64 screens with the FP64 bound, the paths ending in FP32 with theirs
+    const bool split = prec != OKMI_PREC_F64;
+    a.shells = split ? b->d_fshells_s : b->d_fshells;
+    a.groups = split ? b->d_fgroups_s : b->d_fgroups;
+    a.T = b->d_T; a.chunks = b->d_chunks;
     a.grid = g; a.nao = b->nao; a.nchunk = b->nchunk;
     a.C = cpl; a.kpad = kpad; a.mpad = mpad; a.occ = occp;
     a.term = term; a.term2 = term2; a.ld_term = chunk; a.need_mask = need;
@@ -601,7 +624,10 @@ int okmi_batch_build(int nmol, OkmiBasis* const* bases, const GridDesc* grids, c
         const OkmiBasis* bs = bases[m];
         const GridDesc& g = grids[m];
         FusedArgs a{};
-        a.shells = bs->d_fshells; a.groups = bs->d_fgroups; a.T = bs->d_T;
+        const bool split = B->prec != OKMI_PREC_F64;   // the screening bound of the arithmetic
+        a.shells = split ? bs->d_fshells_s : bs->d_fshells;
+        a.groups = split ? bs->d_fgroups_s : bs->d_fgroups;
+        a.T = bs->d_T;
         a.chunks = bs->d_chunks;
         a.grid = g; a.p0 = 0; a.np = npts[m];
         a.nao = bs->nao; a.nchunk = bs->nchunk;
@@ -716,7 +742,7 @@ int64_t okmi_batch_ntile(const OkmiBatch* B) { return B->ntile; }
 int okmi_prepare_fused(OkmiBasis* b) { return prepare_fused_tables(b); }
 
 extern "C" int okmi_fused_live_rows(const okmi_basis* b, const double* x, int64_t nx, const double* y, int64_t ny,
-                                    const double* z, int64_t nz, double* rows_points)
+                                    const double* z, int64_t nz, int prec, double* rows_points)
 {
     if (!b || !x || !y || !z || nx <= 0 || ny <= 0 || nz <= 0 || !rows_points) {
         okmi_set_error("fused_live_rows: invalid arguments");
@@ -755,7 +781,7 @@ extern "C" int okmi_fused_live_rows(const okmi_basis* b, const double* x, int64_
                     const int2 cw = b->h_chunks[c];
                     const int nsh = cw.y & 0xffff;
                     for (int e = 0; e < nsh; ++e) {
-                        const ShellRec& sh = b->h_fshells[cw.x + e];
+                        const ShellRec& sh = (prec == OKMI_PREC_F64 ? b->h_fshells : b->h_fshells_s)[cw.x + e];
                         const double cc[3] = {sh.cx, sh.cy, sh.cz};
                         double d2 = 0.0;
                         for (int d = 0; d < 3; ++d) {

@@ -538,8 +538,9 @@ def test_electron_count(ok):
 def test_fused_live_rows(ok):
   '''okmi_fused_live_rows (the screened-work measure of bench.py): within
   [0, padded rows x points], zero on a grid far from every atom, additive
-  over bricks (a grid split at a brick boundary), and on the medium grid
-  between a third and all of the dense count.'''
+  over bricks (a grid split at a brick boundary), on the medium grid between
+  a third and all of the dense count for the FP64 radii, and no more with
+  the smaller radii of the paths whose AO values end in FP32.'''
   import ctypes
   from orbkit_amd import _native
   from orbkit_amd.basis import device_basis
@@ -547,12 +548,12 @@ def test_fused_live_rows(ok):
   b = device_basis(qc.geo_spec, qc.ao_spec)
   L = _native.lib()
 
-  def rows(x, y, z):
+  def rows(x, y, z, prec='f64'):
     ax = [numpy.ascontiguousarray(a, dtype=numpy.float64) for a in (x, y, z)]
     P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     out = ctypes.c_double()
     _native.check(L.okmi_fused_live_rows(b.handle, P(ax[0]), len(ax[0]), P(ax[1]), len(ax[1]), P(ax[2]), len(ax[2]),
-                                         ctypes.byref(out)), 'okmi_fused_live_rows')
+                                         _native.PREC[prec], ctypes.byref(out)), 'okmi_fused_live_rows')
     return out.value
   x, y, z = axes
   npts = len(x) * len(y) * len(z)
@@ -560,6 +561,9 @@ def test_fused_live_rows(ok):
   assert 0.33 * b.nao * npts < full <= 416 * npts
   assert rows(x + 1e4, y, z) == 0.0
   assert rows(x[:12], y, z) + rows(x[12:], y, z) == full
+  split = rows(x, y, z, 'bf16x6')
+  assert 0 < split <= full and split == rows(x, y, z, 'f32')
+  print('live rows x points: FP64 radii %.4g, FP32 radii %.4g' % (full, split))
 
 
 def test_okmi_rho_staged_subrange(ok):
