@@ -364,7 +364,7 @@ int okmi_profile_query(const char* kernel, int* count, double* total_ms);
  * the screened contraction's flops are 2 * NMO * sets * *rows_points.
  * prec: OKMI_PREC_F64 for the FP64 paths' radii (terms below 1e-24), any
  * other precision for the radii of the paths whose AO values end in FP32
- * (terms below 1e-14).
+ * (terms below 1e-12).
  * No counterpart in the reference (orbkit evaluates every AO everywhere).
  */
 int okmi_fused_live_rows(const okmi_basis* b, const double* x, int64_t nx, const double* y, int64_t ny,

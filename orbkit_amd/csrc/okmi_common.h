@@ -82,12 +82,12 @@ __device__ __forceinline__ void grid_point(const GridDesc& g, int64_t p, double&
 // kernel skips terms below 1e-40; the FP64 paths that screen per brick (the
 // fused FP64 kernel, the staged FP64 path, the backward) below 1e-24; the
 // paths whose AO values end in FP32 (FP32 and bf16-split fused kernels, the
-// many-MO and batch paths) below 1e-14: at any point where the density
+// many-MO and batch paths) below 1e-12: at any point where the density
 // exceeds 1e-10 of its maximum some AO value there is above about 1e-5, and
-// terms below 1e-14 stay under its FP32 rounding (2^-24 relative).
+// terms below 1e-12 stay at the FP32 rounding of such a value (2^-24 * 1e-5).
 #define OKMI_SCREEN 1e-40
 #define OKMI_SCREEN_FUSED 1e-24
-#define OKMI_SCREEN_SPLIT 1e-14
+#define OKMI_SCREEN_SPLIT 1e-12
 
 #define OKMI_CONST __attribute__((address_space(4)))
 template <typename T>
