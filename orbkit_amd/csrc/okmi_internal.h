@@ -120,7 +120,7 @@ int okmi_launch_rho_f64(const double* A, int64_t ld, const int64_t* set_off, int
 // points p0 + column) the 32-row chunks of the fused layout with a shell in
 // reach, in order: list[tile][..], count[tile]
 int okmi_launch_live_chunks(const OkmiBasis* b, const GridDesc& g, int brick, int64_t bk0, int64_t p0, int64_t n,
-                            int np, int* list, int* count, hipStream_t st, unsigned long long* mask = nullptr);
+                            int np, int* list, int* count, hipStream_t st, unsigned long long* mask = nullptr, bool split = false);
 // 64-bit words per tile of a live-chunk mask
 inline int live_mask_words(const OkmiBasis* b) { return (b->nchunk + 63) / 64; }
 // C [nmo][nao] -> [nmo][kpad] in the fused row order (colmap[k]: C column of row k, -1: zero)

@@ -866,6 +866,8 @@ int launch_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
     const bool scr = g.regular && p0 == 0 && np == g.nx * g.ny * g.nz && b->nchunk > 0 && b->d_ishells &&
                      b->nkpad > 0 && b->nkpad <= Lo.krows;
     const int K = scr ? b->nkpad : b->nao;   // rows of A and G
+    // the FP32 backward screens with the bound of the paths ending in FP32
+    const bool split = es == 4;
     const R* dg = d_grad;
     R* Cp = (R*)(w + Lo.cp);
     R* dCa = (R*)(w + Lo.dca);
@@ -909,12 +911,12 @@ int launch_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
         int rc;
         if (scr) {
             bm.bk0 = c0 / 64;
-            rc = okmi_launch_live_chunks(b, g, 1, bm.bk0, 0, n, 64, live, lcount, st, lmask);
+            rc = okmi_launch_live_chunks(b, g, 1, bm.bk0, 0, n, 64, live, lcount, st, lmask, split);
             if (rc) return rc;
             const AoBricks bk{bm.bk0, n / 64, bm.nbx, bm.nby, bm.nbz};
-            const AoLiveMask am{lmask, nw, 0, nullptr};
-            rc = okmi_launch_ao_eval_ex(b, b->d_ishells, b->d_fprims, g, 0, n, &bk, &set0, 1,
-                                        es == 8 ? OKMI_F64 : OKMI_F32, A, chunk, 0, st, &am);
+            const AoLiveMask am{lmask, nw, 0, split ? b->d_wshells_s : nullptr};
+            rc = okmi_launch_ao_eval_ex(b, split ? b->d_ishells_s : b->d_ishells, split ? b->d_fprims_s : b->d_fprims,
+                                        g, 0, n, &bk, &set0, 1, es == 8 ? OKMI_F64 : OKMI_F32, A, chunk, 0, st, &am);
             if (rc) return rc;
             hipLaunchKernelGGL(k_gather_bricks<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_grad, bm, n,
                                gb);
@@ -966,8 +968,8 @@ int launch_backward(const OkmiBasis* b, const GridDesc& g, int64_t p0, int64_t n
             launch_gemm_nn<R>(WCT, K, nmo, B, chunk, n64, G, chunk, st, scr ? 2 : 0, live, lcount, b->nchunk);
         }
         ShellGradArgs<R> sa{};
-        sa.shells = scr ? b->d_ishells : b->d_shells;
-        sa.prims = scr ? b->d_fprims : b->d_prims;
+        sa.shells = scr ? (split ? b->d_ishells_s : b->d_ishells) : b->d_shells;
+        sa.prims = scr ? (split ? b->d_fprims_s : b->d_fprims) : b->d_prims;
         sa.T = b->d_T; sa.grid = g;
         sa.p0 = p0 + c0; sa.np = n; sa.ld = chunk; sa.nprim_total = b->nprim; sa.is_normalized = b->is_normalized;
         sa.G = G; sa.partial = part; sa.ncol = Lo.ncol;

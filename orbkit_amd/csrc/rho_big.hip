@@ -528,7 +528,7 @@ BigLayout big_layout(const OkmiBasis* b, int nmo, int prec, int S)
 }  // namespace
 
 int okmi_launch_live_chunks(const OkmiBasis* b, const GridDesc& g, int brick, int64_t bk0, int64_t p0, int64_t n,
-                            int np, int* list, int* count, hipStream_t st, unsigned long long* mask)
+                            int np, int* list, int* count, hipStream_t st, unsigned long long* mask, bool split)
 {
     if (n <= 0) return 0;
     LiveArgs la{};
@@ -540,7 +540,7 @@ int okmi_launch_live_chunks(const OkmiBasis* b, const GridDesc& g, int brick, in
         la.bm.bk0 = bk0;
     }
     la.p0 = p0; la.n = n; la.np = np;
-    la.chunks = b->d_chunks; la.shells = b->d_fshells; la.nchunk = b->nchunk;
+    la.chunks = b->d_chunks; la.shells = split ? b->d_fshells_s : b->d_fshells; la.nchunk = b->nchunk;
     la.list = list; la.count = count;
     la.mask = mask; la.words = live_mask_words(b);
     hipLaunchKernelGGL(k_big_live, dim3((unsigned)((n + np - 1) / np)), dim3(64), 0, st, la);
