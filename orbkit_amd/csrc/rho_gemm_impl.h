@@ -10,11 +10,12 @@
 // registers.  In density mode the MOs are reduced in registers and never
 // written; in MO mode Phi is stored [set][mo][point].  With more than one MO
 // group (density mode) the waves of a workgroup instead share 16 points and
-// split the groups between them (one wave per group up to 8 groups, else 4
-// waves), so that the AO rows of a K step are fetched once for several
-// groups.
+// split the MO tiles between them in even contiguous shares (one wave per
+// group up to 4 groups, 8 waves for 5 to 8, else 4 waves), so that the AO
+// rows of a K step are fetched once for several groups.
 #pragma once
 #include "okmi_internal.h"
+#include <type_traits>
 
 namespace okmi_gemm {
 
@@ -47,10 +48,10 @@ struct GemmArgs {
     int screened;
     F64Screen scr;
     // density mode: the W waves of a workgroup (W = blockDim / 64) share 16
-    // points and take the MO groups round-robin (A rows read by W waves
-    // together, not once per group), their sums added in wave order at the
-    // end; else 4 waves each own 16 points of the workgroup's 64 and run
-    // every group
+    // points and split the MO tiles in contiguous shares of nmt / W (A rows
+    // read by W waves together, not once per group), their sums added in
+    // wave order at the end; else 4 waves each own 16 points of the
+    // workgroup's 64 and run every group
     int wave_groups;
 };
 
@@ -182,7 +183,7 @@ __device__ __forceinline__ void rho_terms(const d4 (&acc)[NSC], int i, double f0
 //   d_c   += 2 w (f0 f_c + e_c),  e_c = f_a^2 (c = aa), f_a f_b (c = ab), 0 (c = a)
 // FULL: every code 0..NSC-1 present (no per-set branches) and the loads
 // unconditional (rows clamped, the zeroing done by selects), a straight
-// sequence of MG x NSC MFMAs per K step
+// sequence of NT x NSC MFMAs per K step (NT <= MG tiles in the pass)
 template <int MG, int NSC, bool FULL>
 __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
 {
@@ -223,24 +224,39 @@ __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
     };
     const int ldc = a.screened ? a.scr.ldc : a.nao;
     const int nmt = (a.nmo + 15) >> 4;
-    const int ngroups = (nmt + MG - 1) / MG;
+    // this wave's MO tiles.  With four-tile groups where a pass short of MG
+    // tiles can run the instance of its exact count (the accumulators leave
+    // room: MG x NSC <= 16): all nmt tiles, or (wave groups) a contiguous
+    // share as even as the wave count allows, so that no wave runs MFMAs on
+    // tiles past the last MO and the waves sharing a SIMD carry about the
+    // same work.  Otherwise groups of MG tiles, round-robin over the waves,
+    // the last group masked past the last MO: the larger instances sit at
+    // the register limit, where the extra code spilled, and with fewer than
+    // four tiles per group a single group always holds exactly the nmt tiles
+    // (MG = nmt) while several groups only occur beyond 16 products per K
+    // step (no exact-count passes there).
+    constexpr bool EXACT = MG == 4 && MG * NSC <= 16;
+    const int t_beg = wg ? (int)((int64_t)nmt * wave / nwave) : 0;
+    const int t_end = wg ? (int)((int64_t)nmt * (wave + 1) / nwave) : nmt;
 
     double rho_acc = 0.0;
     double dsum[NSC];
 #pragma unroll
     for (int d = 0; d < NSC; ++d) dsum[d] = 0.0;
 
-    for (int g = wg ? wave : 0; g < ngroups; g += wg ? nwave : 1) {
-        d4 acc[MG][NSC];
+    // one pass over the K steps for the NT (<= MG) tiles from t0
+    auto pass = [&](const int t0, auto ntc) {
+        constexpr int NT = decltype(ntc)::value;
+        d4 acc[NT][NSC];
 #pragma unroll
-        for (int mt = 0; mt < MG; ++mt)
+        for (int mt = 0; mt < NT; ++mt)
 #pragma unroll
             for (int d = 0; d < NSC; ++d) acc[mt][d] = d4{0.0, 0.0, 0.0, 0.0};
-        const double* cbase[MG];
-        bool min_[MG];
+        const double* cbase[NT];
+        bool min_[NT];
 #pragma unroll
-        for (int mt = 0; mt < MG; ++mt) {
-            const int m = (g * MG + mt) * 16 + n;
+        for (int mt = 0; mt < NT; ++mt) {
+            const int m = (t0 + mt) * 16 + n;
             min_[mt] = m < a.nmo;
             cbase[mt] = a.C + (int64_t)(min_[mt] ? m : 0) * ldc;
         }
@@ -268,9 +284,9 @@ __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
             // loaded before the MFMAs of step ks (raw loads; the masks
             // applied when they are used); with more, the registers of a
             // second operand set would spill.
-            constexpr bool PREF = MG * NSC <= 16;
+            constexpr bool PREF = NT * NSC <= 16;
             const double pm = pin ? 1.0 : 0.0;
-            double ra[MG], rb[NSC], rm = 0.0;
+            double ra[NT], rb[NSC], rm = 0.0;
             auto fetch = [&](int ks) {
                 int k, kcol;
                 bool kin, cin;
@@ -278,7 +294,7 @@ __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
                 const int kc = kin ? k : a.nao - 1;
                 const int cc = cin ? kcol : 0;
 #pragma unroll
-                for (int mt = 0; mt < MG; ++mt) ra[mt] = cbase[mt][cc];
+                for (int mt = 0; mt < NT; ++mt) ra[mt] = cbase[mt][cc];
 #pragma unroll
                 for (int d = 0; d < NSC; ++d) rb[d] = abase[a.set_off[d] + (int64_t)kc * a.ld];
                 rm = cin ? 1.0 : 0.0;
@@ -286,9 +302,9 @@ __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
             if (PREF && nsteps > 0) fetch(0);
             for (int ks = 0; ks < nsteps; ++ks) {
                 if constexpr (!PREF) fetch(ks);
-                double av[MG], bv[NSC];
+                double av[NT], bv[NSC];
 #pragma unroll
-                for (int mt = 0; mt < MG; ++mt) av[mt] = ra[mt] * (min_[mt] ? rm : 0.0);
+                for (int mt = 0; mt < NT; ++mt) av[mt] = ra[mt] * (min_[mt] ? rm : 0.0);
 #pragma unroll
                 for (int d = 0; d < NSC; ++d) bv[d] = rb[d] * pm;
                 if constexpr (PREF)
@@ -296,23 +312,27 @@ __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
 #pragma unroll
                 for (int d = 0; d < NSC; ++d)
 #pragma unroll
-                    for (int mt = 0; mt < MG; ++mt)
+                    for (int mt = 0; mt < NT; ++mt)
                         acc[mt][d] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bv[d], acc[mt][d], 0, 0, 0);
+                // nothing of the next step is scheduled among the MFMAs: the
+                // copies of the prefetched operands into place would each
+                // wait there for their load
+                if constexpr (PREF) __builtin_amdgcn_sched_barrier(0);
             }
         } else {
             for (int ks = 0; ks < nsteps; ++ks) {
                 int k, kcol;
                 bool kin, cin;
                 row_of(ks, k, kin, kcol, cin);
-                double av[MG];
+                double av[NT];
 #pragma unroll
-                for (int mt = 0; mt < MG; ++mt) av[mt] = (cin && min_[mt]) ? cbase[mt][kcol] : 0.0;
+                for (int mt = 0; mt < NT; ++mt) av[mt] = (cin && min_[mt]) ? cbase[mt][kcol] : 0.0;
 #pragma unroll
                 for (int d = 0; d < NSC; ++d) {
                     if (a.set_mask & (1 << d)) {
                         const double bv = (kin && pin) ? abase[a.set_off[d] + (int64_t)k * a.ld] : 0.0;
 #pragma unroll
-                        for (int mt = 0; mt < MG; ++mt)
+                        for (int mt = 0; mt < NT; ++mt)
                             acc[mt][d] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bv, acc[mt][d], 0, 0, 0);
                     }
                 }
@@ -320,10 +340,10 @@ __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
         }
         // item i of this lane is Phi[m = tile*16 + 4*i + kq][p]
 #pragma unroll
-        for (int mt = 0; mt < MG; ++mt) {
+        for (int mt = 0; mt < NT; ++mt) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int m = (g * MG + mt) * 16 + kq + 4 * i;
+                const int m = (t0 + mt) * 16 + kq + 4 * i;
                 const double w = m < a.nmo ? a.occ[m] : 0.0;
                 const double f0 = acc[mt][0][i];
                 rho_acc += w * (f0 * f0);
@@ -338,6 +358,20 @@ __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
                 }
             }
         }
+    };
+    if constexpr (EXACT) {
+        for (int t0 = t_beg; t0 < t_end; t0 += MG) {
+            const int nt = t_end - t0 < MG ? t_end - t0 : MG;   // wave-uniform
+            if (nt == MG) pass(t0, std::integral_constant<int, MG>{});
+            else if (nt == 1) pass(t0, std::integral_constant<int, 1>{});
+            else if constexpr (MG > 2) {
+                if (nt == 2) pass(t0, std::integral_constant<int, 2>{});
+                else if constexpr (MG > 3) pass(t0, std::integral_constant<int, 3>{});
+            }
+        }
+    } else {
+        const int ngroups = (nmt + MG - 1) / MG;
+        for (int g = wg ? wave : 0; g < ngroups; g += wg ? nwave : 1) pass(g * MG, std::integral_constant<int, MG>{});
     }
     rho_acc += __shfl_xor(rho_acc, 16);
     rho_acc += __shfl_xor(rho_acc, 32);
@@ -383,9 +417,14 @@ __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
 template <int MG, bool FULL>
 int launch_rho_full(const GemmArgs& a, int nsc, hipStream_t st)
 {
-    // wave_groups: one wave per MO group up to 8 groups, else 4 waves
+    // wave_groups: one wave per MO group up to 4 groups; for 5 to 8 groups
+    // 8 waves where passes run their exact tile count (four-tile groups, at
+    // most 16 products per K step: the tiles dealt evenly, so each SIMD
+    // holds two waves of about equal work), else one wave per group; beyond
+    // 8 groups 4 waves
     const int nmt = (a.nmo + 15) / 16, ng = (nmt + MG - 1) / MG;
-    dim3 blk(a.wave_groups ? 64 * (ng <= 8 ? ng : 4) : 256);
+    const int w8 = MG == 4 && nsc * MG <= 16 ? 8 : ng;
+    dim3 blk(a.wave_groups ? 64 * (ng <= 4 ? ng : ng <= 8 ? w8 : 4) : 256);
     dim3 grd((unsigned)(a.wave_groups ? (a.np + 15) / 16 : (a.np + 63) / 64));
     switch (nsc) {
     case 1: hipLaunchKernelGGL((k_rho_f64<MG, 1, FULL>), grd, blk, 0, st, a); return 0;

@@ -500,6 +500,31 @@ def test_large_basis_slices(ok):
   assert rel_err(mo.reshape(400, -1), mo_o) <= TOL
 
 
+@pytest.mark.parametrize('name,nmo', [('medium', 70), ('medium', 260), ('large', 520)])
+def test_staged_mo_shares(ok, name, nmo):
+  '''The FP64 path beyond 64 MOs splits the 16-MO tiles between the waves of
+  a workgroup in contiguous shares, each run in passes of at most four tiles
+  with the exact count of the last one: 5 tiles over 2 waves (passes of 2
+  and 3), 17 over 8 waves (2 and 3), 33 over 4 waves (4 + 4 and 4 + 4 + 1).
+  rho and rho + grad rho on a screened regular grid and on a vector grid,
+  against the oracle.'''
+  core, grid, O = ok['core'], ok['grid'], ok['O']
+  qc, axes, _ = ok['W'].make(name, npts_side=9, nmo=nmo)
+  print('case staged MO shares', name, nmo, flush=True)
+  grid.set_grid(*axes, is_vector=False)
+  rho = core.rho_compute(qc)
+  rho4, d = core.rho_compute(qc, drv=['x', 'y', 'z'])
+  X, Y, Z = numpy.meshgrid(*axes, indexing='ij')
+  r_o = O.rho_compute(qc, X.ravel(), Y.ravel(), Z.ravel(), True, drv=['x', 'y', 'z'])
+  assert rel_err(rho.ravel(), r_o[0]) <= TOL and rel_err(rho4.ravel(), r_o[0]) <= TOL
+  assert rel_err(d.reshape(3, -1), r_o[1]) <= TOL
+  x, y, z = _pts(axes, 777, seed=5)
+  grid.set_grid(x, y, z, is_vector=True)
+  rho, d = core.rho_compute(qc, drv=['x', 'y', 'z'])
+  r_o = O.rho_compute(qc, x, y, z, True, drv=['x', 'y', 'z'])
+  assert rel_err(rho, r_o[0]) <= TOL and rel_err(d, r_o[1]) <= TOL
+
+
 def test_large_full_grid_laplacian(ok):
   """The large workload at its full size with the Laplacian: 2,950 AOs, 400
   MOs, the whole 128^3 grid (2.1e6 points), rho + grad rho + the three

@@ -163,3 +163,64 @@ def test_primitive_groups_need_radius_order():
     if abs(_group_sum(alpha, d, rcut2, d2, rr) - exact) > 1e-12 * abs(exact):
       dropped += 1
   assert dropped > 100   # file order loses live primitives in many configurations
+
+
+def _mo_group(nsets, nmt):
+  '''MO tiles per group of the FP64 density contraction (rho_gemm_impl.h
+  mo_group): at most 24 accumulator tiles, at most four MO tiles.'''
+  mg = max(1, min(4, 24 // max(nsets, 1)))
+  return min(nmt, mg)
+
+
+def _rho_f64_passes(nsc, nmo):
+  '''The passes each wave of one k_rho_f64 workgroup runs (launch_rho,
+  launch_rho_full and the kernel's share arithmetic): a list per wave of
+  (first tile, tiles run).'''
+  nmt = (nmo + 15) // 16
+  mg = _mo_group(nsc, nmt)
+  ng = (nmt + mg - 1) // mg
+  wg = ng >= 2
+  exact = mg == 4 and mg * nsc <= 16
+  w8 = 8 if exact else ng
+  nwave = (ng if ng <= 4 else w8 if ng <= 8 else 4) if wg else 1
+  assert 1 <= nwave <= 8                       # __launch_bounds__(512), red[8]
+  waves = []
+  for wave in range(nwave):
+    passes = []
+    if exact:
+      t_beg = nmt * wave // nwave if wg else 0
+      t_end = nmt * (wave + 1) // nwave if wg else nmt
+      t0 = t_beg
+      while t0 < t_end:
+        nt = min(mg, t_end - t0)
+        passes.append((t0, nt))
+        t0 += mg
+    else:
+      g = wave if wg else 0
+      while g < ng:
+        passes.append((g * mg, mg))
+        g += nwave if wg else 1
+    waves.append(passes)
+  return nmt, mg, waves
+
+
+@pytest.mark.parametrize('nsc', [1, 4, 7, 10])
+def test_rho_f64_tile_shares_cover_each_mo_once(nsc):
+  '''Every 16-MO tile is contracted by exactly one pass of one wave, every
+  pass runs at most MG tiles, and the tiles a pass runs past the last MO are
+  only those of the last group (masked by m < nmo in the kernel).'''
+  for nmo in range(1, 1200):
+    nmt, mg, waves = _rho_f64_passes(nsc, nmo)
+    seen = numpy.zeros(nmt + 4, dtype=int)
+    for passes in waves:
+      for t0, nt in passes:
+        assert 1 <= nt <= mg
+        seen[t0:t0 + nt] += 1
+    assert (seen[:nmt] == 1).all(), (nsc, nmo)
+    assert seen[nmt:].sum() <= mg - 1, (nsc, nmo)
+    assert all(t0 < nmt for passes in waves for t0, nt in passes)
+    if mg == 4 and mg * nsc <= 16:
+      # exact passes: none past the last tile, shares differ by one tile
+      assert all(t0 + nt <= nmt for passes in waves for t0, nt in passes)
+      shares = [sum(nt for _, nt in passes) for passes in waves]
+      assert max(shares) - min(shares) <= 1, (nsc, nmo, shares)
