@@ -32,6 +32,7 @@
 //           FP32 terms are added in FP64 in a fixed order).
 #include "okmi_internal.h"
 #include "ao_device.h"
+#include <type_traits>
 
 namespace {
 
@@ -137,8 +138,17 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(const R* __restrict__ Wp, in
     // the list's first 128 entries held across the lanes, read with readlane
     // (no scalar load for a step to wait on)
     const int lv0 = MODE && lane < nlive ? lst[lane] : 0, lv1 = MODE && lane + 64 < nlive ? lst[lane + 64] : 0;
+    // LONG: a list beyond 128 entries, whose tail is read from memory.  The
+    // common short list is read by readlane alone: a possible memory read
+    // would put each step's index behind a branch whose join waits for every
+    // load in flight (the prefetched operands included).
+    auto run = [&](auto longc) {
+    constexpr bool LONG = decltype(longc)::value;
     auto entry = [&](int j) {
-        return j < 64 ? __builtin_amdgcn_readlane(lv0, j) : j < 128 ? __builtin_amdgcn_readlane(lv1, j - 64) : lst[j];
+        if constexpr (LONG)
+            return j < 64 ? __builtin_amdgcn_readlane(lv0, j) : j < 128 ? __builtin_amdgcn_readlane(lv1, j - 64) : lst[j];
+        else
+            return j < 64 ? __builtin_amdgcn_readlane(lv0, j) : __builtin_amdgcn_readlane(lv1, j - 64);
     };
     // K steps of this wave and the packed step index of step st
     const int nst = MODE == 1 ? 8 * nlive : nks;
@@ -146,18 +156,26 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(const R* __restrict__ Wp, in
     const int ngroup = MODE == 2 ? nlive : (nmt + MG - 1) / MG;
     for (int gi = 0; gi < ngroup; ++gi) {
         const int g0 = MODE == 2 ? 2 * entry(gi) : gi * MG;   // first M tile of the group
-        const R* wp = Wp + (int64_t)g0 * nks * 64 + lane;
         V xr[D];
         R wr[D][MG];
+        // operands of step st (0 <= st < nst).  Every load is unconditional
+        // and in bounds (M tiles past the last read the last tile: their
+        // accumulators are never stored), so the loads of a step are not
+        // split into branches of their own, each with its own wait for every
+        // load in flight.
         auto load = [&](int st, V& xv, R (&wv)[MG]) {
             const int ks = kstep(st);
             xv = *(const V*)(xp + (int64_t)min(4 * ks + q, K - 1) * ldx);
 #pragma unroll
-            for (int mt = 0; mt < MG; ++mt) wv[mt] = g0 + mt < nmt ? wp[((int64_t)mt * nks + ks) * 64] : R(0);
+            for (int mt = 0; mt < MG; ++mt) {
+                const int t = min(g0 + mt, nmt - 1);
+                wv[mt] = Wp[((int64_t)t * nks + ks) * 64 + lane];
+            }
         };
+        if (nst > 0) {
 #pragma unroll
-        for (int d = 0; d < D; ++d)
-            if (d < nst) load(d, xr[d], wr[d]);
+            for (int d = 0; d < D; ++d) load(min(d, nst - 1), xr[d], wr[d]);
+        }
         V acc[MG][4];
 #pragma unroll
         for (int mt = 0; mt < MG; ++mt)
@@ -167,17 +185,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(const R* __restrict__ Wp, in
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const int st = s0 + d;
-                if (st < nst) {
-                    const V xv = xr[d];
-                    R wv[MG];
+                const R on = st < nst ? R(1) : R(0);   // steps past the end add nothing
+                const V xv = xr[d];
+                R wv[MG];
 #pragma unroll
-                    for (int mt = 0; mt < MG; ++mt) wv[mt] = wr[d][mt];
-                    if (st + D < nst) load(st + D, xr[d], wr[d]);
+                for (int mt = 0; mt < MG; ++mt) wv[mt] = wr[d][mt] * on;
+                load(min(st + D, nst - 1), xr[d], wr[d]);
 #pragma unroll
-                    for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < 4; ++j)
 #pragma unroll
-                        for (int mt = 0; mt < MG; ++mt) acc[mt][j] = Mfma<R>::mac(wv[mt], xv[j], acc[mt][j]);
-                }
+                    for (int mt = 0; mt < MG; ++mt) acc[mt][j] = Mfma<R>::mac(wv[mt], xv[j], acc[mt][j]);
             }
         }
 #pragma unroll
@@ -190,6 +207,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(const R* __restrict__ Wp, in
                         V{acc[mt][0][i], acc[mt][1][i], acc[mt][2][i], acc[mt][3][i]};
             }
     }
+    };
+    if (MODE && nlive > 128) run(std::true_type{});
+    else run(std::false_type{});
 }
 
 // The same product for a short K (at most 4 KS rows, K <= 64: G = C^T B
