@@ -238,24 +238,49 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn_xr(const R* __restrict__ Wp,
         xr[st] = st < nks ? *(const V*)(xp + (int64_t)min(4 * st + q, K - 1) * ldx) : V{0, 0, 0, 0};
     const int* lst = MODE ? live + tile * nchunk : nullptr;
     const int ngroup = MODE == 2 ? __builtin_amdgcn_readfirstlane(lcount[tile]) : (nmt + MG - 1) / MG;
+    // the list's first 128 entries held across the lanes, read with readlane;
+    // LONG: a longer list, whose tail is read from memory
+    const int lv0 = MODE && lane < ngroup ? lst[lane] : 0, lv1 = MODE && lane + 64 < ngroup ? lst[lane + 64] : 0;
+    auto run = [&](auto longc) {
+    constexpr bool LONG = decltype(longc)::value;
+    auto entry = [&](int j) {
+        if constexpr (LONG)
+            return j < 64 ? __builtin_amdgcn_readlane(lv0, j) : j < 128 ? __builtin_amdgcn_readlane(lv1, j - 64) : lst[j];
+        else
+            return j < 64 ? __builtin_amdgcn_readlane(lv0, j) : __builtin_amdgcn_readlane(lv1, j - 64);
+    };
     for (int gi = 0; gi < ngroup; ++gi) {
-        const int g0 = MODE == 2 ? 2 * lst[gi] : gi * MG;
-        const R* wp = Wp + (int64_t)g0 * nks * 64 + lane;
+        const int g0 = MODE == 2 ? 2 * entry(gi) : gi * MG;
         V acc[MG][4];
 #pragma unroll
         for (int mt = 0; mt < MG; ++mt)
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[mt][j] = V{0, 0, 0, 0};
+        // the group's W operands, loaded in batches of WB K steps before
+        // their MFMAs (one wait per batch, not one per K step; FP64 at 16
+        // steps takes two batches, which keeps it within the registers); the
+        // loads are unconditional and in bounds: M tiles past the last read
+        // the last tile (never stored)
+        constexpr int WB = sizeof(R) == 8 && KS > 12 ? KS / 4 : KS;
 #pragma unroll
-        for (int st = 0; st < KS; ++st) {
-            if (st < nks) {
-                R wv[MG];
+        for (int s0 = 0; s0 < KS; s0 += WB) {
+            R wv[WB][MG];
 #pragma unroll
-                for (int mt = 0; mt < MG; ++mt) wv[mt] = g0 + mt < nmt ? wp[((int64_t)mt * nks + st) * 64] : R(0);
+            for (int u = 0; u < WB; ++u)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int mt = 0; mt < MG; ++mt) {
+                    const int t = min(g0 + mt, nmt - 1), ss = min(s0 + u, nks - 1);
+                    wv[u][mt] = Wp[((int64_t)t * nks + ss) * 64 + lane];
+                }
 #pragma unroll
-                    for (int mt = 0; mt < MG; ++mt) acc[mt][j] = Mfma<R>::mac(wv[mt], xr[st][j], acc[mt][j]);
+            for (int u = 0; u < WB; ++u) {
+                if (s0 + u < nks) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+#pragma unroll
+                        for (int mt = 0; mt < MG; ++mt)
+                            acc[mt][j] = Mfma<R>::mac(wv[u][mt], xr[s0 + u][j], acc[mt][j]);
+                }
             }
         }
 #pragma unroll
@@ -268,6 +293,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn_xr(const R* __restrict__ Wp,
                         V{acc[mt][0][i], acc[mt][1][i], acc[mt][2][i], acc[mt][3][i]};
             }
     }
+    };
+    if (MODE && ngroup > 128) run(std::true_type{});
+    else run(std::false_type{});
 }
 
 // dL/dC partials: part[s][m][n] = sum over the points k of range s of
