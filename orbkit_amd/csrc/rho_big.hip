@@ -85,7 +85,9 @@ __device__ __forceinline__ int64_t logical_block(int64_t b, int64_t nb)
     return (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + j;
 }
 
-template <int PREC, int S>
+// LONG: live lists may exceed 128 entries (more than 128 chunks of 32 rows),
+// and their tail is read from memory
+template <int PREC, int S, bool LONG>
 __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
 {
     using LD = BigLds<PREC, S>;
@@ -109,10 +111,6 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
     // j % 64 of lv[j / 64]) and read with readlane: a step's chunk index is
     // not a scalar load the step waits on
     const int lv0 = lane < cnt ? lst_mem[lane] : 0, lv1 = lane + 64 < cnt ? lst_mem[lane + 64] : 0;
-    auto lst = [&](int j) {
-        return j < 64 ? __builtin_amdgcn_readlane(lv0, j) : j < 128 ? __builtin_amdgcn_readlane(lv1, j - 64)
-                                                                    : lst_mem[j];
-    };
     // 16-MO tiles of this wave that hold MOs (wave-uniform): the last MO
     // tile group of a padded count skips the MFMAs (and the AO split) of
     // its padding tiles, e.g. 400 MOs = 25 tiles of the 32 launched
@@ -134,11 +132,11 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
         for (int v = 0; v < AO_V4; ++v) {
             const int e = v * NT + threadIdx.x;          // float4 index in [S][KS][NP/4]
             const int pc = e % (NP / 4), r = (e / (NP / 4)) % KS, s = e / (NP / 4 * KS);
-            const int k = k0 + r;
-            if (k < a.nao)
-                R.ao[v] = *(const fl4*)(a.ao + s * a.set_stride + (int64_t)k * a.ld + p0 + 4 * pc);
-            else
-                R.ao[v] = fl4{0.f, 0.f, 0.f, 0.f};
+            // a listed chunk's rows lie below the padded row count; the
+            // clamp only keeps the load in bounds.  Unconditional loads: a
+            // load behind a branch joins with a wait for every load in flight
+            const int k = min(k0 + r, a.nao - 1);
+            R.ao[v] = *(const fl4*)(a.ao + s * a.set_stride + (int64_t)k * a.ld + p0 + 4 * pc);
         }
 #pragma unroll
         for (int v = 0; v < C_V4; ++v) {
@@ -240,15 +238,26 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
     // R_next holds step i + 1 (loading), R_free held step i (already in LDS)
     // and takes step i + 2 (measured at C5: S = 7 280 -> 266 ms; S = 4
     // 125 -> 137 ms, so S = 4 keeps one stage).
+    // Without LONG the list is read by readlane alone (an entry that might
+    // come from memory puts each step behind a branch whose join waits for
+    // every load in flight).  The next steps' loads are unconditional: past
+    // the last step they read the last chunk again, never stored to LDS.
+    auto lst = [&](int j) {
+        if constexpr (LONG)
+            return j < 64 ? __builtin_amdgcn_readlane(lv0, j) : j < 128 ? __builtin_amdgcn_readlane(lv1, j - 64)
+                                                                        : lst_mem[j];
+        else
+            return j < 64 ? __builtin_amdgcn_readlane(lv0, j) : __builtin_amdgcn_readlane(lv1, j - 64);
+    };
     if (cnt > 0) {
         Stage R0, R1;
         gload(lst(0), R0);
         sstore(0, R0);
         if constexpr (S >= 7) {
-            if (cnt > 1) gload(lst(1), R1);
+            gload(lst(min(1, cnt - 1)), R1);
             __syncthreads();
             auto body = [&](int i, int buf, Stage& Rnext, Stage& Rfree) {
-                if (i + 2 < cnt) gload(lst(i + 2), Rfree);
+                gload(lst(min(i + 2, cnt - 1)), Rfree);
                 compute(buf);
                 if (i + 1 < cnt) sstore(buf ^ 1, Rnext);
                 __syncthreads();
@@ -261,7 +270,7 @@ __global__ __launch_bounds__(64 * BigCfg<S>::NW, 1) void k_rho_big(BigArgs a)
             __syncthreads();
             int buf = 0;
             for (int i = 0; i < cnt; ++i) {
-                if (i + 1 < cnt) gload(lst(i + 1), R0);
+                gload(lst(min(i + 1, cnt - 1)), R0);
                 compute(buf);
                 if (i + 1 < cnt) sstore(buf ^ 1, R0);
                 __syncthreads();
@@ -468,20 +477,26 @@ __global__ void k_zero_rows(T* ao, int S, int64_t set_stride, int64_t ld, const 
     ao[s * set_stride + (int64_t)rows[r] * ld + c] = T(0);
 }
 
-template <int PREC, int S>
-int launch_big_s(const BigArgs& a, hipStream_t st)
+template <int PREC, int S, bool LONG>
+int launch_big_l(const BigArgs& a, hipStream_t st)
 {
     using LD = BigLds<PREC, S>;
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_rho_big<PREC, S>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           LD::TOTAL);
+        hipError_t e = hipFuncSetAttribute((const void*)k_rho_big<PREC, S, LONG>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, LD::TOTAL);
         if (e != hipSuccess) return okmi_hip_fail(e, "rho_big: LDS size");
         attr_set = true;
     }
     OkmiProfScope prof("rho_big", st);
-    hipLaunchKernelGGL((k_rho_big<PREC, S>), dim3((unsigned)a.nblocks), dim3(BigCfg<S>::NT), LD::TOTAL, st, a);
+    hipLaunchKernelGGL((k_rho_big<PREC, S, LONG>), dim3((unsigned)a.nblocks), dim3(BigCfg<S>::NT), LD::TOTAL, st, a);
     return 0;
+}
+
+template <int PREC, int S>
+int launch_big_s(const BigArgs& a, hipStream_t st)
+{
+    return a.nchunk > 128 ? launch_big_l<PREC, S, true>(a, st) : launch_big_l<PREC, S, false>(a, st);
 }
 
 template <int PREC>
