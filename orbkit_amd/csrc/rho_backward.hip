@@ -484,15 +484,15 @@ struct ShellGradArgs {
 // w_c += sum over the rows r of a standard spherical shell of G_r T[r][c]
 // (the transpose of sph_rows_to: at most three terms per row)
 template <int L, int RW, typename R>
-__device__ __forceinline__ void sph_rows_back(const R* Gi, int64_t ld, R (&w)[ncart(L)])
+__device__ __forceinline__ void sph_rows_back(const R (&gv)[ncart(L)], R (&w)[ncart(L)])
 {
     if constexpr (L <= 3 && RW < 2 * L + 1) {
         constexpr SphRow row = sph_row(L, RW);
-        const R g = Gi[(int64_t)RW * ld];
+        const R g = gv[RW];
         w[row.t[0].c] = fma(g, (R)row.t[0].v, w[row.t[0].c]);
         if constexpr (row.n > 1) w[row.t[1].c] = fma(g, (R)row.t[1].v, w[row.t[1].c]);
         if constexpr (row.n > 2) w[row.t[2].c] = fma(g, (R)row.t[2].v, w[row.t[2].c]);
-        sph_rows_back<L, RW + 1, R>(Gi, ld, w);
+        sph_rows_back<L, RW + 1, R>(gv, w);
     }
 }
 
@@ -505,6 +505,16 @@ __device__ __forceinline__ void shell_point(const ShellGradArgs<R>& a, const She
     const double X = px - sh.cx, Y = py - sh.cy, Z = pz - sh.cz;
     const double rr = X * X + Y * Y + Z * Z;
     const R Xr = (R)X, Yr = (R)Y, Zr = (R)Z;
+    // the G rows of a standard shell (2L + 1 spherical, NC Cartesian) are
+    // loaded before the primitive loop, whose exponentials then cover their
+    // latency; the loads are unconditional (rows past the shell's last read
+    // its last row, unused)
+    const R* Gi = a.G + (int64_t)sh.row0 * a.ld + i;
+    R gv[NC];
+    if (sh.kind != SHELL_T_GENERAL) {
+#pragma unroll
+        for (int k = 0; k < NC; ++k) gv[k] = Gi[(int64_t)min(k, sh.nrow - 1) * a.ld];
+    }
     R S0 = 0, S1 = 0;
     R dw[MP];
 #pragma unroll
@@ -526,12 +536,11 @@ __device__ __forceinline__ void shell_point(const ShellGradArgs<R>& a, const She
     R w[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) w[k] = 0;
-    const R* Gi = a.G + (int64_t)sh.row0 * a.ld + i;
     if (sh.kind == SHELL_T_SPHERICAL) {
-        sph_rows_back<L, 0, R>(Gi, a.ld, w);
+        sph_rows_back<L, 0, R>(gv, w);
     } else if (sh.kind == SHELL_T_CARTESIAN) {
 #pragma unroll
-        for (int k = 0; k < NC; ++k) w[k] = Gi[(int64_t)k * a.ld] * (R)cart_norm(L, k);
+        for (int k = 0; k < NC; ++k) w[k] = gv[k] * (R)cart_norm(L, k);
     } else {
         for (int rw = 0; rw < sh.nrow; ++rw) {
             const R g = Gi[(int64_t)rw * a.ld];
