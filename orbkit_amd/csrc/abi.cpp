@@ -570,7 +570,7 @@ extern "C" int okmi_rho_staged(const okmi_basis* b, const okmi_grid* g, int64_t 
     if (!rc) rc = okmi_launch_zero_rows(A, nsets, rows * ch, ch, b->d_padrows, b->npadrows, ch, st);
     if (rc) return rc;
     F64Screen scr{};
-    scr.colmap = nullptr; scr.live = live; scr.live_count = live_count; scr.nchunk = b->nchunk;
+    scr.live = live; scr.live_count = live_count; scr.nchunk = b->nchunk;
     scr.ldc = (int)rows; scr.brick = brick ? 1 : 0;
     scr.nbx = nbx; scr.nby = nby; scr.nbz = nbz; scr.nx = g->nx; scr.ny = g->ny; scr.nz = g->nz;
     const AoLiveMask am{lmask, nw, 0, nullptr};

@@ -93,11 +93,10 @@ int okmi_launch_gemm_f64(const double* A, int64_t ld, int64_t set_stride, int ns
                          int nmo, int64_t np, double* mo_out, int64_t ld_mo, hipStream_t st);
 
 // Screening of the staged FP64 density: AO rows in the fused row order
-// (C read through colmap), per 64-column tile the chunks with a shell in
+// (C permuted to the same order), per 64-column tile the chunks with a shell in
 // reach, and optionally brick-ordered columns (column c is lane c % 64 of
 // brick bk0 + c / 64 of an nbx x nby x nbz brick grid over nx x ny x nz).
 struct F64Screen {
-    const int* colmap;       // C column of each A row (-1: zero); null: C is already in the A row order
     const int* live;         // [tile][nchunk]
     const int* live_count;   // [tile]
     int nchunk;

@@ -41,10 +41,10 @@ struct GemmArgs {
     int64_t set_off[OKMI_MAX_SETS];   // offset of code's AO set in A
     int nout;
     int out_code[16];        // derivative code of each output row
-    // density mode, screened (okmi_launch_rho_f64 with F64Screen): A rows in
-    // the fused row order, C columns through scr.colmap, each 64-column
-    // block contracts its live chunks only, brick-ordered columns mapped to
-    // grid points for the outputs
+    // density mode, screened (okmi_launch_rho_f64 with F64Screen): A rows and
+    // C columns in the fused row order, each 64-column block contracts its
+    // live chunks only, brick-ordered columns mapped to grid points for the
+    // outputs
     int screened;
     F64Screen scr;
     // density mode: the W waves of a workgroup (W = blockDim / 64) share 16
@@ -184,7 +184,9 @@ __device__ __forceinline__ void rho_terms(const d4 (&acc)[NSC], int i, double f0
 // FULL: every code 0..NSC-1 present (no per-set branches) and the loads
 // unconditional (rows clamped, the zeroing done by selects), a straight
 // sequence of NT x NSC MFMAs per K step (NT <= MG tiles in the pass)
-template <int MG, int NSC, bool FULL>
+// LONG: live lists may exceed 128 entries (more than 128 chunks of 32 rows),
+// and their tail is read from memory
+template <int MG, int NSC, bool FULL, bool LONG>
 __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
@@ -219,8 +221,15 @@ __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
         lv0 = lane < cnt ? lst[lane] : 0;
         lv1 = lane + 64 < cnt ? lst[lane + 64] : 0;
     }
-    auto chunk_of = [&](int j) {   // entry j of the list (wave-uniform j)
-        return j < 64 ? __builtin_amdgcn_readlane(lv0, j) : j < 128 ? __builtin_amdgcn_readlane(lv1, j - 64) : lst[j];
+    // entry j of the list (wave-uniform j).  Without LONG by readlane alone:
+    // an entry that might come from memory puts each K step behind a branch
+    // whose join waits for every load in flight (the prefetched operands
+    // included)
+    auto chunk_of = [&](int j) {
+        if constexpr (LONG)
+            return j < 64 ? __builtin_amdgcn_readlane(lv0, j) : j < 128 ? __builtin_amdgcn_readlane(lv1, j - 64) : lst[j];
+        else
+            return j < 64 ? __builtin_amdgcn_readlane(lv0, j) : __builtin_amdgcn_readlane(lv1, j - 64);
     };
     const int ldc = a.screened ? a.scr.ldc : a.nao;
     const int nmt = (a.nmo + 15) >> 4;
@@ -262,16 +271,12 @@ __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
         }
         const double* abase = a.A + (p < a.np ? p : 0);
         // A row of K step ks for this lane, and the C column it pairs with
-        // (fused row order: through colmap, -1 = zero row)
+        // (the same index: C is given in the A row order)
         auto row_of = [&](int ks, int& k, bool& kin, int& kcol, bool& cin) {
             k = (a.screened ? chunk_of(ks >> 3) * 32 + 4 * (ks & 7) : 4 * ks) + kq;
             kin = k < a.nao;
             kcol = k;
             cin = kin;
-            if (a.screened && a.scr.colmap) {
-                kcol = a.scr.colmap[kin ? k : 0];
-                cin = kin && kcol >= 0;
-            }
         };
         if constexpr (FULL) {
             // Rows past the end read row nao - 1 and are zeroed through av;
@@ -284,40 +289,102 @@ __global__ __launch_bounds__(512) void k_rho_f64(GemmArgs a)
             // loaded before the MFMAs of step ks (raw loads; the masks
             // applied when they are used); with more, the registers of a
             // second operand set would spill.
-            constexpr bool PREF = NT * NSC <= 16;
+            // K steps whose operands are in flight: two with groups of up to
+            // three MO tiles, one with four-tile groups (whose waves share a
+            // workgroup's AO rows: waves with one and with two steps in
+            // flight side by side ran the large workload 9% slower, two in
+            // every wave 1% slower), none beyond 16 accumulator tiles (their
+            // registers would spill)
+            constexpr int DEPTH = NT * NSC > 16 ? 0 : MG == 4 ? 1 : 2;
+            constexpr int NSET = DEPTH == 2 ? 3 : 1;
             const double pm = pin ? 1.0 : 0.0;
-            double ra[NT], rb[NSC], rm = 0.0;
-            auto fetch = [&](int ks) {
+            // operands of K step ks into set st (a step past the end reads
+            // step nsteps - 1 again: in bounds, never used)
+            double ra[NSET][NT], rb[NSET][NSC], rm[NSET];
+            auto fetch = [&](int ks, int st) {
                 int k, kcol;
                 bool kin, cin;
-                row_of(ks, k, kin, kcol, cin);
+                row_of(ks < nsteps ? ks : nsteps - 1, k, kin, kcol, cin);
                 const int kc = kin ? k : a.nao - 1;
                 const int cc = cin ? kcol : 0;
 #pragma unroll
-                for (int mt = 0; mt < NT; ++mt) ra[mt] = cbase[mt][cc];
+                for (int mt = 0; mt < NT; ++mt) ra[st][mt] = cbase[mt][cc];
 #pragma unroll
-                for (int d = 0; d < NSC; ++d) rb[d] = abase[a.set_off[d] + (int64_t)kc * a.ld];
-                rm = cin ? 1.0 : 0.0;
+                for (int d = 0; d < NSC; ++d) rb[st][d] = abase[a.set_off[d] + (int64_t)kc * a.ld];
+                rm[st] = cin ? 1.0 : 0.0;
             };
-            if (PREF && nsteps > 0) fetch(0);
-            for (int ks = 0; ks < nsteps; ++ks) {
-                if constexpr (!PREF) fetch(ks);
-                double av[NT], bv[NSC];
+            auto prep = [&](int st, double (&av)[NT], double (&bv)[NSC]) {
 #pragma unroll
-                for (int mt = 0; mt < NT; ++mt) av[mt] = ra[mt] * (min_[mt] ? rm : 0.0);
+                for (int mt = 0; mt < NT; ++mt) av[mt] = ra[st][mt] * (min_[mt] ? rm[st] : 0.0);
 #pragma unroll
-                for (int d = 0; d < NSC; ++d) bv[d] = rb[d] * pm;
-                if constexpr (PREF)
-                    if (ks + 1 < nsteps) fetch(ks + 1);
+                for (int d = 0; d < NSC; ++d) bv[d] = rb[st][d] * pm;
+            };
+            auto mma = [&](const double (&av)[NT], const double (&bv)[NSC]) {
 #pragma unroll
                 for (int d = 0; d < NSC; ++d)
 #pragma unroll
                     for (int mt = 0; mt < NT; ++mt)
                         acc[mt][d] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bv[d], acc[mt][d], 0, 0, 0);
-                // nothing of the next step is scheduled among the MFMAs: the
-                // copies of the prefetched operands into place would each
-                // wait there for their load
-                if constexpr (PREF) __builtin_amdgcn_sched_barrier(0);
+                // nothing of the next step is scheduled among the MFMAs: a
+                // copy of a prefetched operand there would wait for its load
+                __builtin_amdgcn_sched_barrier(0);
+            };
+            auto step = [&](int st) {
+                double av[NT], bv[NSC];
+                prep(st, av, bv);
+                mma(av, bv);
+            };
+            if constexpr (DEPTH == 2) {
+                // three operand sets in turn: the loads of step ks + 2 are
+                // issued before the MFMAs of step ks, so two steps of MFMAs
+                // (and the other waves') cover their latency; no operand is
+                // copied between sets (a copy waits for its load)
+                if (nsteps > 0) {
+                    fetch(0, 0);
+                    fetch(1, 1);
+                }
+                int ks = 0;
+                for (; ks + 3 <= nsteps; ks += 3) {
+                    fetch(ks + 2, 2);
+                    step(0);
+                    fetch(ks + 3, 0);
+                    step(1);
+                    fetch(ks + 4, 1);
+                    step(2);
+                }
+                if (ks < nsteps) step(0);       // the one or two steps left are in sets 0 and 1
+                if (ks + 1 < nsteps) step(1);
+            } else if constexpr (DEPTH == 1) {
+                // one set: the next step's loads are issued once the step's
+                // operands are taken, before its MFMAs
+                double ra1[NT], rb1[NSC], rm1 = 0.0;
+                auto fetch1 = [&](int ks) {
+                    int k, kcol;
+                    bool kin, cin;
+                    row_of(ks, k, kin, kcol, cin);
+                    const int kc = kin ? k : a.nao - 1;
+                    const int cc = cin ? kcol : 0;
+#pragma unroll
+                    for (int mt = 0; mt < NT; ++mt) ra1[mt] = cbase[mt][cc];
+#pragma unroll
+                    for (int d = 0; d < NSC; ++d) rb1[d] = abase[a.set_off[d] + (int64_t)kc * a.ld];
+                    rm1 = cin ? 1.0 : 0.0;
+                };
+                if (nsteps > 0) fetch1(0);
+                for (int ks = 0; ks < nsteps; ++ks) {
+                    double av[NT], bv[NSC];
+#pragma unroll
+                    for (int mt = 0; mt < NT; ++mt) av[mt] = ra1[mt] * (min_[mt] ? rm1 : 0.0);
+#pragma unroll
+                    for (int d = 0; d < NSC; ++d) bv[d] = rb1[d] * pm;
+                    if (ks + 1 < nsteps) fetch1(ks + 1);
+                    mma(av, bv);
+                }
+            } else {
+                for (int ks = 0; ks < nsteps; ++ks) {
+                    fetch(ks, 0);
+                    step(0);
+                }
             }
         } else {
             for (int ks = 0; ks < nsteps; ++ks) {
@@ -426,22 +493,24 @@ int launch_rho_full(const GemmArgs& a, int nsc, hipStream_t st)
     const int w8 = MG == 4 && nsc * MG <= 16 ? 8 : ng;
     dim3 blk(a.wave_groups ? 64 * (ng <= 4 ? ng : ng <= 8 ? w8 : 4) : 256);
     dim3 grd((unsigned)(a.wave_groups ? (a.np + 15) / 16 : (a.np + 63) / 64));
+    const bool lng = a.screened && a.scr.nchunk > 128;
+#define OKMI_RHO_LAUNCH(NS)                                                                   \
+    do {                                                                                      \
+        if (lng) hipLaunchKernelGGL((k_rho_f64<MG, NS, FULL, true>), grd, blk, 0, st, a);     \
+        else hipLaunchKernelGGL((k_rho_f64<MG, NS, FULL, false>), grd, blk, 0, st, a);        \
+        return 0;                                                                             \
+    } while (0)
     switch (nsc) {
-    case 1: hipLaunchKernelGGL((k_rho_f64<MG, 1, FULL>), grd, blk, 0, st, a); return 0;
-    case 4: hipLaunchKernelGGL((k_rho_f64<MG, 4, FULL>), grd, blk, 0, st, a); return 0;
+    case 1: OKMI_RHO_LAUNCH(1);
+    case 4: OKMI_RHO_LAUNCH(4);
     case 7:
-        if constexpr (MG <= 3) {
-            hipLaunchKernelGGL((k_rho_f64<MG, 7, FULL>), grd, blk, 0, st, a);
-            return 0;
-        }
+        if constexpr (MG <= 3) OKMI_RHO_LAUNCH(7);
         break;
     default:
-        if constexpr (MG <= 2) {
-            hipLaunchKernelGGL((k_rho_f64<MG, 10, FULL>), grd, blk, 0, st, a);
-            return 0;
-        }
+        if constexpr (MG <= 2) OKMI_RHO_LAUNCH(10);
         break;
     }
+#undef OKMI_RHO_LAUNCH
     okmi_set_error("rho_f64: unsupported code range %d for MO group %d", nsc, MG);
     return 1;
 }
