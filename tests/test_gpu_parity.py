@@ -525,6 +525,56 @@ def test_staged_mo_shares(ok, name, nmo):
   assert rel_err(rho, r_o[0]) <= TOL and rel_err(d, r_o[1]) <= TOL
 
 
+def test_long_live_lists(ok):
+  '''A basis of more than 128 chunks of 32 rows (300 atoms in a 15 A box,
+  6,000 spherical AOs) on a coarse grid whose first brick holds the whole
+  molecule and reaches nearly every chunk: the live-chunk
+  lists run past 128 entries, so the FP64 staged contraction and the many-MO
+  bf16x6 contraction take their long-list instances (entries past 128 read
+  from memory).  70 MOs (random coefficients: the density needs no
+  orthonormality); FP64 against the oracle, bf16x6 against FP64.'''
+  core, grid, O, W = ok['core'], ok['grid'], ok['O'], ok['W']
+  from orbkit_amd.orbitals import MOClass
+  from orbkit_amd.qcinfo import QCinfo
+  rng = numpy.random.default_rng(17)
+  geo = W._atoms(rng, 300, 15.0)
+  ao = W._basis(rng, geo, [['s', 's', 'p', 'p', 'd', 'f']] * 300, 0.05, 1e5, True)
+  assert ao.get_ao_num() == 6000
+  C = 0.05 * rng.standard_normal((70, 6000))
+  qc = QCinfo()
+  qc.geo_spec = geo
+  qc.geo_info = numpy.array([['X', str(i + 1), '0'] for i in range(len(geo))])
+  qc.ao_spec = ao
+  qc.mo_spec = MOClass([{'coeffs': C[i], 'occ_num': 2.0, 'energy': -1.0, 'sym': '%d.1' % (i + 1)} for i in range(70)])
+  axes = W._grid(geo, 6)
+  print('case long live lists', flush=True)
+  # the first brick's list runs past 128 chunks, with the FP64 radii and with
+  # the smaller ones of the FP32-ending paths (okmi_fused_live_rows counts
+  # live rows x points)
+  import ctypes
+  from orbkit_amd import _native
+  from orbkit_amd.basis import device_basis
+  b = device_basis(qc.geo_spec, qc.ao_spec)
+  L = _native.lib()
+  ax = [numpy.ascontiguousarray(a[:4], dtype=numpy.float64) for a in axes]
+  P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+  for prec in ('f64', 'bf16x6'):
+    out = ctypes.c_double()
+    _native.check(L.okmi_fused_live_rows(b.handle, P(ax[0]), 4, P(ax[1]), 4, P(ax[2]), 4, _native.PREC[prec],
+                                         ctypes.byref(out)), 'okmi_fused_live_rows')
+    print(prec, 'live chunks of the first brick', out.value / 64 / 32)
+    assert out.value > 128 * 32 * 64, (prec, out.value)
+  grid.set_grid(*axes, is_vector=False)
+  rho, d = core.rho_compute(qc, drv=['x', 'y', 'z'])
+  X, Y, Z = numpy.meshgrid(*axes, indexing='ij')
+  r_o = O.rho_compute(qc, X.ravel(), Y.ravel(), Z.ravel(), True, drv=['x', 'y', 'z'])
+  assert rel_err(rho.ravel(), r_o[0]) <= TOL and rel_err(d.reshape(3, -1), r_o[1]) <= TOL
+  frho, fd = core.rho_compute(qc, drv=['x', 'y', 'z'], precision='bf16x6')
+  e = [numpy.linalg.norm(frho - rho) / numpy.linalg.norm(rho), numpy.linalg.norm(fd - d) / numpy.linalg.norm(d)]
+  print('long lists bf16x6 vs f64', e)
+  assert max(e) <= 1e-6, e
+
+
 def test_large_full_grid_laplacian(ok):
   """The large workload at its full size with the Laplacian: 2,950 AOs, 400
   MOs, the whole 128^3 grid (2.1e6 points), rho + grad rho + the three
