@@ -222,6 +222,33 @@ def test_backward_screened_equals_dense(precision, tol):
     assert numpy.max(numpy.abs(a - b)) <= tol * numpy.max(numpy.abs(b))
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize('precision,tol,nmo', [('f64', 1e-12, 40), ('f64', 1e-12, 70), ('f32', 2e-6, 70)])
+def test_backward_long_live_lists(precision, tol, nmo):
+  """The screened backward with live-chunk lists past 128 entries (a
+  6,000-AO basis whose first brick reaches about 170 chunks of 32 rows: the
+  long-list instances of the C products; 40 MOs take the short-K product
+  for G, 70 the streaming one) equals the dense backward on the same points
+  given as a vector grid."""
+  torch = _gpu()
+  from conftest import long_list_case
+  from orbkit_amd.autograd import DensityModel
+  qc, axes = long_list_case(nmo)
+  print('case long-list backward', precision, nmo, flush=True)
+  X, Y, Z = numpy.meshgrid(*axes, indexing='ij')
+  vec = tuple(numpy.ascontiguousarray(v.reshape(-1)) for v in (X, Y, Z))
+  reg = DensityModel(qc, axes, precision=precision)
+  den = DensityModel(qc, vec, is_vector=True, precision=precision)
+  C, alpha, geo = reg.parameters(requires_grad=False)
+  g = torch.randn(reg.npts, dtype=torch.float64, device=C.device,
+                  generator=torch.Generator(device=C.device).manual_seed(5)).to(C.dtype)
+  out_r = reg.vjp(C, alpha, geo, g)
+  out_d = den.vjp(C, alpha, geo, g)
+  for a, b in zip(out_r, out_d):
+    a, b = a.double().cpu().numpy(), b.double().cpu().numpy()
+    assert numpy.max(numpy.abs(a - b)) <= tol * numpy.max(numpy.abs(b))
+
+
 # ------------------------------------------------------------ FP32 backward
 #
 # The FP32 model rounds its parameters (C, exponents, positions) to FP32; the

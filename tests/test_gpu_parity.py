@@ -9,7 +9,7 @@ import os
 import numpy
 import pytest
 
-from conftest import rel_err
+from conftest import long_list_case, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -533,20 +533,8 @@ def test_long_live_lists(ok):
   bf16x6 contraction take their long-list instances (entries past 128 read
   from memory).  70 MOs (random coefficients: the density needs no
   orthonormality); FP64 against the oracle, bf16x6 against FP64.'''
-  core, grid, O, W = ok['core'], ok['grid'], ok['O'], ok['W']
-  from orbkit_amd.orbitals import MOClass
-  from orbkit_amd.qcinfo import QCinfo
-  rng = numpy.random.default_rng(17)
-  geo = W._atoms(rng, 300, 15.0)
-  ao = W._basis(rng, geo, [['s', 's', 'p', 'p', 'd', 'f']] * 300, 0.05, 1e5, True)
-  assert ao.get_ao_num() == 6000
-  C = 0.05 * rng.standard_normal((70, 6000))
-  qc = QCinfo()
-  qc.geo_spec = geo
-  qc.geo_info = numpy.array([['X', str(i + 1), '0'] for i in range(len(geo))])
-  qc.ao_spec = ao
-  qc.mo_spec = MOClass([{'coeffs': C[i], 'occ_num': 2.0, 'energy': -1.0, 'sym': '%d.1' % (i + 1)} for i in range(70)])
-  axes = W._grid(geo, 6)
+  core, grid, O = ok['core'], ok['grid'], ok['O']
+  qc, axes = long_list_case()
   print('case long live lists', flush=True)
   # the first brick's list runs past 128 chunks, with the FP64 radii and with
   # the smaller ones of the FP32-ending paths (okmi_fused_live_rows counts
